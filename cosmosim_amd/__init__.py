@@ -1,0 +1,335 @@
+"""cosmosim_amd - MI355X-native gravity core for cosmosim (octree build,
+Barnes-Hut walk, KDK leapfrog) behind the reference's C entry points.
+
+The product is the shared library ``cosmosim_amd/libcosmosim.so`` (HIP kernels
+for gfx950 + a C host layer).  This module is a thin ctypes mirror of its C-ABI
+with the reference's names and argument meanings, so tests read like the
+reference's own (tests/test_physics.c).  There is no Python or CPU fallback:
+every call goes through the library, and a missing library or device raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from ._abi import (BODY_DTYPE, NODE_DTYPE, CosmosimConfig, ExtParams, ExtStats, PRECISIONS,
+                   PREC_FP64_STRICT, PREC_FP64, PREC_FP32, IC_DISK, IC_MERGER, make_bodies,
+                   positions, velocities, accelerations, ptr)
+
+__all__ = [
+    "BODY_DTYPE", "NODE_DTYPE", "CosmosimConfig", "ExtParams", "ExtStats", "PRECISIONS",
+    "PREC_FP64_STRICT", "PREC_FP64", "PREC_FP32", "IC_DISK", "IC_MERGER", "make_bodies",
+    "positions", "velocities", "accelerations", "lib", "library_path", "CosmosimError",
+    "octree_build", "octree_compute_forces", "octree_find_neighbors", "integrator_step",
+    "integrator_init_accelerations", "generate", "default_orbit", "orbit_coefficients",
+    "Simulation", "Handle", "device_info", "EXPORTED_SYMBOLS",
+]
+
+library_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcosmosim.so")
+
+_VP = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+
+# name -> (restype, argtypes) of every function declared in include/*.h
+_SIGS = {
+    # cosmosim_api.h
+    "cosmosim_default_config": (CosmosimConfig, []),
+    "cosmosim_create": (_VP, [CosmosimConfig]),
+    "cosmosim_step": (None, [_VP]),
+    "cosmosim_get_bodies": (_VP, [_VP]),
+    "cosmosim_get_count": (_I, [_VP]),
+    "cosmosim_get_active_count": (_I, [_VP]),
+    "cosmosim_get_sim_time": (_D, [_VP]),
+    "cosmosim_destroy": (None, [_VP]),
+    # octree.h / integrator.h
+    "octree_build": (None, [_VP, _VP, _VP, _I]),
+    "octree_compute_forces": (None, [_VP, _I, _VP, _I, _D, _D, _D]),
+    "octree_find_neighbors": (_I, [_VP, _I, _VP, _I, _D, _D, _D, _D, _VP, _I]),
+    "integrator_step": (None, [_VP, _I, _D, _D, _D, _D, _VP, _I]),
+    "integrator_init_accelerations": (None, [_VP, _I, _D, _D, _D, _VP]),
+    # cosmosim_ext.h
+    "cosmosim_ext_generate": (_I, [_I, _I, ctypes.c_ulonglong, _VP, _VP]),
+    "cosmosim_ext_default_orbit": (_I, [_VP]),
+    "cosmosim_ext_orbit_coefficients": (_I, [_I, _I, _VP]),
+    "cosmosim_ext_create": (_I, [_VP, _I, ctypes.POINTER(ExtParams), ctypes.POINTER(_VP)]),
+    "cosmosim_ext_set_params": (_I, [_VP, ctypes.POINTER(ExtParams)]),
+    "cosmosim_ext_get_params": (_I, [_VP, ctypes.POINTER(ExtParams)]),
+    "cosmosim_ext_step": (_I, [_VP, _I]),
+    "cosmosim_ext_sync": (_I, [_VP]),
+    "cosmosim_ext_upload": (_I, [_VP]),
+    "cosmosim_ext_compute_forces": (_I, [_VP]),
+    "cosmosim_ext_get_state": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "cosmosim_ext_set_state": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "cosmosim_ext_direct_forces": (_I, [_VP, _VP, _I, _VP]),
+    "cosmosim_ext_energy": (_I, [_VP, _VP, _VP, _VP]),
+    "cosmosim_ext_interaction_counts": (_I, [_VP, _VP, _I]),
+    "cosmosim_ext_get_stats": (_I, [_VP, ctypes.POINTER(ExtStats)]),
+    "cosmosim_ext_set_timing": (_I, [_VP, _I]),
+    "cosmosim_ext_export_tree": (_I, [_VP, _VP, _I, _VP]),
+    "cosmosim_ext_handle_error": (_I, [_VP]),
+    "cosmosim_ext_last_error": (_I, []),
+    "cosmosim_ext_clear_error": (None, []),
+    "cosmosim_ext_error_string": (ctypes.c_char_p, [_I]),
+    "cosmosim_ext_device_info": (_I, [_I, ctypes.c_char_p, _I, _VP]),
+    # csg.h (device-core C-ABI)
+    "csg_device_count": (_I, [_VP]),
+    "csg_device_info": (_I, [_I, ctypes.c_char_p, _I, _VP]),
+    "csg_create": (_I, [_I, _I, _VP]),
+    "csg_destroy": (None, [_VP]),
+    "csg_reserve": (_I, [_VP, _I]),
+    "csg_size": (_I, [_VP]),
+    "csg_set_bodies": (_I, [_VP, _VP, _I]),
+    "csg_get_bodies": (_I, [_VP, _VP, _I]),
+    "csg_set_accelerations": (_I, [_VP, _VP, _I]),
+    "csg_get_state": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "csg_set_state": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "csg_kick_drift": (_I, [_VP, _D]),
+    "csg_kick": (_I, [_VP, _D]),
+    "csg_build": (_I, [_VP]),
+    "csg_walk": (_I, [_VP, _I, _D, _D, _D]),
+    "csg_substeps": (_I, [_VP, _I, _D, _D, _D, _D, _I]),
+    "csg_export_pool": (_I, [_VP, _VP, _I, _VP]),
+    "csg_walk_pool": (_I, [_VP, _VP, _I, _I, _D, _D, _D]),
+    "csg_direct_forces": (_I, [_VP, _VP, _I, _D, _D, _VP]),
+    "csg_energy": (_I, [_VP, _D, _D, _VP, _VP, _VP]),
+    "csg_interaction_counts": (_I, [_VP, _VP, _I]),
+    "csg_get_stats": (_I, [_VP, _VP]),
+    "csg_set_timing": (_I, [_VP, _I]),
+    "csg_synchronize": (_I, [_VP]),
+    "csg_host_alloc": (_VP, [ctypes.c_size_t]),
+    "csg_host_free": (None, [_VP]),
+}
+EXPORTED_SYMBOLS = tuple(_SIGS)
+
+_lib = None
+
+
+class CosmosimError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        msg = lib().cosmosim_ext_error_string(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{where}: {msg} (code {code})")
+
+
+def lib():
+    """The loaded libcosmosim.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise ImportError(
+                f"{library_path} is missing: build it with `python -m cosmosim_amd.build`")
+        L = ctypes.CDLL(library_path)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code, where):
+    if code != 0:
+        raise CosmosimError(code, where)
+
+
+def _legacy_check(where):
+    code = lib().cosmosim_ext_last_error()
+    if code != 0:
+        lib().cosmosim_ext_clear_error()
+        raise CosmosimError(code, where)
+
+
+def device_info(device=0):
+    name = ctypes.create_string_buffer(256)
+    cu = ctypes.c_int(0)
+    _check(lib().cosmosim_ext_device_info(device, name, 256, ctypes.byref(cu)), "device_info")
+    return name.value.decode(), cu.value
+
+
+# ------------------------------------------------------------ legacy (src/octree.h)
+def octree_build(bodies):
+    """Tree of the live bodies in reference layout (root at 0); returns the used pool."""
+    n = len(bodies)
+    pool = np.zeros(max(8 * max(n, 1), 16), dtype=NODE_DTYPE)
+    used = ctypes.c_int(0)
+    lib().octree_build(ptr(pool), ctypes.byref(used), ptr(bodies), n)
+    _legacy_check("octree_build")
+    return pool[: used.value].copy(), pool
+
+
+def octree_compute_forces(pool, bodies, G, softening, theta, root=0):
+    lib().octree_compute_forces(ptr(pool), root, ptr(bodies), len(bodies), G, softening, theta)
+    _legacy_check("octree_compute_forces")
+
+
+def octree_find_neighbors(pool, bodies, center, radius, capacity, root=0):
+    buf = np.zeros(max(capacity, 1), dtype=np.int32)
+    k = lib().octree_find_neighbors(ptr(pool), root, ptr(bodies), len(bodies), center[0],
+                                    center[1], center[2], radius, ptr(buf), capacity)
+    return buf[:k].copy()
+
+
+def integrator_init_accelerations(bodies, G, softening, theta, pool=None):
+    if pool is None:
+        pool = np.zeros(max(8 * len(bodies), 16), dtype=NODE_DTYPE)
+    lib().integrator_init_accelerations(ptr(bodies), len(bodies), G, softening, theta, ptr(pool))
+    _legacy_check("integrator_init_accelerations")
+    return pool
+
+
+def integrator_step(bodies, dt, G, softening, theta, pool=None, sph_enabled=0):
+    if pool is None:
+        pool = np.zeros(max(8 * len(bodies), 16), dtype=NODE_DTYPE)
+    lib().integrator_step(ptr(bodies), len(bodies), dt, G, softening, theta, ptr(pool),
+                          sph_enabled)
+    _legacy_check("integrator_step")
+    return pool
+
+
+# ------------------------------------------------------------ initial conditions
+def generate(recipe, n, seed, orbit=None):
+    """Seeded bodies: recipe IC_DISK or IC_MERGER, n bodies including SMBHs."""
+    out = np.zeros(n, dtype=BODY_DTYPE)
+    o = None if orbit is None else np.ascontiguousarray(orbit, dtype=np.float64)
+    _check(lib().cosmosim_ext_generate(recipe, n, seed, ptr(o), ptr(out)), "generate")
+    return out
+
+
+def default_orbit():
+    o = np.zeros(6)
+    _check(lib().cosmosim_ext_default_orbit(ptr(o)), "default_orbit")
+    return o
+
+
+def orbit_coefficients(n, recipe=IC_MERGER):
+    c = np.zeros(n)
+    _check(lib().cosmosim_ext_orbit_coefficients(recipe, n, ptr(c)), "orbit_coefficients")
+    return c
+
+
+# ------------------------------------------------------------ handles
+class Handle:
+    """The eight legacy exports of src/cosmosim_api.h on one handle."""
+
+    def __init__(self, config=None, **kw):
+        L = lib()
+        cfg = L.cosmosim_default_config() if config is None else config
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        self.config = cfg
+        self.h = L.cosmosim_create(cfg)
+        if not self.h:
+            raise CosmosimError(L.cosmosim_ext_last_error() or -3, "cosmosim_create")
+
+    def step(self):
+        lib().cosmosim_step(self.h)
+        _check(lib().cosmosim_ext_handle_error(self.h), "cosmosim_step")
+
+    def bodies(self):
+        n = self.count()
+        p = lib().cosmosim_get_bodies(self.h)
+        buf = (ctypes.c_char * (168 * n)).from_address(p)
+        return np.frombuffer(buf, dtype=BODY_DTYPE, count=n).copy()
+
+    def count(self):
+        return lib().cosmosim_get_count(self.h)
+
+    def active_count(self):
+        return lib().cosmosim_get_active_count(self.h)
+
+    def sim_time(self):
+        return lib().cosmosim_get_sim_time(self.h)
+
+    def close(self):
+        if self.h:
+            lib().cosmosim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Simulation(Handle):
+    """Extension handle: device-resident stepping of caller-supplied bodies."""
+
+    def __init__(self, bodies, G=1.0, softening=0.01, theta=0.5, dt=1e-3, precision="fp64",
+                 device=0):
+        L = lib()
+        self.params = ExtParams(G, softening, theta, dt, PRECISIONS.get(precision, precision),
+                                device)
+        self.n = len(bodies)
+        h = ctypes.c_void_p()
+        b = np.ascontiguousarray(bodies, dtype=BODY_DTYPE)
+        _check(L.cosmosim_ext_create(ptr(b), self.n, ctypes.byref(self.params), ctypes.byref(h)),
+               "cosmosim_ext_create")
+        self.h = h.value
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            if k == "precision":
+                v = PRECISIONS.get(v, v)
+            setattr(self.params, k, v)
+        _check(lib().cosmosim_ext_set_params(self.h, ctypes.byref(self.params)), "set_params")
+
+    def advance(self, substeps):
+        _check(lib().cosmosim_ext_step(self.h, substeps), "cosmosim_ext_step")
+
+    def compute_forces(self):
+        _check(lib().cosmosim_ext_compute_forces(self.h), "compute_forces")
+
+    def state(self):
+        pos, vel, acc = (np.zeros((self.n, 3)) for _ in range(3))
+        mass = np.zeros(self.n)
+        _check(lib().cosmosim_ext_get_state(self.h, ptr(pos), ptr(vel), ptr(acc), ptr(mass)),
+               "get_state")
+        return pos, vel, acc, mass
+
+    def set_state(self, pos=None, vel=None, acc=None, mass=None):
+        arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+               for a in (pos, vel, acc, mass)]
+        _check(lib().cosmosim_ext_set_state(self.h, *(ptr(a) for a in arr)), "set_state")
+
+    def accelerations(self):
+        return self.state()[2]
+
+    def direct_forces(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        acc = np.zeros((len(idx), 3))
+        _check(lib().cosmosim_ext_direct_forces(self.h, ptr(idx), len(idx), ptr(acc)),
+               "direct_forces")
+        return acc
+
+    def energy(self):
+        ke, pe = ctypes.c_double(), ctypes.c_double()
+        p = np.zeros(3)
+        _check(lib().cosmosim_ext_energy(self.h, ctypes.byref(ke), ctypes.byref(pe), ptr(p)),
+               "energy")
+        return ke.value, pe.value, p
+
+    def interaction_counts(self):
+        c = np.zeros(self.n, dtype=np.int32)
+        _check(lib().cosmosim_ext_interaction_counts(self.h, ptr(c), self.n), "counts")
+        return c
+
+    def stats(self):
+        s = ExtStats()
+        _check(lib().cosmosim_ext_get_stats(self.h, ctypes.byref(s)), "stats")
+        return s
+
+    def set_timing(self, on=True):
+        _check(lib().cosmosim_ext_set_timing(self.h, int(on)), "set_timing")
+
+    def export_tree(self):
+        cap = 8 * self.n + 4096
+        pool = np.zeros(cap, dtype=NODE_DTYPE)
+        used = ctypes.c_int(0)
+        _check(lib().cosmosim_ext_export_tree(self.h, ptr(pool), cap, ctypes.byref(used)),
+               "export_tree")
+        return pool[: used.value].copy()
+
+    def sync(self):
+        _check(lib().cosmosim_ext_sync(self.h), "sync")

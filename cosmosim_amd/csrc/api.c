@@ -1,0 +1,528 @@
+/*
+ * api.c - the public handle API (src/cosmosim_api.h:35-42) and the extension
+ * API (include/cosmosim_ext.h) over the HIP core.
+ *
+ * Handle state follows src/cosmosim_api.c:19-28 (bodies, pool, n_alloc,
+ * current_n, config, sim_time, compact counter) plus a device context.  Two
+ * stepping modes:
+ *   quasar == 0 : device-resident.  The substep loop (kick+drift, build, walk,
+ *                 kick) never leaves the GPU; the host Body[] is refreshed once
+ *                 at the end of cosmosim_step (src/cosmosim_api.c:115-144).
+ *   quasar == 1 : the host Body[] is the source of truth, because the caller's
+ *                 quasar_step (src/quasar.c:477-509) changes n, masses, positions
+ *                 and accelerations between substeps and sph.c needs the host
+ *                 pool; each substep goes through the legacy path.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "cosmosim_api.h"
+#include "host_common.h"
+
+#define API_G 1.0         /* src/cosmosim_api.c:10 */
+#define API_SOFTENING 1.5 /* src/cosmosim_api.c:11 */
+#define API_GALAXY_MASS 100.0
+#define API_DISK_RADIUS 15.0
+#define API_SEPARATION 40.0
+#define API_APPROACH_VEL 0.15
+
+/* ABI image of the caller's QuasarConfig (src/quasar.h:6-22); this library only
+ * passes it to the caller's quasar_* functions. */
+typedef struct {
+    double accretion_radius, swallow_radius, isco_radius, viscosity_alpha, jet_speed, jet_mass,
+        jet_lifetime, feedback_strength, eta_eff, eddington_k, accretion_smoothing;
+    int jet_cap, jet_count, max_bodies, jet_ring_count;
+} QuasarConfigABI;
+
+/* Initial conditions and quasar physics are the caller's code
+ * (src/initial_conditions.c, src/quasar.c): weak references. */
+extern void generate_spiral_galaxy(Body *, int, double, double, double, double, double, double)
+    __attribute__((weak));
+extern void generate_merger(Body *, int, double, double) __attribute__((weak));
+extern void generate_quasar_galaxy(Body *, int, double, double, double, double, double, double,
+                                   double) __attribute__((weak));
+extern void generate_quasar_merger(Body *, int, double, double, double) __attribute__((weak));
+extern void generate_merger_dust(Body *, int, int, double) __attribute__((weak));
+extern QuasarConfigABI quasar_default_config(void) __attribute__((weak));
+extern void quasar_step(Body *, int *, int, QuasarConfigABI *, double) __attribute__((weak));
+extern int quasar_compact(Body *, int) __attribute__((weak));
+
+typedef struct {
+    Body *bodies;     /* pinned host array of n_alloc records */
+    OctreeNode *pool; /* host pool, quasar mode only */
+    int n_alloc;
+    int current_n;
+    CosmosimConfig config;
+    QuasarConfigABI qcfg;
+    double sim_time;
+    int compact_counter;
+    csg_ctx *ctx;
+    CosmosimExtParams params;
+    int host_stale; /* device state newer than bodies[] */
+    int error;
+} SimState;
+
+static void state_free(SimState *s)
+{
+    if (!s)
+        return;
+    csg_destroy(s->ctx);
+    csg_host_free(s->bodies);
+    free(s->pool);
+    free(s);
+}
+
+/* Seeded stand-in used only when the caller's generators are not linked: the
+ * library's own recipes, scaled to the reference API's galaxy mass 100 and disk
+ * radius 15 (mass x100, length x3, velocity x sqrt(100/3)). */
+static int fallback_ics(Body *bodies, int n, int merger)
+{
+    int st = cosmosim_ext_generate(merger && n >= 4 ? COSMOSIM_IC_MERGER : COSMOSIM_IC_DISK, n,
+                                   0x5eedull + (unsigned long long)n, NULL, bodies);
+    if (st)
+        return st;
+    const double L = API_DISK_RADIUS / 5.0, Mf = API_GALAXY_MASS, V = sqrt(Mf / L);
+    for (int i = 0; i < n; ++i) {
+        bodies[i].x *= L; bodies[i].y *= L; bodies[i].z *= L;
+        bodies[i].vx *= V; bodies[i].vy *= V; bodies[i].vz *= V;
+        bodies[i].mass *= Mf;
+    }
+    return COSMOSIM_OK;
+}
+
+COSMOSIM_API CosmosimConfig cosmosim_default_config(void)
+{
+    CosmosimConfig cfg;
+    memset(&cfg, 0, sizeof(cfg));
+    cfg.n_bodies = 20000;
+    cfg.merger = 0;
+    cfg.quasar = 0;
+    cfg.dt = 0.005;
+    cfg.theta = 0.5;
+    cfg.smbh_mass_frac = 0.05;
+    cfg.accretion_radius = 6.0;
+    cfg.jet_speed = 15.0;
+    cfg.feedback_strength = 0.3;
+    cfg.substeps = 2;
+    return cfg;
+}
+
+static void default_params(CosmosimExtParams *p, const CosmosimConfig *cfg)
+{
+    p->G = API_G;
+    p->softening = API_SOFTENING;
+    p->theta = cfg->theta;
+    p->dt = cfg->dt;
+    p->precision = COSMOSIM_PREC_FP64_STRICT;
+    p->device = cs_default_device();
+}
+
+COSMOSIM_API SimHandle cosmosim_create(CosmosimConfig config)
+{
+    int n = config.n_bodies;
+    if (n < 2)
+        n = 2;
+    const int n_dust = (config.merger && config.quasar) ? n / 5 : 0;
+    const int n_alloc = config.quasar ? n + n_dust + n / 4 : n;
+
+    int ndev = 0;
+    if (csg_device_count(&ndev) != COSMOSIM_OK || cs_default_device() >= ndev) {
+        cs_set_error(COSMOSIM_E_NO_DEVICE, "cosmosim_create");
+        return NULL;
+    }
+    SimState *s = (SimState *)calloc(1, sizeof(SimState));
+    if (!s)
+        return NULL;
+    s->bodies = (Body *)csg_host_alloc(sizeof(Body) * (size_t)n_alloc);
+    if (!s->bodies) {
+        state_free(s);
+        return NULL;
+    }
+    s->n_alloc = n_alloc;
+    s->config = config;
+    default_params(&s->params, &config);
+
+    if (config.quasar) {
+        if (!generate_quasar_merger || !generate_quasar_galaxy || !generate_merger_dust ||
+            !quasar_default_config || !quasar_step || !quasar_compact) {
+            cs_set_error(COSMOSIM_E_UNSUPPORTED, "cosmosim_create (quasar mode needs quasar.c)");
+            state_free(s);
+            return NULL;
+        }
+        s->pool = (OctreeNode *)malloc(8 * (size_t)n_alloc * sizeof(OctreeNode));
+        if (!s->pool) {
+            state_free(s);
+            return NULL;
+        }
+        if (config.merger) {
+            generate_quasar_merger(s->bodies, n, API_SEPARATION, API_APPROACH_VEL,
+                                   config.smbh_mass_frac);
+            if (n_dust > 0)
+                generate_merger_dust(s->bodies, n, n_dust, API_SEPARATION);
+        } else {
+            generate_quasar_galaxy(s->bodies, n, 0.0, 0.0, (double)n * 2.0, API_DISK_RADIUS, 0.0,
+                                   0.0, config.smbh_mass_frac);
+        }
+    } else if (config.merger && generate_merger) {
+        generate_merger(s->bodies, n, API_SEPARATION, API_APPROACH_VEL);
+    } else if (!config.merger && generate_spiral_galaxy) {
+        generate_spiral_galaxy(s->bodies, n, 0.0, 0.0, API_GALAXY_MASS, API_DISK_RADIUS, 0.0, 0.0);
+    } else if (fallback_ics(s->bodies, n, config.merger) != COSMOSIM_OK) {
+        state_free(s);
+        return NULL;
+    }
+
+    const int initial_n = n + n_dust;
+    if (csg_create(s->params.device, n_alloc, &s->ctx) != COSMOSIM_OK) {
+        cs_set_error(COSMOSIM_E_NO_DEVICE, "cosmosim_create");
+        state_free(s);
+        return NULL;
+    }
+    int st;
+    if (config.quasar) {
+        st = cs_legacy_init(s->ctx, s->bodies, initial_n, API_G, API_SOFTENING, config.theta,
+                            s->pool, 8 * n_alloc);
+        QuasarConfigABI q = quasar_default_config();
+        q.accretion_radius = config.accretion_radius;
+        q.jet_speed = config.jet_speed;
+        q.feedback_strength = config.feedback_strength;
+        q.jet_cap = n / 10;
+        q.max_bodies = n_alloc;
+        s->qcfg = q;
+    } else {
+        st = csg_set_bodies(s->ctx, s->bodies, initial_n);
+        if (!st)
+            st = csg_build(s->ctx);
+        if (!st)
+            st = csg_walk(s->ctx, s->params.precision, API_G, API_SOFTENING, config.theta);
+        if (!st)
+            st = csg_get_bodies(s->ctx, s->bodies, initial_n);
+    }
+    if (st != COSMOSIM_OK) {
+        cs_set_error(st, "cosmosim_create");
+        state_free(s);
+        return NULL;
+    }
+    s->current_n = initial_n;
+    s->sim_time = 0.0;
+    s->compact_counter = 0;
+    return (SimHandle)s;
+}
+
+static int sync_host(SimState *s)
+{
+    if (!s->host_stale)
+        return COSMOSIM_OK;
+    int st = csg_get_bodies(s->ctx, s->bodies, s->current_n);
+    if (st == COSMOSIM_OK)
+        s->host_stale = 0;
+    return st;
+}
+
+COSMOSIM_API void cosmosim_step(SimHandle handle)
+{
+    if (!handle)
+        return;
+    SimState *s = (SimState *)handle;
+    int st = COSMOSIM_OK;
+    if (s->config.quasar) {
+        for (int sub = 0; sub < s->config.substeps && st == COSMOSIM_OK; ++sub) {
+            st = cs_legacy_step(s->ctx, s->bodies, s->current_n, s->config.dt, API_G,
+                                API_SOFTENING, s->config.theta, s->pool, 8 * s->n_alloc, 1);
+            if (st == COSMOSIM_OK)
+                quasar_step(s->bodies, &s->current_n, s->n_alloc, &s->qcfg, s->config.dt);
+        }
+        s->compact_counter++;
+        if (s->compact_counter >= 60) {
+            s->current_n = quasar_compact(s->bodies, s->current_n);
+            s->compact_counter = 0;
+        }
+    } else {
+        st = csg_substeps(s->ctx, s->config.substeps, s->config.dt, s->params.G,
+                          s->params.softening, s->config.theta, s->params.precision);
+        if (st == COSMOSIM_OK) {
+            s->host_stale = 1;
+            st = sync_host(s);
+        }
+    }
+    s->error = st;
+    if (st != COSMOSIM_OK)
+        fprintf(stderr, "cosmosim: cosmosim_step failed: %s\n", cosmosim_ext_error_string(st));
+    s->sim_time += s->config.dt * s->config.substeps;
+}
+
+COSMOSIM_API const Body *cosmosim_get_bodies(SimHandle handle)
+{
+    if (!handle)
+        return NULL;
+    SimState *s = (SimState *)handle;
+    int st = sync_host(s);
+    if (st != COSMOSIM_OK)
+        s->error = st;
+    return s->bodies;
+}
+
+COSMOSIM_API int cosmosim_get_count(SimHandle handle)
+{
+    if (!handle)
+        return 0;
+    return ((SimState *)handle)->current_n;
+}
+
+COSMOSIM_API int cosmosim_get_active_count(SimHandle handle)
+{
+    if (!handle)
+        return 0;
+    const SimState *s = (const SimState *)handle;
+    int count = 0;
+    for (int i = 0; i < s->current_n; i++)
+        if (s->bodies[i].mass > 0.0)
+            count++;
+    return count;
+}
+
+COSMOSIM_API double cosmosim_get_sim_time(SimHandle handle)
+{
+    if (!handle)
+        return 0.0;
+    return ((SimState *)handle)->sim_time;
+}
+
+COSMOSIM_API void cosmosim_destroy(SimHandle handle)
+{
+    if (!handle)
+        return;
+    state_free((SimState *)handle);
+}
+
+/* =================================================================== ext */
+COSMOSIM_API const char *cosmosim_ext_error_string(int code)
+{
+    switch (code) {
+    case COSMOSIM_OK: return "ok";
+    case COSMOSIM_E_ARG: return "invalid argument";
+    case COSMOSIM_E_NO_DEVICE: return "no usable HIP device";
+    case COSMOSIM_E_ALLOC: return "allocation failed";
+    case COSMOSIM_E_HIP: return "HIP runtime error";
+    case COSMOSIM_E_CAPACITY: return "capacity exceeded (tree nodes or output buffer)";
+    case COSMOSIM_E_STATE: return "invalid state for this call";
+    case COSMOSIM_E_UNSUPPORTED: return "unsupported in this build";
+    default: return "unknown error";
+    }
+}
+
+static int valid_params(const CosmosimExtParams *p)
+{
+    return p && p->precision >= COSMOSIM_PREC_FP64_STRICT && p->precision <= COSMOSIM_PREC_FP32 &&
+           p->theta >= 0.0 && p->softening >= 0.0 && p->device >= 0;
+}
+
+COSMOSIM_API int cosmosim_ext_create(const Body *bodies, int n, const CosmosimExtParams *params,
+                                     SimHandle *out)
+{
+    if (!out || n < 0 || (n > 0 && !bodies) || !valid_params(params))
+        return COSMOSIM_E_ARG;
+    *out = NULL;
+    int ndev = 0;
+    if (csg_device_count(&ndev) != COSMOSIM_OK || params->device >= ndev)
+        return COSMOSIM_E_NO_DEVICE;
+    SimState *s = (SimState *)calloc(1, sizeof(SimState));
+    if (!s)
+        return COSMOSIM_E_ALLOC;
+    s->bodies = (Body *)csg_host_alloc(sizeof(Body) * (size_t)(n > 0 ? n : 1));
+    if (!s->bodies) {
+        state_free(s);
+        return COSMOSIM_E_ALLOC;
+    }
+    if (n > 0)
+        memcpy(s->bodies, bodies, sizeof(Body) * (size_t)n);
+    s->n_alloc = n;
+    s->current_n = n;
+    s->config = cosmosim_default_config();
+    s->config.n_bodies = n;
+    s->config.dt = params->dt;
+    s->config.theta = params->theta;
+    s->config.substeps = 1;
+    s->params = *params;
+    int st = csg_create(params->device, n, &s->ctx);
+    if (!st)
+        st = csg_set_bodies(s->ctx, s->bodies, n);
+    if (!st)
+        st = csg_build(s->ctx);
+    if (!st)
+        st = csg_walk(s->ctx, params->precision, params->G, params->softening, params->theta);
+    if (!st)
+        st = csg_get_bodies(s->ctx, s->bodies, n);
+    if (st) {
+        state_free(s);
+        return st;
+    }
+    *out = (SimHandle)s;
+    return COSMOSIM_OK;
+}
+
+COSMOSIM_API int cosmosim_ext_set_params(SimHandle h, const CosmosimExtParams *p)
+{
+    if (!h || !valid_params(p))
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    if (p->device != s->params.device)
+        return COSMOSIM_E_STATE;
+    s->params = *p;
+    s->config.dt = p->dt;
+    s->config.theta = p->theta;
+    return COSMOSIM_OK;
+}
+
+COSMOSIM_API int cosmosim_ext_get_params(SimHandle h, CosmosimExtParams *p)
+{
+    if (!h || !p)
+        return COSMOSIM_E_ARG;
+    *p = ((SimState *)h)->params;
+    return COSMOSIM_OK;
+}
+
+COSMOSIM_API int cosmosim_ext_step(SimHandle h, int substeps)
+{
+    if (!h || substeps < 0)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    if (s->config.quasar)
+        return COSMOSIM_E_STATE;
+    int st = csg_substeps(s->ctx, substeps, s->params.dt, s->params.G, s->params.softening,
+                          s->params.theta, s->params.precision);
+    if (st == COSMOSIM_OK) {
+        s->host_stale = 1;
+        s->sim_time += s->params.dt * substeps;
+    }
+    return st;
+}
+
+COSMOSIM_API int cosmosim_ext_sync(SimHandle h)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    if (s->config.quasar)
+        return COSMOSIM_OK;
+    s->host_stale = 1;
+    return sync_host(s);
+}
+
+COSMOSIM_API int cosmosim_ext_upload(SimHandle h)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    s->host_stale = 0;
+    return csg_set_bodies(s->ctx, s->bodies, s->current_n);
+}
+
+COSMOSIM_API int cosmosim_ext_compute_forces(SimHandle h)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    int st = csg_build(s->ctx);
+    if (!st)
+        st = csg_walk(s->ctx, s->params.precision, s->params.G, s->params.softening,
+                      s->params.theta);
+    if (!st)
+        s->host_stale = 1;
+    return st;
+}
+
+COSMOSIM_API int cosmosim_ext_get_state(SimHandle h, double *pos, double *vel, double *acc,
+                                        double *mass)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    return csg_get_state(((SimState *)h)->ctx, pos, vel, acc, mass);
+}
+
+COSMOSIM_API int cosmosim_ext_set_state(SimHandle h, const double *pos, const double *vel,
+                                        const double *acc, const double *mass)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    int st = csg_set_state(s->ctx, pos, vel, acc, mass);
+    if (!st)
+        s->host_stale = 1;
+    return st;
+}
+
+COSMOSIM_API int cosmosim_ext_direct_forces(SimHandle h, const int *idx, int k, double *acc)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    return csg_direct_forces(s->ctx, idx, k, s->params.G, s->params.softening, acc);
+}
+
+COSMOSIM_API int cosmosim_ext_energy(SimHandle h, double *kinetic, double *potential,
+                                     double momentum[3])
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    return csg_energy(s->ctx, s->params.G, s->params.softening, kinetic, potential, momentum);
+}
+
+COSMOSIM_API int cosmosim_ext_interaction_counts(SimHandle h, int *counts, int n)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    return csg_interaction_counts(((SimState *)h)->ctx, counts, n);
+}
+
+COSMOSIM_API int cosmosim_ext_get_stats(SimHandle h, CosmosimExtStats *out)
+{
+    if (!h || !out)
+        return COSMOSIM_E_ARG;
+    csg_stats cs;
+    int st = csg_get_stats(((SimState *)h)->ctx, &cs);
+    if (st)
+        return st;
+    memset(out, 0, sizeof(*out));
+    out->n = cs.n;
+    out->n_live = cs.n_live;
+    out->n_nodes = cs.n_nodes;
+    out->max_depth = cs.max_depth;
+    out->collision_runs = cs.collision_runs;
+    out->interactions = cs.interactions;
+    for (int a = 0; a < 3; ++a)
+        out->root_center[a] = cs.root[a];
+    out->root_half = cs.root[3];
+    out->ms_build = cs.ms_build;
+    out->ms_walk = cs.ms_walk;
+    out->ms_kdk = cs.ms_kdk;
+    return COSMOSIM_OK;
+}
+
+COSMOSIM_API int cosmosim_ext_set_timing(SimHandle h, int enabled)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    return csg_set_timing(((SimState *)h)->ctx, enabled);
+}
+
+COSMOSIM_API int cosmosim_ext_export_tree(SimHandle h, OctreeNode *pool, int capacity,
+                                          int *pool_size)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    return csg_export_pool(((SimState *)h)->ctx, pool, capacity, pool_size);
+}
+
+COSMOSIM_API int cosmosim_ext_handle_error(SimHandle h)
+{
+    return h ? ((SimState *)h)->error : COSMOSIM_E_ARG;
+}
+
+COSMOSIM_API int cosmosim_ext_device_info(int device, char *name, int len, int *compute_units)
+{
+    return csg_device_info(device, name, len, compute_units);
+}

@@ -1,0 +1,759 @@
+// csg_core.hip - context management and the csg_* C-ABI (include/csg.h) of the
+// HIP gravity core.  Every entry point makes the context's device current on the
+// calling thread, launches on the context's stream and returns 0 or a negative
+// COSMOSIM_E_* code.  Kernels live in csg_device.h.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "csg_device.h"
+#include "cosmosim_ext.h"
+#include "csg.h"
+
+struct csg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cap = 0;       // body capacity
+    int n = 0;         // bodies held
+    int node_cap = 0;  // node records (8 per body, as the reference pool)
+    // state, caller's body order
+    double *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr,
+           *vz = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr, *m = nullptr;
+    Body *aos = nullptr;  // last uploaded records (untouched fields travel back from here)
+    // build
+    BuildParams *bp = nullptr;
+    BuildParams *hbp = nullptr;  // pinned mirror
+    double *bbox_part = nullptr;
+    uint64_t *key = nullptr, *key2 = nullptr, *ext = nullptr, *ext_tmp = nullptr;
+    int *idx = nullptr, *idx2 = nullptr;
+    int *rs_hist = nullptr, *rs_tot = nullptr, *rs_base = nullptr;
+    int *runs = nullptr;
+    signed char *lcp = nullptr;
+    int *cnt = nullptr, *nbase = nullptr, *scan_part = nullptr;
+    DD *pref = nullptr, *dd_part = nullptr;
+    WNode *nodes = nullptr;
+    int *icount = nullptr;
+    unsigned long long *itotal = nullptr;
+    int tree_valid = 0;
+    // export / foreign pool / diagnostics
+    OctreeNode *pool_dev = nullptr;
+    size_t pool_dev_cap = 0;
+    int *err_dev = nullptr;
+    void *diag = nullptr;
+    size_t diag_cap = 0;
+    // timing
+    int timing = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double ms_build = 0, ms_walk = 0, ms_kdk = 0;
+};
+
+#define CSG_CHECK(expr)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            fprintf(stderr, "cosmosim: %s failed: %s\n", #expr, hipGetErrorString(e_));      \
+            return COSMOSIM_E_HIP;                                                           \
+        }                                                                                    \
+    } while (0)
+
+#define CSG_LAUNCHED()                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = hipGetLastError();                                                   \
+        if (e_ != hipSuccess) {                                                              \
+            fprintf(stderr, "cosmosim: kernel launch failed: %s\n", hipGetErrorString(e_));  \
+            return COSMOSIM_E_HIP;                                                           \
+        }                                                                                    \
+    } while (0)
+
+static inline int blocks_for(long long n, int per) { return (int)((n + per - 1) / per); }
+static inline int at_least1(int v) { return v < 1 ? 1 : v; }
+
+template <class T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <class T>
+static bool dalloc(T *&p, size_t count) {
+    p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void **)&p, count * sizeof(T)) == hipSuccess;
+}
+
+static void free_buffers(csg_ctx *c) {
+    dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->vx); dfree(c->vy); dfree(c->vz);
+    dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->m); dfree(c->aos);
+    dfree(c->bbox_part); dfree(c->key); dfree(c->key2); dfree(c->ext); dfree(c->ext_tmp);
+    dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_base);
+    dfree(c->runs); dfree(c->lcp); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
+    dfree(c->pref); dfree(c->dd_part); dfree(c->nodes); dfree(c->icount);
+    c->cap = 0;
+    c->node_cap = 0;
+}
+
+// Device buffers for `cap` bodies; about 900 bytes per body (the node records,
+// 8 per body at 64 B, dominate; 4096 extra records cover deep chains of
+// near-coincident bodies in small inputs).
+static int alloc_buffers(csg_ctx *c, int cap) {
+    const size_t C = (size_t)(cap < 1 ? 1 : cap);
+    const size_t C1 = C + 1;
+    const int nb = at_least1(blocks_for((long long)C1, CSG_TILE));
+    bool ok = dalloc(c->x, C) && dalloc(c->y, C) && dalloc(c->z, C) && dalloc(c->vx, C) &&
+              dalloc(c->vy, C) && dalloc(c->vz, C) && dalloc(c->ax, C) && dalloc(c->ay, C) &&
+              dalloc(c->az, C) && dalloc(c->m, C) && dalloc(c->aos, C) &&
+              dalloc(c->bbox_part, (size_t)CSG_BBOX_BLOCKS * 8) && dalloc(c->key, C) &&
+              dalloc(c->key2, C) && dalloc(c->ext, 3 * C) && dalloc(c->ext_tmp, 3 * C) &&
+              dalloc(c->idx, C) && dalloc(c->idx2, C) && dalloc(c->rs_hist, (size_t)256 * nb) &&
+              dalloc(c->rs_tot, 256) && dalloc(c->rs_base, 256) && dalloc(c->runs, C) &&
+              dalloc(c->lcp, C1) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
+              dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
+              dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->nodes, 8 * C + 4096) &&
+              dalloc(c->icount, C);
+    if (!ok) {
+        (void)hipGetLastError();
+        free_buffers(c);
+        return COSMOSIM_E_ALLOC;
+    }
+    c->cap = (int)C;
+    c->node_cap = (int)(8 * C + 4096);
+    return COSMOSIM_OK;
+}
+
+static int use(csg_ctx *c) {
+    if (!c) return COSMOSIM_E_ARG;
+    CSG_CHECK(hipSetDevice(c->device));
+    return COSMOSIM_OK;
+}
+
+#define CSG_USE(c)                          \
+    do {                                    \
+        int s_ = use(c);                    \
+        if (s_ != COSMOSIM_OK) return s_;   \
+    } while (0)
+
+static int ensure_diag(csg_ctx *c, size_t bytes) {
+    if (bytes <= c->diag_cap) return COSMOSIM_OK;
+    if (c->diag) (void)hipFree(c->diag);
+    c->diag = nullptr;
+    c->diag_cap = 0;
+    if (hipMalloc(&c->diag, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return COSMOSIM_E_ALLOC;
+    }
+    c->diag_cap = bytes;
+    return COSMOSIM_OK;
+}
+
+static void timer_start(csg_ctx *c) {
+    if (c->timing) (void)hipEventRecord(c->ev0, c->stream);
+}
+static double timer_stop(csg_ctx *c) {
+    if (!c->timing) return 0.0;
+    (void)hipEventRecord(c->ev1, c->stream);
+    (void)hipEventSynchronize(c->ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    return ms;
+}
+
+static int read_params(csg_ctx *c) {
+    CSG_CHECK(hipMemcpyAsync(c->hbp, c->bp, sizeof(BuildParams), hipMemcpyDeviceToHost, c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+extern "C" {
+
+int csg_device_count(int *count) {
+    if (!count) return COSMOSIM_E_ARG;
+    int k = 0;
+    if (hipGetDeviceCount(&k) != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        return COSMOSIM_E_NO_DEVICE;
+    }
+    *count = k;
+    return COSMOSIM_OK;
+}
+
+int csg_device_info(int device, char *name, int len, int *compute_units) {
+    int k = 0;
+    if (csg_device_count(&k) != COSMOSIM_OK || device < 0 || device >= k)
+        return COSMOSIM_E_NO_DEVICE;
+    hipDeviceProp_t p;
+    CSG_CHECK(hipGetDeviceProperties(&p, device));
+    if (name && len > 0) {
+        snprintf(name, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (compute_units) *compute_units = p.multiProcessorCount;
+    return COSMOSIM_OK;
+}
+
+int csg_create(int device, int capacity, csg_ctx **out) {
+    if (!out || capacity < 0) return COSMOSIM_E_ARG;
+    *out = nullptr;
+    int k = 0;
+    if (csg_device_count(&k) != COSMOSIM_OK || k == 0 || device < 0 || device >= k)
+        return COSMOSIM_E_NO_DEVICE;
+    csg_ctx *c = new (std::nothrow) csg_ctx();
+    if (!c) return COSMOSIM_E_ALLOC;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return COSMOSIM_E_HIP;
+    }
+    if (hipMalloc((void **)&c->bp, sizeof(BuildParams)) != hipSuccess ||
+        hipHostMalloc((void **)&c->hbp, sizeof(BuildParams), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&c->itotal, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void **)&c->err_dev, sizeof(int)) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        (void)hipGetLastError();
+        csg_destroy(c);
+        return COSMOSIM_E_ALLOC;
+    }
+    memset(c->hbp, 0, sizeof(BuildParams));
+    int s = alloc_buffers(c, capacity);
+    if (s != COSMOSIM_OK) {
+        csg_destroy(c);
+        return s;
+    }
+    *out = c;
+    return COSMOSIM_OK;
+}
+
+void csg_destroy(csg_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_buffers(c);
+    dfree(c->bp);
+    dfree(c->itotal);
+    dfree(c->err_dev);
+    dfree(c->pool_dev);
+    if (c->diag) (void)hipFree(c->diag);
+    if (c->hbp) (void)hipHostFree(c->hbp);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    (void)hipGetLastError();
+    delete c;
+}
+
+// Grow to `capacity` bodies, keeping the body state.
+int csg_reserve(csg_ctx *c, int capacity) {
+    CSG_USE(c);
+    if (capacity < 0) return COSMOSIM_E_ARG;
+    if (capacity <= c->cap) return COSMOSIM_OK;
+    const int keep = c->n;
+    double *old[10] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->m};
+    Body *old_aos = c->aos;
+    // detach the state arrays so free_buffers leaves them alone
+    c->x = c->y = c->z = c->vx = c->vy = c->vz = c->ax = c->ay = c->az = c->m = nullptr;
+    c->aos = nullptr;
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    free_buffers(c);
+    int s = alloc_buffers(c, capacity);
+    if (s == COSMOSIM_OK && keep > 0) {
+        double *nw[10] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->m};
+        for (int a = 0; a < 10; ++a)
+            if (hipMemcpyAsync(nw[a], old[a], sizeof(double) * keep, hipMemcpyDeviceToDevice,
+                               c->stream) != hipSuccess)
+                s = COSMOSIM_E_HIP;
+        if (hipMemcpyAsync(c->aos, old_aos, sizeof(Body) * keep, hipMemcpyDeviceToDevice,
+                           c->stream) != hipSuccess)
+            s = COSMOSIM_E_HIP;
+        (void)hipStreamSynchronize(c->stream);
+    }
+    for (int a = 0; a < 10; ++a) dfree(old[a]);
+    dfree(old_aos);
+    c->tree_valid = 0;
+    if (s != COSMOSIM_OK) c->n = 0;
+    return s;
+}
+
+int csg_size(const csg_ctx *c) { return c ? c->n : 0; }
+
+int csg_set_bodies(csg_ctx *c, const Body *bodies, int n) {
+    CSG_USE(c);
+    if (n < 0 || (n > 0 && !bodies)) return COSMOSIM_E_ARG;
+    int s = csg_reserve(c, n);
+    if (s != COSMOSIM_OK) return s;
+    c->n = n;
+    c->tree_valid = 0;
+    if (n == 0) return COSMOSIM_OK;
+    CSG_CHECK(hipMemcpyAsync(c->aos, bodies, sizeof(Body) * (size_t)n, hipMemcpyHostToDevice,
+                             c->stream));
+    k_unpack<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, c->stream>>>(c->aos, n, c->x, c->y, c->z, c->vx,
+                                                               c->vy, c->vz, c->ax, c->ay, c->az,
+                                                               c->m);
+    CSG_LAUNCHED();
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+int csg_set_accelerations(csg_ctx *c, const Body *bodies, int n) {
+    CSG_USE(c);
+    if (n != c->n || (n > 0 && !bodies)) return COSMOSIM_E_ARG;
+    if (n == 0) return COSMOSIM_OK;
+    CSG_CHECK(hipMemcpyAsync(c->aos, bodies, sizeof(Body) * (size_t)n, hipMemcpyHostToDevice,
+                             c->stream));
+    k_unpack_acc<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, c->stream>>>(c->aos, n, c->ax, c->ay, c->az);
+    CSG_LAUNCHED();
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+int csg_get_bodies(csg_ctx *c, Body *bodies, int n) {
+    CSG_USE(c);
+    if (n < 0 || n > c->n || (n > 0 && !bodies)) return COSMOSIM_E_ARG;
+    if (n == 0) return COSMOSIM_OK;
+    k_pack<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, c->stream>>>(c->aos, n, c->x, c->y, c->z, c->vx,
+                                                             c->vy, c->vz, c->ax, c->ay, c->az, c->m);
+    CSG_LAUNCHED();
+    CSG_CHECK(hipMemcpyAsync(bodies, c->aos, sizeof(Body) * (size_t)n, hipMemcpyDeviceToHost,
+                             c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+__global__ void k_interleave3(const double *a, const double *b, const double *cc, int n,
+                              double *out) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n) return;
+    out[3 * (size_t)i] = a[i];
+    out[3 * (size_t)i + 1] = b[i];
+    out[3 * (size_t)i + 2] = cc[i];
+}
+__global__ void k_deinterleave3(const double *in, int n, double *a, double *b, double *cc) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n) return;
+    a[i] = in[3 * (size_t)i];
+    b[i] = in[3 * (size_t)i + 1];
+    cc[i] = in[3 * (size_t)i + 2];
+}
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+__global__ void k_set_i32(int *p, int v) { *p = v; }
+
+static int get3(csg_ctx *c, const double *a, const double *b, const double *cc, double *host) {
+    const int n = c->n;
+    k_interleave3<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, c->stream>>>(a, b, cc, n, (double *)c->diag);
+    CSG_LAUNCHED();
+    CSG_CHECK(hipMemcpyAsync(host, c->diag, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost,
+                             c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+static int set3(csg_ctx *c, double *a, double *b, double *cc, const double *host) {
+    const int n = c->n;
+    CSG_CHECK(hipMemcpyAsync(c->diag, host, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice,
+                             c->stream));
+    k_deinterleave3<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, c->stream>>>((const double *)c->diag, n,
+                                                                      a, b, cc);
+    CSG_LAUNCHED();
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+int csg_get_state(csg_ctx *c, double *pos, double *vel, double *acc, double *mass) {
+    CSG_USE(c);
+    if (c->n == 0) return COSMOSIM_OK;
+    int s = ensure_diag(c, sizeof(double) * 3 * (size_t)c->n);
+    if (s) return s;
+    if (pos && (s = get3(c, c->x, c->y, c->z, pos))) return s;
+    if (vel && (s = get3(c, c->vx, c->vy, c->vz, vel))) return s;
+    if (acc && (s = get3(c, c->ax, c->ay, c->az, acc))) return s;
+    if (mass) {
+        CSG_CHECK(hipMemcpyAsync(mass, c->m, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+        CSG_CHECK(hipStreamSynchronize(c->stream));
+    }
+    return COSMOSIM_OK;
+}
+
+int csg_set_state(csg_ctx *c, const double *pos, const double *vel, const double *acc,
+                  const double *mass) {
+    CSG_USE(c);
+    if (c->n == 0) return COSMOSIM_OK;
+    int s = ensure_diag(c, sizeof(double) * 3 * (size_t)c->n);
+    if (s) return s;
+    if (pos && (s = set3(c, c->x, c->y, c->z, pos))) return s;
+    if (vel && (s = set3(c, c->vx, c->vy, c->vz, vel))) return s;
+    if (acc && (s = set3(c, c->ax, c->ay, c->az, acc))) return s;
+    if (mass) {
+        CSG_CHECK(hipMemcpyAsync(c->m, mass, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+        CSG_CHECK(hipStreamSynchronize(c->stream));
+    }
+    if (pos || mass) c->tree_valid = 0;
+    return COSMOSIM_OK;
+}
+
+static int launch_kick_drift(csg_ctx *c, double dt, int kicks) {
+    if (c->n == 0) return COSMOSIM_OK;
+    k_kick_drift<<<blocks_for(c->n, CSG_TPB), CSG_TPB, 0, c->stream>>>(
+        c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->m, c->n, dt, kicks);
+    CSG_LAUNCHED();
+    return COSMOSIM_OK;
+}
+
+static int launch_kick(csg_ctx *c, double dt) {
+    if (c->n == 0) return COSMOSIM_OK;
+    k_kick<<<blocks_for(c->n, CSG_TPB), CSG_TPB, 0, c->stream>>>(c->vx, c->vy, c->vz, c->ax, c->ay,
+                                                                c->az, c->m, c->n, dt);
+    CSG_LAUNCHED();
+    return COSMOSIM_OK;
+}
+
+// Whole build on the stream; no host synchronisation.
+static int launch_build(csg_ctx *c) {
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    const int nbb = n > 0 ? (blocks_for(n, CSG_TPB) < CSG_BBOX_BLOCKS ? blocks_for(n, CSG_TPB)
+                                                                       : CSG_BBOX_BLOCKS)
+                          : 1;
+    k_bbox_partial<<<nbb, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bbox_part);
+    k_bbox_final<<<1, CSG_TPB, 0, st>>>(c->bbox_part, nbb, c->bp, c->node_cap);
+    CSG_LAUNCHED();
+    if (n > 0) {
+        k_keys<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bp, c->key,
+                                                          c->idx);
+        const int nb = blocks_for(n, CSG_TILE);
+        uint64_t *kin = c->key, *kout = c->key2;
+        int *vin = c->idx, *vout = c->idx2;
+        for (int pass = 0; pass < 8; ++pass) {
+            const int shift = 8 * pass;
+            k_rs_hist<<<nb, CSG_TPB, 0, st>>>(kin, n, shift, c->rs_hist, nb);
+            k_rs_rowscan<<<256, CSG_TPB, 0, st>>>(c->rs_hist, nb, c->rs_tot);
+            k_rs_binscan<<<1, CSG_TPB, 0, st>>>(c->rs_tot, c->rs_base);
+            k_rs_scatter<<<nb, CSG_TPB, 0, st>>>(kin, vin, kout, vout, n, shift, c->rs_hist,
+                                                 c->rs_base, nb);
+            uint64_t *tk = kin; kin = kout; kout = tk;
+            int *tv = vin; vin = vout; vout = tv;
+        }
+        CSG_LAUNCHED();
+        // 8 passes: the sorted pairs are back in key / idx
+        k_runs_mark<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->key, c->bp, c->runs);
+        k_runs_fix<<<256, CSG_TPB, 0, st>>>(c->key, c->idx, c->x, c->y, c->z, c->bp, c->runs,
+                                            c->ext, c->idx2, c->ext_tmp);
+        CSG_LAUNCHED();
+    }
+    const int n1 = n + 1;
+    k_lcp<<<blocks_for(n1, CSG_TPB), CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt);
+    const int nt = blocks_for(n1, CSG_TILE);
+    k_iscan_reduce<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->bp, c->scan_part);
+    k_iscan_top<<<1, CSG_TPB, 0, st>>>(c->scan_part, nt);
+    k_iscan_down<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->scan_part, c->bp, c->nbase);
+    k_dd_reduce<<<nt, CSG_TPB, 0, st>>>(c->idx, c->x, c->y, c->z, c->m, c->bp, c->dd_part);
+    k_dd_top<<<1, CSG_TPB, 0, st>>>(c->dd_part, nt);
+    k_dd_down<<<nt, CSG_TPB, 0, st>>>(c->idx, c->x, c->y, c->z, c->m, c->bp, c->dd_part, c->pref,
+                                      c->cap + 1);
+    k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
+        c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->x, c->y, c->z, c->m,
+        c->bp, c->nodes);
+    CSG_LAUNCHED();
+    c->tree_valid = 1;
+    return COSMOSIM_OK;
+}
+
+static int launch_walk(csg_ctx *c, int precision, double G, double softening, double theta) {
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    const double eps2 = softening * softening, th2 = theta * theta;
+    k_set_u64<<<1, 1, 0, st>>>(c->itotal, 0ull);
+    if (n == 0) return COSMOSIM_OK;
+    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, c->icount);
+    int nb8 = blocks_for(n, CSG_TPB);
+    nb8 = (nb8 + 7) & ~7;
+    switch (precision) {
+    case COSMOSIM_PREC_FP64_STRICT:
+        k_walk<WALK_FP64_STRICT><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
+                                                         c->ax, c->ay, c->az, c->icount, c->itotal,
+                                                         G, eps2, th2, nb8);
+        break;
+    case COSMOSIM_PREC_FP64:
+        k_walk<WALK_FP64><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z, c->ax,
+                                                  c->ay, c->az, c->icount, c->itotal, G, eps2, th2,
+                                                  nb8);
+        break;
+    case COSMOSIM_PREC_FP32:
+        k_walk<WALK_FP32><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z, c->ax,
+                                                  c->ay, c->az, c->icount, c->itotal, G, eps2, th2,
+                                                  nb8);
+        break;
+    default:
+        return COSMOSIM_E_ARG;
+    }
+    CSG_LAUNCHED();
+    return COSMOSIM_OK;
+}
+
+int csg_kick_drift(csg_ctx *c, double dt) {
+    CSG_USE(c);
+    timer_start(c);
+    int s = launch_kick_drift(c, dt, 1);
+    if (s) return s;
+    c->ms_kdk = timer_stop(c);
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+int csg_kick(csg_ctx *c, double dt) {
+    CSG_USE(c);
+    timer_start(c);
+    int s = launch_kick(c, dt);
+    if (s) return s;
+    c->ms_kdk += timer_stop(c);
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+static int check_overflow(csg_ctx *c) {
+    int s = read_params(c);
+    if (s) return s;
+    if (c->hbp->overflow) {
+        c->tree_valid = 0;
+        return COSMOSIM_E_CAPACITY;
+    }
+    return COSMOSIM_OK;
+}
+
+int csg_build(csg_ctx *c) {
+    CSG_USE(c);
+    timer_start(c);
+    int s = launch_build(c);
+    if (s) return s;
+    c->ms_build = timer_stop(c);
+    return check_overflow(c);
+}
+
+int csg_walk(csg_ctx *c, int precision, double G, double softening, double theta) {
+    CSG_USE(c);
+    if (!c->tree_valid) return COSMOSIM_E_STATE;
+    if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
+    timer_start(c);
+    int s = launch_walk(c, precision, G, softening, theta);
+    if (s) return s;
+    c->ms_walk = timer_stop(c);
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, double theta,
+                 int precision) {
+    CSG_USE(c);
+    if (precision < 0 || precision > 2 || k < 0) return COSMOSIM_E_ARG;
+    if (k == 0) return COSMOSIM_OK;
+    int s;
+    c->ms_build = c->ms_walk = c->ms_kdk = 0.0;
+    timer_start(c);
+    if ((s = launch_kick_drift(c, dt, 1))) return s;
+    c->ms_kdk += timer_stop(c);
+    for (int sub = 0; sub < k; ++sub) {
+        timer_start(c);
+        if ((s = launch_build(c))) return s;
+        c->ms_build += timer_stop(c);
+        timer_start(c);
+        if ((s = launch_walk(c, precision, G, softening, theta))) return s;
+        c->ms_walk += timer_stop(c);
+        timer_start(c);
+        // end-of-substep half kick fused with the next substep's kick + drift
+        s = (sub + 1 < k) ? launch_kick_drift(c, dt, 2) : launch_kick(c, dt);
+        if (s) return s;
+        c->ms_kdk += timer_stop(c);
+    }
+    return check_overflow(c);
+}
+
+int csg_export_pool(csg_ctx *c, OctreeNode *pool, int capacity, int *pool_size) {
+    CSG_USE(c);
+    if (!pool || !pool_size) return COSMOSIM_E_ARG;
+    if (!c->tree_valid) return COSMOSIM_E_STATE;
+    int s = read_params(c);
+    if (s) return s;
+    const int nn = c->hbp->n_nodes;
+    if (c->hbp->overflow || nn > capacity) return COSMOSIM_E_CAPACITY;
+    if ((size_t)nn > c->pool_dev_cap) {
+        dfree(c->pool_dev);
+        c->pool_dev_cap = 0;
+        if (!dalloc(c->pool_dev, (size_t)nn)) {
+            (void)hipGetLastError();
+            return COSMOSIM_E_ALLOC;
+        }
+        c->pool_dev_cap = (size_t)nn;
+    }
+    const int g = blocks_for(nn, CSG_TPB);
+    k_export_nodes<<<g, CSG_TPB, 0, c->stream>>>(c->nodes, c->key, c->ext, c->bp, c->pool_dev);
+    k_export_link<<<g, CSG_TPB, 0, c->stream>>>(c->nodes, c->key, c->ext, c->lcp, c->nbase, c->bp,
+                                                c->pool_dev);
+    CSG_LAUNCHED();
+    CSG_CHECK(hipMemcpyAsync(pool, c->pool_dev, sizeof(OctreeNode) * (size_t)nn,
+                             hipMemcpyDeviceToHost, c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    *pool_size = nn;
+    return COSMOSIM_OK;
+}
+
+int csg_walk_pool(csg_ctx *c, const OctreeNode *pool, int pool_size, int root, double G,
+                  double softening, double theta) {
+    CSG_USE(c);
+    if (!pool || pool_size <= 0 || root < 0 || root >= pool_size) return COSMOSIM_E_ARG;
+    if ((size_t)pool_size > c->pool_dev_cap) {
+        dfree(c->pool_dev);
+        c->pool_dev_cap = 0;
+        if (!dalloc(c->pool_dev, (size_t)pool_size)) {
+            (void)hipGetLastError();
+            return COSMOSIM_E_ALLOC;
+        }
+        c->pool_dev_cap = (size_t)pool_size;
+    }
+    CSG_CHECK(hipMemcpyAsync(c->pool_dev, pool, sizeof(OctreeNode) * (size_t)pool_size,
+                             hipMemcpyHostToDevice, c->stream));
+    k_set_i32<<<1, 1, 0, c->stream>>>(c->err_dev, 0);
+    if (c->n > 0) {
+        k_walk_pool<<<blocks_for(c->n, CSG_TPB), CSG_TPB, 0, c->stream>>>(
+            c->pool_dev, pool_size, root, c->x, c->y, c->z, c->m, c->n, c->ax, c->ay, c->az,
+            c->icount, G, softening * softening, theta, c->err_dev);
+    }
+    CSG_LAUNCHED();
+    int err = 0;
+    CSG_CHECK(hipMemcpyAsync(&err, c->err_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    if (err & 2) return COSMOSIM_E_ARG;
+    if (err & 1) return COSMOSIM_E_CAPACITY;
+    return COSMOSIM_OK;
+}
+
+int csg_direct_forces(csg_ctx *c, const int *idx, int k, double G, double softening,
+                      double *acc) {
+    CSG_USE(c);
+    if (k < 0 || (k > 0 && (!idx || !acc))) return COSMOSIM_E_ARG;
+    if (k == 0 || c->n == 0) {
+        for (int t = 0; t < 3 * k; ++t) acc[t] = 0.0;
+        return COSMOSIM_OK;
+    }
+    for (int t = 0; t < k; ++t)
+        if (idx[t] < 0 || idx[t] >= c->n) return COSMOSIM_E_ARG;
+    const int n = c->n;
+    const int groups = blocks_for(k, 64);
+    int S = 1;
+    if (n > 4096) {
+        S = blocks_for(4096, groups);
+        if (S > 512) S = 512;
+        if (S < 1) S = 1;
+    }
+    int chunk = blocks_for(n, S);
+    S = blocks_for(n, chunk);
+    const size_t part_bytes = sizeof(double) * 3 * (size_t)k * S;
+    const size_t need = part_bytes + sizeof(double) * 3 * (size_t)k + sizeof(int) * (size_t)k + 256;
+    int s = ensure_diag(c, need);
+    if (s) return s;
+    double *part = (double *)c->diag;
+    double *out = part + 3 * (size_t)k * S;
+    int *didx = (int *)(out + 3 * (size_t)k);
+    CSG_CHECK(hipMemcpyAsync(didx, idx, sizeof(int) * k, hipMemcpyHostToDevice, c->stream));
+    k_direct<<<dim3(groups, S), 64, 0, c->stream>>>(didx, k, c->x, c->y, c->z, c->m, n, chunk, G,
+                                                    softening * softening, part);
+    k_direct_reduce<<<blocks_for(k, CSG_TPB), CSG_TPB, 0, c->stream>>>(part, k, S, out);
+    CSG_LAUNCHED();
+    CSG_CHECK(hipMemcpyAsync(acc, out, sizeof(double) * 3 * k, hipMemcpyDeviceToHost, c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+int csg_energy(csg_ctx *c, double G, double softening, double *kinetic, double *potential,
+               double momentum[3]) {
+    CSG_USE(c);
+    const int n = c->n;
+    double res[5] = {0, 0, 0, 0, 0};
+    if (n > 0) {
+        const int nbk = blocks_for(n, CSG_TPB);
+        int S = blocks_for(4096, nbk);
+        if (S < 1) S = 1;
+        if (S > 1024) S = 1024;
+        int chunk = blocks_for(n, S);
+        S = blocks_for(n, chunk);
+        const size_t pe_parts = (size_t)nbk * S;
+        const size_t need = sizeof(double) * (4 * (size_t)nbk + pe_parts + 8);
+        int s = ensure_diag(c, need);
+        if (s) return s;
+        double *kp = (double *)c->diag;
+        double *pp = kp + 4 * (size_t)nbk;
+        double *out = pp + pe_parts;
+        k_kinetic<<<nbk, CSG_TPB, 0, c->stream>>>(c->vx, c->vy, c->vz, c->m, n, kp);
+        k_potential<<<dim3(nbk, S), CSG_TPB, 0, c->stream>>>(c->x, c->y, c->z, c->m, n, chunk, G,
+                                                             softening * softening, pp);
+        for (int q = 0; q < 4; ++q)
+            k_sum<<<1, CSG_TPB, 0, c->stream>>>(kp, nbk, 4, q, out + q);
+        k_sum<<<1, CSG_TPB, 0, c->stream>>>(pp, (int)pe_parts, 1, 0, out + 4);
+        CSG_LAUNCHED();
+        CSG_CHECK(hipMemcpyAsync(res, out, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+        CSG_CHECK(hipStreamSynchronize(c->stream));
+    }
+    if (kinetic) *kinetic = res[0];
+    if (momentum) { momentum[0] = res[1]; momentum[1] = res[2]; momentum[2] = res[3]; }
+    if (potential) *potential = -0.5 * res[4];
+    return COSMOSIM_OK;
+}
+
+int csg_interaction_counts(csg_ctx *c, int *counts, int n) {
+    CSG_USE(c);
+    if (n != c->n || (n > 0 && !counts)) return COSMOSIM_E_ARG;
+    if (n == 0) return COSMOSIM_OK;
+    CSG_CHECK(hipMemcpyAsync(counts, c->icount, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+int csg_get_stats(csg_ctx *c, csg_stats *o) {
+    CSG_USE(c);
+    if (!o) return COSMOSIM_E_ARG;
+    int s = read_params(c);
+    if (s) return s;
+    unsigned long long it = 0;
+    CSG_CHECK(hipMemcpyAsync(&it, c->itotal, sizeof(it), hipMemcpyDeviceToHost, c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    memset(o, 0, sizeof(*o));
+    o->n = c->n;
+    o->n_live = c->hbp->n_live;
+    o->n_nodes = c->hbp->n_nodes;
+    o->max_depth = c->hbp->max_depth;
+    o->collision_runs = c->hbp->n_runs;
+    o->overflow = c->hbp->overflow;
+    o->interactions = (long long)it;
+    for (int a = 0; a < 4; ++a) o->root[a] = c->hbp->root[a];
+    o->ms_build = c->ms_build;
+    o->ms_walk = c->ms_walk;
+    o->ms_kdk = c->ms_kdk;
+    return COSMOSIM_OK;
+}
+
+int csg_set_timing(csg_ctx *c, int enabled) {
+    if (!c) return COSMOSIM_E_ARG;
+    c->timing = enabled ? 1 : 0;
+    return COSMOSIM_OK;
+}
+
+int csg_synchronize(csg_ctx *c) {
+    CSG_USE(c);
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+void *csg_host_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 1;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    memset(p, 0, bytes);
+    return p;
+}
+
+void csg_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
+}
+
+}  // extern "C"

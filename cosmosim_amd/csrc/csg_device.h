@@ -1,0 +1,1223 @@
+// csg_device.h - device records and HIP kernels of the gravity core (gfx950).
+// Included by csg_core.hip only (one translation unit holds every kernel).
+//
+// Pipeline per substep (all on one stream, no host synchronisation):
+//   bbox  : min/max over live bodies -> root cube            (src/octree.c:119-159)
+//   keys  : 21-level octant path per body from the FP64 cell recurrence
+//                                                            (src/octree.c:30-50, 87-108)
+//   sort  : LSD radix sort of (key, body) pairs, 8 passes of 8 bits
+//   runs  : bodies sharing all 21 levels get their levels 21..63 and are
+//           re-ranked inside the run                         (src/octree.c:71-79 depth limit)
+//   lcp   : common-prefix depth of sorted neighbours -> nodes per body
+//   scan  : node ids in depth-first pre-order
+//   dd    : double-double prefix sums of m, m*x, m*y, m*z over sorted bodies
+//   emit  : one 64-byte walk record per node (COM, mass, (2h)^2, skip, leaf body)
+//   walk  : one wavefront per 64 sorted bodies, stackless pre-order traversal
+//           with each lane applying its own opening test   (src/octree.c:171-218)
+//   kdk   : kick / drift                                    (src/integrator.c:22-31, 45-51)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "body.h"
+#include "octree.h"
+
+#define CSG_TPB 256
+#define CSG_ITEMS 8
+#define CSG_TILE (CSG_TPB * CSG_ITEMS)
+#define CSG_KEY_LEVELS 21
+#define CSG_DEPTH_LIMIT 64
+#define CSG_BBOX_BLOCKS 1024
+
+// Device-resident build state, written by the kernels themselves.
+struct BuildParams {
+    double root[4];  // centre x, y, z and half size H of the root cube
+    int n_live;      // bodies with !(mass <= 0)
+    int n_nodes;     // nodes of the tree (reference count: internal + leaves)
+    int n_runs;      // groups of sorted bodies with equal 21-level keys
+    int max_depth;   // deepest leaf
+    int overflow;    // 1 when n_nodes exceeds the node capacity
+    int node_cap;
+    int pad[2];
+    unsigned long long interactions;  // body-node interactions of the last walk
+};
+
+// One tree node in depth-first pre-order.  skip = index of the first node after
+// this node's subtree, so "descend" is i+1 and "accept / leaf" is skip.
+struct __align__(64) WNode {
+    double cx, cy, cz;  // centre of mass
+    double m;           // total mass
+    double s2;          // (2h)^2 with h the cell half size
+    int skip;
+    int body;           // leaf: original body index (first one for a depth-limit leaf); else -1
+    int depth;
+    int first;          // member range [first, last] in sorted order
+    int last;
+    int pad;
+};
+static_assert(sizeof(WNode) == 64, "WNode is one 64-byte record");
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ bool live_mass(double m) { return !(m <= 0.0); }
+
+// Octant code: bit0 x >= c, bit1 y < c, bit2 z < c.
+__device__ __forceinline__ int cell_octant(double px, double py, double pz, double cx, double cy,
+                                           double cz) {
+    return (px >= cx ? 1 : 0) | (py < cy ? 2 : 0) | (pz < cz ? 4 : 0);
+}
+
+// Child cell of octant q: centre moves by half the parent half size (the
+// product is exact), half size halves.
+__device__ __forceinline__ void cell_child(int q, double &cx, double &cy, double &cz, double &h) {
+    const double o = 0.5 * h;
+    cx = (q & 1) ? cx + o : cx - o;
+    cy = (q & 2) ? cy - o : cy + o;
+    cz = (q & 4) ? cz - o : cz + o;
+    h = h * 0.5;
+}
+
+// `levels` octant codes of point p starting from cell (c, h), 3 bits each,
+// first level in the most significant position; (c, h) advance.
+__device__ __forceinline__ uint64_t path_word(double px, double py, double pz, double &cx,
+                                              double &cy, double &cz, double &h, int levels) {
+    uint64_t k = 0;
+    for (int l = 0; l < levels; ++l) {
+        const int q = cell_octant(px, py, pz, cx, cy, cz);
+        k = (k << 3) | (uint64_t)q;
+        cell_child(q, cx, cy, cz, h);
+    }
+    return k;
+}
+
+// Levels 21..63 of a body path: ext[0] levels 21..41, ext[1] 42..62, ext[2] level 63.
+__device__ __forceinline__ void path_ext(double px, double py, double pz, const double *root,
+                                         uint64_t *e) {
+    double cx = root[0], cy = root[1], cz = root[2], h = root[3];
+    (void)path_word(px, py, pz, cx, cy, cz, h, 21);
+    e[0] = path_word(px, py, pz, cx, cy, cz, h, 21);
+    e[1] = path_word(px, py, pz, cx, cy, cz, h, 21);
+    e[2] = path_word(px, py, pz, cx, cy, cz, h, 1);
+}
+
+// Number of leading levels two sorted bodies share (0..64).  Bodies with equal
+// 21-level keys belong to one collision run and have their ext words.
+__device__ __forceinline__ int lcp_sorted(const uint64_t *__restrict__ key,
+                                          const uint64_t *__restrict__ ext, int a, int b) {
+    const uint64_t ka = key[a], kb = key[b];
+    if (ka != kb) return (__clzll((long long)(ka ^ kb)) - 1) / 3;
+    const uint64_t *ea = ext + 3 * (size_t)a, *eb = ext + 3 * (size_t)b;
+    if (ea[0] != eb[0]) return 21 + (__clzll((long long)(ea[0] ^ eb[0])) - 1) / 3;
+    if (ea[1] != eb[1]) return 42 + (__clzll((long long)(ea[1] ^ eb[1])) - 1) / 3;
+    if (ea[2] != eb[2]) return 63;
+    return 64;
+}
+
+// Octant digit of sorted body b at level l.
+__device__ __forceinline__ int digit_at(const uint64_t *key, const uint64_t *ext, int b, int l) {
+    if (l < 21) return (int)((key[b] >> (3 * (20 - l))) & 7);
+    const uint64_t *e = ext + 3 * (size_t)b;
+    if (l < 42) return (int)((e[0] >> (3 * (41 - l))) & 7);
+    if (l < 63) return (int)((e[1] >> (3 * (62 - l))) & 7);
+    return (int)(e[2] & 7);
+}
+
+// Last sorted body j >= b sharing at least d levels with b (galloping search).
+__device__ int range_end(const uint64_t *key, const uint64_t *ext, int b, int d, int n_live) {
+    int good = b, step = 1;
+    while (true) {
+        const int j = good + step;
+        if (j >= n_live || lcp_sorted(key, ext, b, j) < d) break;
+        good = j;
+        step <<= 1;
+    }
+    int bad = min(good + step, n_live);
+    while (bad - good > 1) {
+        const int mid = good + ((bad - good) >> 1);
+        if (lcp_sorted(key, ext, b, mid) >= d) good = mid; else bad = mid;
+    }
+    return good;
+}
+
+// First sorted body j <= b sharing at least d levels with b.
+__device__ int range_begin(const uint64_t *key, const uint64_t *ext, int b, int d) {
+    int good = b, step = 1;
+    while (true) {
+        const int j = good - step;
+        if (j < 0 || lcp_sorted(key, ext, j, b) < d) break;
+        good = j;
+        step <<= 1;
+    }
+    int bad = max(good - step, -1);
+    while (good - bad > 1) {
+        const int mid = bad + ((good - bad + 1) >> 1);
+        if (lcp_sorted(key, ext, mid, b) >= d) good = mid; else bad = mid;
+    }
+    return good;
+}
+
+// ---------------------------------------------------------- double-double
+struct DD { double h, l; };
+
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+    double s, e, t, f;
+    two_sum(a.h, b.h, s, e);
+    two_sum(a.l, b.l, t, f);
+    e += t;
+    double s2 = s + e;
+    double e2 = e - (s2 - s);
+    e2 += f;
+    DD r;
+    r.h = s2 + e2;
+    r.l = e2 - (r.h - s2);
+    return r;
+}
+__device__ __forceinline__ DD dd_neg(DD a) { return DD{-a.h, -a.l}; }
+__device__ __forceinline__ DD dd_prod(double a, double b) {
+    DD r;
+    r.h = a * b;
+    r.l = fma(a, b, -r.h);
+    return r;
+}
+
+// --------------------------------------------------------- block helpers
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one int per thread over a 256-thread block.
+__device__ __forceinline__ int block_excl_scan(int v, int *lds4, int &total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int inc = wave_incl_scan(v);
+    if (lane == 63) lds4[w] = inc;
+    __syncthreads();
+    int off = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int s = lds4[k];
+        if (k < w) off += s;
+        total += s;
+    }
+    __syncthreads();
+    return off + inc - v;
+}
+
+// =================================================================== bbox
+__global__ __launch_bounds__(CSG_TPB) void k_bbox_partial(const double *__restrict__ x,
+                                                          const double *__restrict__ y,
+                                                          const double *__restrict__ z,
+                                                          const double *__restrict__ m, int n,
+                                                          double *__restrict__ part) {
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double cnt = 0.0;
+    for (int i = blockIdx.x * CSG_TPB + threadIdx.x; i < n; i += gridDim.x * CSG_TPB) {
+        if (!live_mass(m[i])) continue;
+        cnt += 1.0;
+        const double p[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (p[a] < lo[a]) lo[a] = p[a];
+            if (p[a] > hi[a]) hi[a] = p[a];
+        }
+    }
+    __shared__ double s[7][CSG_TPB];
+    for (int a = 0; a < 3; ++a) { s[a][threadIdx.x] = lo[a]; s[3 + a][threadIdx.x] = hi[a]; }
+    s[6][threadIdx.x] = cnt;
+    __syncthreads();
+    for (int st = CSG_TPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const int o = threadIdx.x + st;
+            for (int a = 0; a < 3; ++a) {
+                if (s[a][o] < s[a][threadIdx.x]) s[a][threadIdx.x] = s[a][o];
+                if (s[3 + a][o] > s[3 + a][threadIdx.x]) s[3 + a][threadIdx.x] = s[3 + a][o];
+            }
+            s[6][threadIdx.x] += s[6][o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) part[blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Root cube exactly as src/octree.c:141-159; also resets the per-build counters.
+__global__ __launch_bounds__(CSG_TPB) void k_bbox_final(const double *__restrict__ part, int nblk,
+                                                        BuildParams *bp, int node_cap) {
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double cnt = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += CSG_TPB) {
+        for (int a = 0; a < 3; ++a) {
+            const double l = part[b * 8 + a], h = part[b * 8 + 3 + a];
+            if (l < lo[a]) lo[a] = l;
+            if (h > hi[a]) hi[a] = h;
+        }
+        cnt += part[b * 8 + 6];
+    }
+    __shared__ double s[7][CSG_TPB];
+    for (int a = 0; a < 3; ++a) { s[a][threadIdx.x] = lo[a]; s[3 + a][threadIdx.x] = hi[a]; }
+    s[6][threadIdx.x] = cnt;
+    __syncthreads();
+    for (int st = CSG_TPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const int o = threadIdx.x + st;
+            for (int a = 0; a < 3; ++a) {
+                if (s[a][o] < s[a][threadIdx.x]) s[a][threadIdx.x] = s[a][o];
+                if (s[3 + a][o] > s[3 + a][threadIdx.x]) s[3 + a][threadIdx.x] = s[3 + a][o];
+            }
+            s[6][threadIdx.x] += s[6][o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int live = (int)s[6][0];
+        bp->n_live = live;
+        bp->n_runs = 0;
+        bp->max_depth = 0;
+        bp->overflow = 0;
+        bp->node_cap = node_cap;
+        bp->interactions = 0ull;
+        bp->n_nodes = 1;
+        if (live == 0) {
+            bp->root[0] = 0.0; bp->root[1] = 0.0; bp->root[2] = 0.0; bp->root[3] = 1.0;
+        } else {
+            double ext = s[3][0] - s[0][0];
+            if (s[4][0] - s[1][0] > ext) ext = s[4][0] - s[1][0];
+            if (s[5][0] - s[2][0] > ext) ext = s[5][0] - s[2][0];
+            double half = ext * 0.5 * 1.01;
+            if (half < 1e-10) half = 1.0;
+            bp->root[0] = (s[0][0] + s[3][0]) * 0.5;
+            bp->root[1] = (s[1][0] + s[4][0]) * 0.5;
+            bp->root[2] = (s[2][0] + s[5][0]) * 0.5;
+            bp->root[3] = half;
+        }
+    }
+}
+
+// =================================================================== keys
+// Dead bodies get the all-ones key and sort behind every live body; their
+// acceleration is zeroed here (src/octree.c:227-231).
+__global__ __launch_bounds__(CSG_TPB) void k_keys(const double *__restrict__ x,
+                                                  const double *__restrict__ y,
+                                                  const double *__restrict__ z,
+                                                  const double *__restrict__ m, int n,
+                                                  const BuildParams *__restrict__ bp,
+                                                  uint64_t *__restrict__ key,
+                                                  int *__restrict__ idx) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = ~0ull;
+    if (live_mass(m[i])) {
+        double cx = bp->root[0], cy = bp->root[1], cz = bp->root[2], h = bp->root[3];
+        k = path_word(x[i], y[i], z[i], cx, cy, cz, h, CSG_KEY_LEVELS);
+    }
+    key[i] = k;
+    idx[i] = i;
+}
+
+// ============================================================ radix sort
+// Tile of CSG_TILE consecutive elements per block; wave w owns the sub-tile
+// [w*512, w*512+512) and walks it in 8 rounds of 64, so (wave, round, lane) is
+// the element order and the per-wave counters make the scatter stable.
+__global__ __launch_bounds__(CSG_TPB) void k_rs_hist(const uint64_t *__restrict__ key, int n,
+                                                     int shift, int *__restrict__ hist, int nb) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * CSG_TILE;
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        const int i = base + r * CSG_TPB + threadIdx.x;
+        if (i < n) atomicAdd(&h[(key[i] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// One block per digit: exclusive scan of the digit's per-tile counts.
+__global__ __launch_bounds__(CSG_TPB) void k_rs_rowscan(int *__restrict__ hist, int nb,
+                                                        int *__restrict__ tot) {
+    __shared__ int lds4[4];
+    int *row = hist + (size_t)blockIdx.x * nb;
+    int carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += CSG_TPB) {
+        const int i = c0 + threadIdx.x;
+        const int v = i < nb ? row[i] : 0;
+        int total;
+        const int ex = block_excl_scan(v, lds4, total);
+        if (i < nb) row[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_rs_binscan(const int *__restrict__ tot,
+                                                        int *__restrict__ binbase) {
+    __shared__ int lds4[4];
+    int total;
+    const int ex = block_excl_scan(tot[threadIdx.x], lds4, total);
+    binbase[threadIdx.x] = ex;
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_rs_scatter(
+    const uint64_t *__restrict__ kin, const int *__restrict__ vin, uint64_t *__restrict__ kout,
+    int *__restrict__ vout, int n, int shift, const int *__restrict__ hist,
+    const int *__restrict__ binbase, int nb) {
+    __shared__ int wcnt[4][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < 4 * 256; k += CSG_TPB) (&wcnt[0][0])[k] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * CSG_TILE + w * (64 * CSG_ITEMS);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint64_t kk[CSG_ITEMS];
+    int vv[CSG_ITEMS], dg[CSG_ITEMS], rk[CSG_ITEMS];
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        const int i = base + r * 64 + lane;
+        const bool valid = i < n;
+        kk[r] = valid ? kin[i] : 0ull;
+        vv[r] = valid ? vin[i] : 0;
+        const int d = (int)((kk[r] >> shift) & 255);
+        dg[r] = d;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool on = (d >> bit) & 1;
+            const uint64_t bal = __ballot(on);
+            peers &= on ? bal : ~bal;
+        }
+        const int below = __popcll(peers & lt);
+        const int old = valid ? wcnt[w][d] : 0;
+        if (valid && below == 0) wcnt[w][d] = old + __popcll(peers);
+        rk[r] = old + below;
+    }
+    __syncthreads();
+    // per-digit offsets of each wave inside the tile
+    __shared__ int woff[4][256];
+    {
+        const int d = threadIdx.x;
+        int s = 0;
+        for (int k = 0; k < 4; ++k) { woff[k][d] = s; s += wcnt[k][d]; }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        const int i = base + r * 64 + lane;
+        if (i < n) {
+            const int d = dg[r];
+            const int pos = binbase[d] + hist[d * nb + blockIdx.x] + woff[w][d] + rk[r];
+            kout[pos] = kk[r];
+            vout[pos] = vv[r];
+        }
+    }
+}
+
+// ======================================================= collision runs
+__global__ __launch_bounds__(CSG_TPB) void k_runs_mark(const uint64_t *__restrict__ key,
+                                                       BuildParams *bp, int *__restrict__ runs) {
+    const int b = blockIdx.x * CSG_TPB + threadIdx.x;
+    const int nl = bp->n_live;
+    if (b + 1 >= nl) return;
+    const uint64_t k = key[b];
+    if (k == key[b + 1] && (b == 0 || key[b - 1] != k)) {
+        const int r = atomicAdd(&bp->n_runs, 1);
+        runs[r] = b;
+    }
+}
+
+// One block per run (grid-stride): levels 21..63 for each member, then members
+// are re-ranked by (levels 21..63, body index); ties keep index order.
+__global__ __launch_bounds__(CSG_TPB) void k_runs_fix(
+    const uint64_t *__restrict__ key, int *__restrict__ idx, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const BuildParams *__restrict__ bp,
+    const int *__restrict__ runs, uint64_t *__restrict__ ext, int *__restrict__ tmp_idx,
+    uint64_t *__restrict__ tmp_ext) {
+    __shared__ int s_end;
+    const int nl = bp->n_live, nr = bp->n_runs;
+    double root[4] = {bp->root[0], bp->root[1], bp->root[2], bp->root[3]};
+    for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+        const int start = runs[r];
+        const uint64_t k0 = key[start];
+        if (threadIdx.x == 0) s_end = nl;
+        __syncthreads();
+        for (int c0 = start + 1;; c0 += CSG_TPB) {
+            const int j = c0 + threadIdx.x;
+            const bool stop = (j >= nl) || (j < nl && key[j] != k0);
+            if (stop) atomicMin(&s_end, j);
+            if (__syncthreads_or(stop)) break;
+        }
+        const int end = min(s_end, nl);
+        for (int p = start + threadIdx.x; p < end; p += CSG_TPB) {
+            const int i = idx[p];
+            path_ext(x[i], y[i], z[i], root, ext + 3 * (size_t)p);
+        }
+        __syncthreads();
+        int differ = 0;
+        for (int p = start + 1 + threadIdx.x; p < end; p += CSG_TPB) {
+            const uint64_t *e = ext + 3 * (size_t)p, *f = ext + 3 * (size_t)start;
+            differ |= (e[0] != f[0]) | (e[1] != f[1]) | (e[2] != f[2]);
+        }
+        if (__syncthreads_or(differ)) {
+            for (int p = start + threadIdx.x; p < end; p += CSG_TPB) {
+                const uint64_t *e = ext + 3 * (size_t)p;
+                const int ip = idx[p];
+                int rank = 0;
+                for (int q = start; q < end; ++q) {
+                    const uint64_t *f = ext + 3 * (size_t)q;
+                    bool less;
+                    if (f[0] != e[0]) less = f[0] < e[0];
+                    else if (f[1] != e[1]) less = f[1] < e[1];
+                    else if (f[2] != e[2]) less = f[2] < e[2];
+                    else less = idx[q] < ip;
+                    rank += less ? 1 : 0;
+                }
+                tmp_idx[start + rank] = ip;
+                tmp_ext[3 * (size_t)(start + rank) + 0] = e[0];
+                tmp_ext[3 * (size_t)(start + rank) + 1] = e[1];
+                tmp_ext[3 * (size_t)(start + rank) + 2] = e[2];
+            }
+            __syncthreads();
+            for (int p = start + threadIdx.x; p < end; p += CSG_TPB) {
+                idx[p] = tmp_idx[p];
+                ext[3 * (size_t)p + 0] = tmp_ext[3 * (size_t)p + 0];
+                ext[3 * (size_t)p + 1] = tmp_ext[3 * (size_t)p + 1];
+                ext[3 * (size_t)p + 2] = tmp_ext[3 * (size_t)p + 2];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ============================================================ lcp + count
+// L[b] = levels shared by sorted bodies b and b+1 (-1 past the end).  Body b
+// opens the internal nodes of depths (min(L[b-1],63), min(L[b],63)] and starts
+// a leaf unless it shares all 64 levels with b-1.
+__global__ __launch_bounds__(CSG_TPB) void k_lcp(const uint64_t *__restrict__ key,
+                                                 const uint64_t *__restrict__ ext, BuildParams *bp,
+                                                 signed char *__restrict__ L,
+                                                 int *__restrict__ cnt) {
+    const int b = blockIdx.x * CSG_TPB + threadIdx.x;
+    const int nl = bp->n_live;
+    int leafdepth = 0;
+    if (b < nl) {
+        const int lc = (b + 1 < nl) ? lcp_sorted(key, ext, b, b + 1) : -1;
+        const int lp = (b > 0) ? lcp_sorted(key, ext, b - 1, b) : -1;
+        L[b] = (signed char)lc;
+        const int opened = max(0, min(lc, 63) - min(lp, 63));
+        cnt[b] = opened + (lp < 64 ? 1 : 0);
+        leafdepth = min(max(lp, lc) + 1, 64);
+    } else if (b == nl) {
+        cnt[b] = 0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) leafdepth = max(leafdepth, __shfl_xor(leafdepth, o, 64));
+    if ((threadIdx.x & 63) == 0 && leafdepth > 0) atomicMax(&bp->max_depth, leafdepth);
+}
+
+// ============================================================ int scan
+// Exclusive scan of cnt[0..n_live] into base[]; n_nodes = base[n_live].
+__global__ __launch_bounds__(CSG_TPB) void k_iscan_reduce(const int *__restrict__ in,
+                                                          const BuildParams *__restrict__ bp,
+                                                          int *__restrict__ part) {
+    __shared__ int lds4[4];
+    const int len = bp->n_live + 1;
+    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) s += (base + r < len) ? in[base + r] : 0;
+    int total;
+    (void)block_excl_scan(s, lds4, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_iscan_top(int *__restrict__ part, int nb) {
+    __shared__ int lds4[4];
+    int carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += CSG_TPB) {
+        const int i = c0 + threadIdx.x;
+        const int v = i < nb ? part[i] : 0;
+        int total;
+        const int ex = block_excl_scan(v, lds4, total);
+        if (i < nb) part[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_iscan_down(const int *__restrict__ in,
+                                                        const int *__restrict__ part,
+                                                        BuildParams *bp, int *__restrict__ out) {
+    __shared__ int lds4[4];
+    const int nl = bp->n_live, len = nl + 1;
+    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
+    int v[CSG_ITEMS], s = 0;
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) { v[r] = (base + r < len) ? in[base + r] : 0; s += v[r]; }
+    int total;
+    int run = block_excl_scan(s, lds4, total) + part[blockIdx.x];
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        const int i = base + r;
+        if (i < len) {
+            out[i] = run;
+            if (i == nl) {
+                const int nn = (nl == 0) ? 1 : run;
+                bp->n_nodes = nn;
+                if (nn > bp->node_cap) bp->overflow = 1;
+            }
+        }
+        run += v[r];
+    }
+}
+
+// ====================================================== double-double scan
+// Exclusive prefix P[b] over sorted bodies of (m, m*x, m*y, m*z), each a DD;
+// P[n_live] is the total.  Layout: pref[c*(cap1) + b], c = 0..3.
+__device__ __forceinline__ void dd_elem(const int *__restrict__ idx, const double *__restrict__ x,
+                                        const double *__restrict__ y,
+                                        const double *__restrict__ z,
+                                        const double *__restrict__ m, int b, int nl, DD v[4]) {
+    if (b < nl) {
+        const int i = idx[b];
+        const double mm = m[i];
+        v[0] = DD{mm, 0.0};
+        v[1] = dd_prod(mm, x[i]);
+        v[2] = dd_prod(mm, y[i]);
+        v[3] = dd_prod(mm, z[i]);
+    } else {
+        for (int c = 0; c < 4; ++c) v[c] = DD{0.0, 0.0};
+    }
+}
+
+// Block-wide inclusive DD scan of one value per thread (Hillis-Steele, fixed order).
+__device__ void block_dd_incl_scan(DD v[4], DD (*lds)[CSG_TPB]) {
+    for (int o = 1; o < CSG_TPB; o <<= 1) {
+        for (int c = 0; c < 4; ++c) lds[c][threadIdx.x] = v[c];
+        __syncthreads();
+        if ((int)threadIdx.x >= o)
+            for (int c = 0; c < 4; ++c) v[c] = dd_add(lds[c][threadIdx.x - o], v[c]);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_dd_reduce(const int *__restrict__ idx,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ y,
+                                                       const double *__restrict__ z,
+                                                       const double *__restrict__ m,
+                                                       const BuildParams *__restrict__ bp,
+                                                       DD *__restrict__ part) {
+    __shared__ DD lds[4][CSG_TPB];
+    const int nl = bp->n_live;
+    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
+    DD s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        DD e[4];
+        dd_elem(idx, x, y, z, m, base + r, nl, e);
+        for (int c = 0; c < 4; ++c) s[c] = dd_add(s[c], e[c]);
+    }
+    block_dd_incl_scan(s, lds);
+    if (threadIdx.x == CSG_TPB - 1)
+        for (int c = 0; c < 4; ++c) part[blockIdx.x * 4 + c] = s[c];
+}
+
+// Exclusive scan of the per-block DD totals (one block, chunks of 256).
+__global__ __launch_bounds__(CSG_TPB) void k_dd_top(DD *__restrict__ part, int nb) {
+    __shared__ DD lds[4][CSG_TPB];
+    DD carry[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int c0 = 0; c0 < nb; c0 += CSG_TPB) {
+        const int i = c0 + threadIdx.x;
+        DD v[4], orig[4];
+        for (int c = 0; c < 4; ++c) v[c] = orig[c] = (i < nb) ? part[i * 4 + c] : DD{0, 0};
+        block_dd_incl_scan(v, lds);
+        // exclusive = carry + (inclusive - own) computed as carry + inclusive of previous thread
+        __shared__ DD last[4];
+        for (int c = 0; c < 4; ++c) lds[c][threadIdx.x] = v[c];
+        __syncthreads();
+        DD ex[4];
+        for (int c = 0; c < 4; ++c)
+            ex[c] = dd_add(carry[c], threadIdx.x > 0 ? lds[c][threadIdx.x - 1] : DD{0, 0});
+        if (threadIdx.x == CSG_TPB - 1)
+            for (int c = 0; c < 4; ++c) last[c] = v[c];
+        __syncthreads();
+        if (i < nb)
+            for (int c = 0; c < 4; ++c) part[i * 4 + c] = ex[c];
+        for (int c = 0; c < 4; ++c) carry[c] = dd_add(carry[c], last[c]);
+        __syncthreads();
+        (void)orig;
+    }
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_dd_down(const int *__restrict__ idx,
+                                                     const double *__restrict__ x,
+                                                     const double *__restrict__ y,
+                                                     const double *__restrict__ z,
+                                                     const double *__restrict__ m,
+                                                     const BuildParams *__restrict__ bp,
+                                                     const DD *__restrict__ part,
+                                                     DD *__restrict__ pref, int cap1) {
+    __shared__ DD lds[4][CSG_TPB];
+    const int nl = bp->n_live;
+    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
+    DD s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        DD e[4];
+        dd_elem(idx, x, y, z, m, base + r, nl, e);
+        for (int c = 0; c < 4; ++c) s[c] = dd_add(s[c], e[c]);
+    }
+    DD inc[4] = {s[0], s[1], s[2], s[3]};
+    block_dd_incl_scan(inc, lds);
+    for (int c = 0; c < 4; ++c) lds[c][threadIdx.x] = inc[c];
+    __syncthreads();
+    DD run[4];
+    for (int c = 0; c < 4; ++c)
+        run[c] = dd_add(part[blockIdx.x * 4 + c],
+                        threadIdx.x > 0 ? lds[c][threadIdx.x - 1] : DD{0, 0});
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        const int b = base + r;
+        if (b <= nl)
+            for (int c = 0; c < 4; ++c) pref[(size_t)c * cap1 + b] = run[c];
+        if (b < nl) {
+            DD e[4];
+            dd_elem(idx, x, y, z, m, b, nl, e);
+            for (int c = 0; c < 4; ++c) run[c] = dd_add(run[c], e[c]);
+        }
+    }
+}
+
+__device__ __forceinline__ double dd_range(const DD *__restrict__ pref, int cap1, int c, int b,
+                                           int e) {
+    const DD hi = pref[(size_t)c * cap1 + e + 1], lo = pref[(size_t)c * cap1 + b];
+    return dd_add(hi, dd_neg(lo)).h;
+}
+
+// ================================================================== emit
+__global__ __launch_bounds__(CSG_TPB) void k_emit(
+    const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext,
+    const int *__restrict__ idx, const signed char *__restrict__ L, const int *__restrict__ nbase,
+    const DD *__restrict__ pref, int cap1, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
+    const BuildParams *__restrict__ bp, WNode *__restrict__ nodes) {
+    const int b = blockIdx.x * CSG_TPB + threadIdx.x;
+    const int nl = bp->n_live;
+    if (bp->overflow) return;
+    const double H = bp->root[3];
+    if (nl == 0) {
+        if (b == 0) {
+            WNode r;
+            r.cx = r.cy = r.cz = 0.0;
+            r.m = 0.0;
+            const double s = 2.0 * H;
+            r.s2 = s * s;
+            r.skip = 1; r.body = -1; r.depth = 0; r.first = 0; r.last = -1; r.pad = 0;
+            nodes[0] = r;
+        }
+        return;
+    }
+    if (b >= nl) return;
+    const int lp = (b > 0) ? L[b - 1] : -1;
+    const int lc = L[b];
+    const int lo = min(lp, 63) + 1, hi = min(lc, 63);
+    int id = nbase[b];
+    for (int d = lo; d <= hi; ++d, ++id) {
+        const int e = range_end(key, ext, b, d, nl);
+        WNode w;
+        const double M = dd_range(pref, cap1, 0, b, e);
+        w.cx = dd_range(pref, cap1, 1, b, e) / M;
+        w.cy = dd_range(pref, cap1, 2, b, e) / M;
+        w.cz = dd_range(pref, cap1, 3, b, e) / M;
+        w.m = M;
+        const double s = 2.0 * ldexp(H, -d);
+        w.s2 = s * s;
+        w.skip = nbase[e + 1];
+        w.body = -1; w.depth = d; w.first = b; w.last = e; w.pad = 0;
+        nodes[id] = w;
+    }
+    if (lp < 64) {
+        WNode w;
+        const int i = idx[b];
+        int e = b;
+        int depth;
+        if (lc >= 64) {
+            depth = 64;
+            e = range_end(key, ext, b, 64, nl);
+            const double M = dd_range(pref, cap1, 0, b, e);
+            w.cx = dd_range(pref, cap1, 1, b, e) / M;
+            w.cy = dd_range(pref, cap1, 2, b, e) / M;
+            w.cz = dd_range(pref, cap1, 3, b, e) / M;
+            w.m = M;
+        } else {
+            depth = max(lp, lc) + 1;
+            w.cx = x[i]; w.cy = y[i]; w.cz = z[i];
+            w.m = m[i];
+        }
+        const double s = 2.0 * ldexp(H, -depth);
+        w.s2 = s * s;
+        w.skip = id + 1;
+        w.body = i; w.depth = depth; w.first = b; w.last = e; w.pad = 0;
+        nodes[id] = w;
+    }
+}
+
+// ================================================================== walk
+enum { WALK_FP64_STRICT = 0, WALK_FP64 = 1, WALK_FP32 = 2 };
+
+// Reference opening test fl(s2 / d2) < theta2 (src/octree.c:202-203) decided
+// from one fused multiply-add; the IEEE division runs only within 1e-14 of the
+// boundary, where the sign of s2 - theta2*d2 could disagree with it.
+__device__ __forceinline__ bool open_accept_exact(double s2, double d2, double theta2) {
+    if (!(d2 > 0.0)) return false;
+    const double r = fma(-theta2, d2, s2);
+    const double tol = 1e-14 * s2;
+    if (r < -tol) return true;
+    if (r > tol) return false;
+    return (s2 / d2) < theta2;
+}
+
+// Work-group -> tile mapping that keeps consecutive tiles on one XCD (work
+// groups are dealt round-robin over the 8 XCDs, each with its own L2).
+__device__ __forceinline__ int xcd_tile(int nb8) {
+    const int per = nb8 >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+template <int PREC>
+__global__ __launch_bounds__(CSG_TPB) void k_walk(
+    const WNode *__restrict__ nodes, const BuildParams *__restrict__ bp,
+    const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
+    double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
+    double G, double eps2, double theta2, int nb8) {
+    const int n_live = bp->n_live;
+    const int nnodes = bp->n_nodes;
+    if (bp->overflow) return;
+    const int lane = threadIdx.x & 63;
+    const int tile = xcd_tile(nb8);
+    const int b0 = (tile * CSG_TPB + (int)(threadIdx.x & ~63));
+    if (b0 >= n_live) return;
+    const int b = b0 + lane;
+    const bool act = b < n_live;
+    const int me = act ? sidx[b] : -1;
+    const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
+    int p = act ? 0 : 0x7fffffff;
+    int cnt = 0;
+    int i = 0;
+    if constexpr (PREC == WALK_FP32) {
+        // FP32 offsets from a per-wave anchor formed in FP64.
+        const double x0 = __shfl(x, 0, 64), y0 = __shfl(y, 0, 64), z0 = __shfl(z, 0, 64);
+        const float xr = (float)(x - x0), yr = (float)(y - y0), zr = (float)(z - z0);
+        const float eps2f = (float)eps2, th2f = (float)theta2;
+        float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+        while (i < nnodes) {
+            const WNode nd = nodes[i];
+            bool desc = false;
+            if (p == i) {
+                const float dx = (float)(nd.cx - x0) - xr;
+                const float dy = (float)(nd.cy - y0) - yr;
+                const float dz = (float)(nd.cz - z0) - zr;
+                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                bool inter;
+                if (nd.body >= 0) inter = nd.body != me;
+                else { inter = (d2 > 0.f) && ((float)nd.s2 < th2f * d2); desc = !inter; }
+                if (nd.m == 0.0) { inter = false; desc = false; }
+                if (inter) {
+                    const float r2 = d2 + eps2f;
+                    const float yv = __builtin_amdgcn_rsqf(r2);
+                    const float f = (float)(G * nd.m) * (yv * yv * yv);
+                    const float tx = dx * f - cx, ty = dy * f - cy, tz = dz * f - cz;
+                    const float ux = sx + tx, uy = sy + ty, uz = sz + tz;
+                    cx = (ux - sx) - tx; cy = (uy - sy) - ty; cz = (uz - sz) - tz;
+                    sx = ux; sy = uy; sz = uz;
+                    ++cnt;
+                }
+                p = desc ? i + 1 : nd.skip;
+            }
+            const int nxt = __any(desc) ? i + 1 : nd.skip;
+            i = nxt > i ? nxt : i + 1;  // strictly increasing: the loop always ends
+        }
+        if (act) {
+            ax[me] = (double)sx - (double)cx;
+            ay[me] = (double)sy - (double)cy;
+            az[me] = (double)sz - (double)cz;
+        }
+    } else {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        while (i < nnodes) {
+            const WNode nd = nodes[i];
+            bool desc = false;
+            if (p == i) {
+                const double dx = nd.cx - x, dy = nd.cy - y, dz = nd.cz - z;
+                double d2;
+                if constexpr (PREC == WALK_FP64_STRICT) d2 = dx * dx + dy * dy + dz * dz;
+                else d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                bool inter;
+                if (nd.body >= 0) inter = nd.body != me;
+                else { inter = open_accept_exact(nd.s2, d2, theta2); desc = !inter; }
+                if (nd.m == 0.0) { inter = false; desc = false; }
+                if (inter) {
+                    const double r2 = d2 + eps2;
+                    const double gm = G * nd.m;
+                    if constexpr (PREC == WALK_FP64_STRICT) {
+                        const double w = 1.0 / (r2 * sqrt(r2));
+                        sx += gm * dx * w;
+                        sy += gm * dy * w;
+                        sz += gm * dz * w;
+                    } else {
+                        double yv = __builtin_amdgcn_rsq(r2);
+                        double e = fma(-r2 * yv, yv, 1.0);
+                        yv = fma(0.5 * yv, e, yv);
+                        e = fma(-r2 * yv, yv, 1.0);
+                        yv = fma(0.5 * yv, e, yv);
+                        const double f = gm * (yv * yv * yv);
+                        sx = fma(dx, f, sx);
+                        sy = fma(dy, f, sy);
+                        sz = fma(dz, f, sz);
+                    }
+                    ++cnt;
+                }
+                p = desc ? i + 1 : nd.skip;
+            }
+            const int nxt = __any(desc) ? i + 1 : nd.skip;
+            i = nxt > i ? nxt : i + 1;  // strictly increasing: the loop always ends
+        }
+        if (act) { ax[me] = sx; ay[me] = sy; az[me] = sz; }
+    }
+    if (act && icount) icount[me] = cnt;
+    unsigned long long c = (unsigned long long)cnt;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) atomicAdd(itotal, c);
+}
+
+// Dead bodies get a = 0 (src/octree.c:227-231).
+__global__ __launch_bounds__(CSG_TPB) void k_zero_dead(const double *__restrict__ m, int n,
+                                                       double *__restrict__ ax,
+                                                       double *__restrict__ ay,
+                                                       double *__restrict__ az,
+                                                       int *__restrict__ icount) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n) return;
+    if (!live_mass(m[i])) {
+        ax[i] = 0.0; ay[i] = 0.0; az[i] = 0.0;
+        if (icount) icount[i] = 0;
+    }
+}
+
+// =================================================================== KDK
+// v += (0.5*a)*dt; x += v*dt for live bodies (src/integrator.c:22-31).
+__global__ __launch_bounds__(CSG_TPB) void k_kick_drift(double *__restrict__ x,
+                                                        double *__restrict__ y,
+                                                        double *__restrict__ z,
+                                                        double *__restrict__ vx,
+                                                        double *__restrict__ vy,
+                                                        double *__restrict__ vz,
+                                                        const double *__restrict__ ax,
+                                                        const double *__restrict__ ay,
+                                                        const double *__restrict__ az,
+                                                        const double *__restrict__ m, int n,
+                                                        double dt, int kicks) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n || !live_mass(m[i])) return;
+    double u = vx[i], v = vy[i], w = vz[i];
+    const double hx = 0.5 * ax[i], hy = 0.5 * ay[i], hz = 0.5 * az[i];
+    for (int k = 0; k < kicks; ++k) {
+        u = u + hx * dt;
+        v = v + hy * dt;
+        w = w + hz * dt;
+    }
+    vx[i] = u; vy[i] = v; vz[i] = w;
+    x[i] = x[i] + u * dt;
+    y[i] = y[i] + v * dt;
+    z[i] = z[i] + w * dt;
+}
+
+// v += (0.5*a)*dt for live bodies (src/integrator.c:45-51).
+__global__ __launch_bounds__(CSG_TPB) void k_kick(double *__restrict__ vx,
+                                                  double *__restrict__ vy,
+                                                  double *__restrict__ vz,
+                                                  const double *__restrict__ ax,
+                                                  const double *__restrict__ ay,
+                                                  const double *__restrict__ az,
+                                                  const double *__restrict__ m, int n,
+                                                  double dt) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n || !live_mass(m[i])) return;
+    vx[i] = vx[i] + 0.5 * ax[i] * dt;
+    vy[i] = vy[i] + 0.5 * ay[i] * dt;
+    vz[i] = vz[i] + 0.5 * az[i] * dt;
+}
+
+// ============================================================= AoS <-> SoA
+__global__ __launch_bounds__(CSG_TPB) void k_unpack(const Body *__restrict__ aos, int n,
+                                                    double *x, double *y, double *z, double *vx,
+                                                    double *vy, double *vz, double *ax,
+                                                    double *ay, double *az, double *m) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n) return;
+    const Body &b = aos[i];
+    x[i] = b.x; y[i] = b.y; z[i] = b.z;
+    vx[i] = b.vx; vy[i] = b.vy; vz[i] = b.vz;
+    ax[i] = b.ax; ay[i] = b.ay; az[i] = b.az;
+    m[i] = b.mass;
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_unpack_acc(const Body *__restrict__ aos, int n,
+                                                        double *ax, double *ay, double *az) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n) return;
+    ax[i] = aos[i].ax; ay[i] = aos[i].ay; az[i] = aos[i].az;
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_pack(Body *__restrict__ aos, int n,
+                                                  const double *x, const double *y,
+                                                  const double *z, const double *vx,
+                                                  const double *vy, const double *vz,
+                                                  const double *ax, const double *ay,
+                                                  const double *az, const double *m) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n) return;
+    Body &b = aos[i];
+    b.x = x[i]; b.y = y[i]; b.z = z[i];
+    b.vx = vx[i]; b.vy = vy[i]; b.vz = vz[i];
+    b.ax = ax[i]; b.ay = ay[i]; b.az = az[i];
+    b.mass = m[i];
+}
+
+// ======================================================= reference-layout export
+__global__ __launch_bounds__(CSG_TPB) void k_export_nodes(const WNode *__restrict__ nodes,
+                                                          const uint64_t *__restrict__ key,
+                                                          const uint64_t *__restrict__ ext,
+                                                          const BuildParams *__restrict__ bp,
+                                                          OctreeNode *__restrict__ pool) {
+    const int v = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (v >= bp->n_nodes) return;
+    const WNode w = nodes[v];
+    OctreeNode o;
+    o.cx = w.cx; o.cy = w.cy; o.cz = w.cz;
+    o.total_mass = w.m;
+    double cx = bp->root[0], cy = bp->root[1], cz = bp->root[2], h = bp->root[3];
+    if (bp->n_live > 0)
+        for (int l = 0; l < w.depth; ++l) cell_child(digit_at(key, ext, w.first, l), cx, cy, cz, h);
+    o.center_x = cx; o.center_y = cy; o.center_z = cz;
+    o.half_size = h;
+    o.body_index = w.body;
+    for (int c = 0; c < 8; ++c) o.children[c] = -1;
+    pool[v] = o;
+}
+
+// Each non-root node writes its id into its parent's slot for its octant.
+__global__ __launch_bounds__(CSG_TPB) void k_export_link(const WNode *__restrict__ nodes,
+                                                         const uint64_t *__restrict__ key,
+                                                         const uint64_t *__restrict__ ext,
+                                                         const signed char *__restrict__ L,
+                                                         const int *__restrict__ nbase,
+                                                         const BuildParams *__restrict__ bp,
+                                                         OctreeNode *__restrict__ pool) {
+    const int v = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (v >= bp->n_nodes || v == 0 || bp->n_live == 0) return;
+    const WNode w = nodes[v];
+    const int b = w.first, d = w.depth;
+    const int lpb = (b > 0) ? L[b - 1] : -1;
+    int parent;
+    if (v > nbase[b]) {
+        parent = v - 1;  // nodes opened by one body are consecutive, depth + 1 each
+    } else {
+        const int bp0 = range_begin(key, ext, b, d - 1);
+        const int lpp = (bp0 > 0) ? L[bp0 - 1] : -1;
+        parent = nbase[bp0] + (d - 1) - (min(lpp, 63) + 1);
+    }
+    (void)lpb;
+    if (parent < 0 || parent >= v) return;  // a parent precedes its children in pre-order
+    pool[parent].children[digit_at(key, ext, b, d - 1)] = v;
+}
+
+// ================================================== walk of a foreign pool
+// Exact reference traversal over a reference-layout pool (src/octree.c:171-218),
+// FP64 with IEEE sqrt/divide, explicit stack, children in slot order.
+#define CSG_POOL_STACK 600
+__global__ __launch_bounds__(CSG_TPB) void k_walk_pool(const OctreeNode *__restrict__ pool,
+                                                       int pool_size, int root,
+                                                       const double *__restrict__ px,
+                                                       const double *__restrict__ py,
+                                                       const double *__restrict__ pz,
+                                                       const double *__restrict__ pm, int n,
+                                                       double *__restrict__ ax,
+                                                       double *__restrict__ ay,
+                                                       double *__restrict__ az,
+                                                       int *__restrict__ icount, double G,
+                                                       double eps2, double theta,
+                                                       int *__restrict__ err) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    int cnt = 0;
+    if (live_mass(pm[i])) {
+        const double x = px[i], y = py[i], z = pz[i];
+        const double th2 = theta * theta;
+        int stack[CSG_POOL_STACK];
+        int top = 0;
+        stack[top++] = root;
+        while (top > 0) {
+            const int v = stack[--top];
+            if (v < 0 || v >= pool_size) { atomicOr(err, 2); continue; }
+            const OctreeNode &nd = pool[v];
+            if (nd.total_mass == 0.0) continue;
+            const double dx = nd.cx - x, dy = nd.cy - y, dz = nd.cz - z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            bool inter;
+            if (nd.body_index >= 0) inter = nd.body_index != i;
+            else {
+                const double s = 2.0 * nd.half_size;
+                inter = d2 > 0.0 && (s * s) / d2 < th2;
+                if (!inter) {
+                    for (int c = 7; c >= 0; --c) {
+                        const int ch = nd.children[c];
+                        if (ch < 0) continue;
+                        if (top >= CSG_POOL_STACK) { atomicOr(err, 1); break; }
+                        stack[top++] = ch;
+                    }
+                    continue;
+                }
+            }
+            if (inter) {
+                const double r2 = d2 + eps2;
+                const double w = 1.0 / (r2 * sqrt(r2));
+                const double gm = G * nd.total_mass;
+                sx += gm * dx * w;
+                sy += gm * dy * w;
+                sz += gm * dz * w;
+                ++cnt;
+            }
+        }
+    }
+    ax[i] = sx; ay[i] = sy; az[i] = sz;
+    if (icount) icount[i] = cnt;
+}
+
+// ================================================================ judges
+// Direct-sum accelerations of targets idx[0..k) against source chunk
+// blockIdx.y, one wave per 64 targets; chunk partials summed in chunk order by
+// k_direct_reduce.  Arithmetic as tests/test_physics.c:54-71.
+__global__ __launch_bounds__(64) void k_direct(const int *__restrict__ idx, int k,
+                                               const double *__restrict__ x,
+                                               const double *__restrict__ y,
+                                               const double *__restrict__ z,
+                                               const double *__restrict__ m, int n, int chunk,
+                                               double G, double eps2, double *__restrict__ part) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const bool act = t < k;
+    const int i = act ? idx[t] : 0;
+    const double xi = x[i], yi = y[i], zi = z[i];
+    const int j0 = blockIdx.y * chunk, j1 = min(n, j0 + chunk);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+        const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+        const double w = 1.0 / (r2 * sqrt(r2));
+        const double gm = G * m[j];
+        if (j != i) {
+            sx += gm * dx * w;
+            sy += gm * dy * w;
+            sz += gm * dz * w;
+        }
+    }
+    if (act) {
+        double *o = part + ((size_t)blockIdx.y * k + t) * 3;
+        o[0] = sx; o[1] = sy; o[2] = sz;
+    }
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_direct_reduce(const double *__restrict__ part, int k,
+                                                           int nchunk, double *__restrict__ out) {
+    const int t = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (t >= k) return;
+    for (int a = 0; a < 3; ++a) {
+        double s = 0.0;
+        for (int c = 0; c < nchunk; ++c) s += part[((size_t)c * k + t) * 3 + a];
+        out[t * 3 + a] = s;
+    }
+}
+
+// Potential energy partials: sum over j != i of G m_i m_j / sqrt(r^2 + eps^2)
+// for the 256 targets of a block against source chunk blockIdx.y.
+__global__ __launch_bounds__(CSG_TPB) void k_potential(const double *__restrict__ x,
+                                                       const double *__restrict__ y,
+                                                       const double *__restrict__ z,
+                                                       const double *__restrict__ m, int n,
+                                                       int chunk, double G, double eps2,
+                                                       double *__restrict__ part) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    const bool act = i < n;
+    const int ii = act ? i : 0;
+    const double xi = x[ii], yi = y[ii], zi = z[ii], mi = act ? m[ii] : 0.0;
+    const int j0 = blockIdx.y * chunk, j1 = min(n, j0 + chunk);
+    double s = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz + eps2);
+        if (j != ii) s += G * mi * m[j] / r;
+    }
+    __shared__ double red[CSG_TPB];
+    red[threadIdx.x] = act ? s : 0.0;
+    __syncthreads();
+    for (int st = CSG_TPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Kinetic energy and momentum per block: out[blk*4 + {0: KE, 1..3: p}].
+__global__ __launch_bounds__(CSG_TPB) void k_kinetic(const double *__restrict__ vx,
+                                                     const double *__restrict__ vy,
+                                                     const double *__restrict__ vz,
+                                                     const double *__restrict__ m, int n,
+                                                     double *__restrict__ part) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    double v[4] = {0, 0, 0, 0};
+    if (i < n) {
+        const double mm = m[i], u = vx[i], w = vy[i], q = vz[i];
+        v[0] = 0.5 * mm * (u * u + w * w + q * q);
+        v[1] = mm * u; v[2] = mm * w; v[3] = mm * q;
+    }
+    __shared__ double red[4][CSG_TPB];
+    for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int st = CSG_TPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Fixed-order compensated sum of cnt values with stride `stride` starting at
+// offset `off` (one block).
+__global__ __launch_bounds__(CSG_TPB) void k_sum(const double *__restrict__ v, int cnt,
+                                                 int stride, int off, double *__restrict__ out) {
+    __shared__ double hs[CSG_TPB], ls[CSG_TPB];
+    double s = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < cnt; k += CSG_TPB) {
+        const double t = v[(size_t)k * stride + off] - c;
+        const double u = s + t;
+        c = (u - s) - t;
+        s = u;
+    }
+    hs[threadIdx.x] = s;
+    ls[threadIdx.x] = -c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD acc{0.0, 0.0};
+        for (int k = 0; k < CSG_TPB; ++k) acc = dd_add(acc, dd_add(DD{hs[k], 0.0}, DD{ls[k], 0.0}));
+        *out = acc.h;
+    }
+}

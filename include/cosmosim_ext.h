@@ -1,0 +1,124 @@
+/*
+ * cosmosim_ext.h - extension entry points of libcosmosim (no counterpart in the
+ * reference).  They expose what the eight legacy exports cannot: caller-chosen
+ * G / softening / theta / precision, device-resident stepping without a host
+ * round trip per substep, diagnostics (direct-sum judge, energy, momentum,
+ * interaction counts, tree export) and a seeded initial-condition generator.
+ *
+ * Every function returns an int status: COSMOSIM_OK (0) or a negative
+ * COSMOSIM_E_* code.  Handles from cosmosim_create() and cosmosim_ext_create()
+ * are interchangeable: the eight legacy getters work on both.
+ */
+#ifndef COSMOSIM_EXT_H
+#define COSMOSIM_EXT_H
+
+#include "body.h"
+#include "octree.h"
+#include "cosmosim_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    COSMOSIM_OK = 0,
+    COSMOSIM_E_ARG = -1,        /* invalid argument (NULL pointer, n < 0, bad enum) */
+    COSMOSIM_E_NO_DEVICE = -2,  /* no usable HIP device */
+    COSMOSIM_E_ALLOC = -3,      /* host or device allocation failed */
+    COSMOSIM_E_HIP = -4,        /* a HIP runtime call or kernel launch failed */
+    COSMOSIM_E_CAPACITY = -5,   /* tree needs more than 8*n nodes, or an output buffer is too small */
+    COSMOSIM_E_STATE = -6,      /* call is not valid in the handle's current state */
+    COSMOSIM_E_UNSUPPORTED = -7 /* feature not available in this build */
+};
+
+/* Arithmetic of the force walk. */
+enum {
+    COSMOSIM_PREC_FP64_STRICT = 0, /* reference arithmetic: IEEE sqrt and divide, no FMA contraction */
+    COSMOSIM_PREC_FP64 = 1,        /* FP64 with refined hardware rsqrt and FMA */
+    COSMOSIM_PREC_FP32 = 2         /* FP32 on FP64-formed node offsets, compensated accumulation */
+};
+
+typedef struct {
+    double G;          /* gravitational constant */
+    double softening;  /* Plummer softening length (eps; eps^2 is added to r^2) */
+    double theta;      /* opening angle: accept a cell when (2h)^2/d^2 < theta^2 */
+    double dt;         /* substep length */
+    int precision;     /* COSMOSIM_PREC_* */
+    int device;        /* HIP device ordinal */
+} CosmosimExtParams;
+
+typedef struct {
+    int n;                 /* bodies held by the handle */
+    int n_live;            /* bodies with mass > 0 */
+    int n_nodes;           /* nodes of the last tree (reference-layout count) */
+    int max_depth;         /* deepest leaf of the last tree */
+    int collision_runs;    /* groups of bodies sharing a 21-level cell path in the last build */
+    int pad0;
+    long long interactions;/* body-node interactions of the last walk */
+    double root_center[3]; /* root cube of the last build */
+    double root_half;
+    double ms_build;       /* GPU time of the last build / walk / KDK (0 unless timing is on) */
+    double ms_walk;
+    double ms_kdk;
+} CosmosimExtStats;
+
+/* Seeded initial conditions (own generator; the reference's generators are
+ * time-seeded).  recipe 0: single exponential disk + central SMBH; recipe 1:
+ * two-disk merger.  orbit (may be NULL for the default) = relative position
+ * (3) and relative velocity (3) of disk 2 with respect to disk 1.  n counts all
+ * bodies including the SMBHs.  out must hold n records. */
+enum { COSMOSIM_IC_DISK = 0, COSMOSIM_IC_MERGER = 1 };
+int cosmosim_ext_generate(int recipe, int n, unsigned long long seed,
+                          const double orbit[6], Body *out);
+/* Default orbit of the merger recipe: separation 30, impact parameter 2,
+ * relative speed 0.3 (written to orbit[6]). */
+int cosmosim_ext_default_orbit(double orbit[6]);
+/* d(body state)/d(orbit): for body i, x_i = local_i + coef[i]*r_rel and
+ * v_i = vlocal_i + coef[i]*v_rel; writes coef[n] (disk 1: -M2/M, disk 2: M1/M). */
+int cosmosim_ext_orbit_coefficients(int recipe, int n, double *coef);
+
+/* Create a device-resident simulation of caller-supplied bodies (copied) and
+ * compute the initial accelerations. */
+int cosmosim_ext_create(const Body *bodies, int n, const CosmosimExtParams *params,
+                        SimHandle *out);
+int cosmosim_ext_set_params(SimHandle h, const CosmosimExtParams *params);
+int cosmosim_ext_get_params(SimHandle h, CosmosimExtParams *params);
+/* Advance k substeps on the device.  The host Body[] is NOT refreshed. */
+int cosmosim_ext_step(SimHandle h, int substeps);
+/* Copy the device state into the host Body[] returned by cosmosim_get_bodies. */
+int cosmosim_ext_sync(SimHandle h);
+/* Replace the device state with the host Body[] (after the caller edited it). */
+int cosmosim_ext_upload(SimHandle h);
+/* Rebuild the tree and recompute accelerations at the current positions. */
+int cosmosim_ext_compute_forces(SimHandle h);
+/* Device state as flat arrays (any pointer may be NULL): pos/vel/acc are
+ * [n][3] row-major, mass is [n]. */
+int cosmosim_ext_get_state(SimHandle h, double *pos, double *vel, double *acc, double *mass);
+int cosmosim_ext_set_state(SimHandle h, const double *pos, const double *vel,
+                           const double *acc, const double *mass);
+/* FP64 direct-sum accelerations (the judge) for the k bodies idx[0..k) against
+ * all bodies, written to acc[k][3]. */
+int cosmosim_ext_direct_forces(SimHandle h, const int *idx, int k, double *acc);
+/* Softened total energy and momentum of the device state (O(N^2) FP64). */
+int cosmosim_ext_energy(SimHandle h, double *kinetic, double *potential, double momentum[3]);
+/* Per-body interaction counts of the last walk (original body order). */
+int cosmosim_ext_interaction_counts(SimHandle h, int *counts, int n);
+int cosmosim_ext_get_stats(SimHandle h, CosmosimExtStats *out);
+/* Enable per-phase GPU event timing (adds a synchronisation per phase). */
+int cosmosim_ext_set_timing(SimHandle h, int enabled);
+/* Write the last tree in reference layout (root 0); capacity in nodes. */
+int cosmosim_ext_export_tree(SimHandle h, OctreeNode *pool, int capacity, int *pool_size);
+/* Last error recorded on the handle by a void legacy call (cosmosim_step). */
+int cosmosim_ext_handle_error(SimHandle h);
+/* Last error recorded on the calling thread by octree_* / integrator_*. */
+int cosmosim_ext_last_error(void);
+void cosmosim_ext_clear_error(void);
+const char *cosmosim_ext_error_string(int code);
+/* Name and compute-unit count of the HIP device (name buffer of len bytes). */
+int cosmosim_ext_device_info(int device, char *name, int len, int *compute_units);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,87 @@
+/*
+ * csg.h - the thin C-ABI between the C host layer and the HIP gravity core
+ * ("cosmosim gravity", gfx950).  No counterpart in the reference: it is the
+ * seam below src/octree.c and src/integrator.c, whose work it carries out on
+ * one GPU.  Plain pointers and sizes only; every call returns 0 or a negative
+ * COSMOSIM_E_* code from cosmosim_ext.h.
+ *
+ * A csg_ctx owns one HIP stream and the device state of up to `capacity`
+ * bodies: FP64 SoA arrays x,y,z,vx,vy,vz,ax,ay,az,m in the caller's body order,
+ * the sort/tree scratch and the tree itself.  All calls on one context must
+ * come from one thread at a time; each call makes the context's device current
+ * on the calling thread.
+ */
+#ifndef CSG_H
+#define CSG_H
+
+#include "body.h"
+#include "octree.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct csg_ctx csg_ctx;
+
+typedef struct {
+    int n, n_live, n_nodes, max_depth, collision_runs, overflow;
+    long long interactions;
+    double root[4];                 /* centre x, y, z and half size of the root cube */
+    double ms_build, ms_walk, ms_kdk;
+} csg_stats;
+
+int  csg_device_count(int *count);
+int  csg_device_info(int device, char *name, int len, int *compute_units);
+int  csg_create(int device, int capacity, csg_ctx **out);
+void csg_destroy(csg_ctx *ctx);
+int  csg_reserve(csg_ctx *ctx, int capacity);
+int  csg_size(const csg_ctx *ctx);
+
+/* Host AoS <-> device.  set_bodies copies all 168-byte records (the fields the
+ * core does not touch are kept on the device and written back unchanged by
+ * get_bodies).  set_accelerations uploads only ax, ay, az. */
+int  csg_set_bodies(csg_ctx *ctx, const Body *bodies, int n);
+int  csg_get_bodies(csg_ctx *ctx, Body *bodies, int n);
+int  csg_set_accelerations(csg_ctx *ctx, const Body *bodies, int n);
+int  csg_get_state(csg_ctx *ctx, double *pos, double *vel, double *acc, double *mass);
+int  csg_set_state(csg_ctx *ctx, const double *pos, const double *vel,
+                   const double *acc, const double *mass);
+
+/* KDK pieces (src/integrator.c:22-31 and :45-51). */
+int  csg_kick_drift(csg_ctx *ctx, double dt);
+int  csg_kick(csg_ctx *ctx, double dt);
+/* Tree over the live bodies at the current positions (src/octree.c:114-169). */
+int  csg_build(csg_ctx *ctx);
+/* Force walk over the last tree (src/octree.c:171-234); precision is a
+ * COSMOSIM_PREC_* value.  Dead bodies get a = 0. */
+int  csg_walk(csg_ctx *ctx, int precision, double G, double softening, double theta);
+/* k full substeps on the device: kick+drift, build, walk, kick (src/integrator.c:12-57
+ * without the SPH hooks). */
+int  csg_substeps(csg_ctx *ctx, int k, double dt, double G, double softening,
+                  double theta, int precision);
+/* Last tree in reference layout (root 0); fails with COSMOSIM_E_CAPACITY when
+ * the tree has more than capacity nodes. */
+int  csg_export_pool(csg_ctx *ctx, OctreeNode *pool, int capacity, int *pool_size);
+/* Walk a caller-supplied reference-layout pool from `root` (exact reference
+ * traversal, FP64 strict arithmetic). */
+int  csg_walk_pool(csg_ctx *ctx, const OctreeNode *pool, int pool_size, int root,
+                   double G, double softening, double theta);
+/* Diagnostics. */
+int  csg_direct_forces(csg_ctx *ctx, const int *idx, int k, double G, double softening,
+                       double *acc);
+int  csg_energy(csg_ctx *ctx, double G, double softening, double *kinetic,
+                double *potential, double momentum[3]);
+int  csg_interaction_counts(csg_ctx *ctx, int *counts, int n);
+int  csg_get_stats(csg_ctx *ctx, csg_stats *out);
+int  csg_set_timing(csg_ctx *ctx, int enabled);
+int  csg_synchronize(csg_ctx *ctx);
+
+/* Pinned host memory for the handle-owned Body[] (zero-filled). */
+void *csg_host_alloc(size_t bytes);
+void  csg_host_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,419 @@
+/*
+ * cpu_ref.c - CPU oracle for the gravity hot path.  TEST INFRASTRUCTURE ONLY:
+ * it is linked by nothing in the product; tests/, __graft_entry__.smoke() and
+ * the cpu_baseline leg of bench.py load it (through oracle/cpu_ref.py) as the
+ * checker and as the timed host baseline.
+ *
+ * It restates, in its own code, the semantics of
+ *   src/octree.c:5-234     (tree build by sequential insertion, monopole walk)
+ *   src/integrator.c:4-57  (KDK leapfrog, without the SPH hooks)
+ *   src/octree.c:236-318   (sphere neighbour query)
+ *   tests/test_physics.c:34-71 (direct-sum and energy judges)
+ * Floating-point expressions are kept in the reference's evaluation order so
+ * that, compiled with -O2 -ffp-contract=off on x86-64, results are bit-identical
+ * to the reference (checked by tests/test_oracle.py against tests/golden/,
+ * which tests/golden/gen_golden.py wrote from the compiled reference).
+ *
+ * Differences in structure, not in results: insertion and the walk are
+ * iterative (explicit loops / stack) instead of recursive, and the walk can
+ * report the number of interactions per body.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "body.h"
+#include "octree.h"
+
+#define CREF_DEPTH_LIMIT 64 /* src/octree.c:5 */
+
+static inline int cref_live(const Body *b) { return !(b->mass <= 0.0); }
+
+/* New node with no mass, no body and no children (src/octree.c:7-28). */
+static int cref_node(OctreeNode *pool, int *used, double gx, double gy, double gz, double half)
+{
+    int id = (*used)++;
+    OctreeNode *nd = &pool[id];
+    memset(nd, 0, sizeof(*nd));
+    nd->center_x = gx;
+    nd->center_y = gy;
+    nd->center_z = gz;
+    nd->half_size = half;
+    nd->body_index = -1;
+    for (int c = 0; c < 8; ++c)
+        nd->children[c] = -1;
+    return id;
+}
+
+/* Octant code: bit0 east (x >= centre), bit1 south (y < centre), bit2 bottom
+ * (z < centre) (src/octree.c:30-40). */
+static inline int cref_octant(const OctreeNode *nd, double x, double y, double z)
+{
+    return (x >= nd->center_x ? 1 : 0) | (y < nd->center_y ? 2 : 0) | (z < nd->center_z ? 4 : 0);
+}
+
+/* Child cell of octant q: centre shifted by half the parent half-size along each
+ * axis (+ for east, - for south/bottom), half-size halved (src/octree.c:42-50,
+ * 87-90, 105-108). */
+static int cref_child(OctreeNode *pool, int *used, int parent, int q)
+{
+    const OctreeNode *p = &pool[parent];
+    double h = p->half_size;
+    double sx = (q & 1) ? 0.5 : -0.5;
+    double sy = (q & 2) ? -0.5 : 0.5;
+    double sz = (q & 4) ? -0.5 : 0.5;
+    double gx = p->center_x + sx * h;
+    double gy = p->center_y + sy * h;
+    double gz = p->center_z + sz * h;
+    return cref_node(pool, used, gx, gy, gz, h * 0.5);
+}
+
+/* Running-average mass/COM update in the reference's order (src/octree.c:96-100). */
+static inline void cref_absorb(OctreeNode *nd, double x, double y, double z, double m)
+{
+    double mm = nd->total_mass + m;
+    nd->cx = (nd->cx * nd->total_mass + x * m) / mm;
+    nd->cy = (nd->cy * nd->total_mass + y * m) / mm;
+    nd->cz = (nd->cz * nd->total_mass + z * m) / mm;
+    nd->total_mass = mm;
+}
+
+/* Insert one live body, walking down from the root (src/octree.c:52-112). */
+static void cref_insert(OctreeNode *pool, int *used, const Body *bodies, int b)
+{
+    const double x = bodies[b].x, y = bodies[b].y, z = bodies[b].z, m = bodies[b].mass;
+    int v = 0;
+    for (int depth = 0;; ++depth) {
+        OctreeNode *nd = &pool[v];
+        if (nd->total_mass == 0.0 && nd->body_index == -1) {
+            /* empty cell: the body becomes its leaf */
+            nd->body_index = b;
+            nd->cx = x;
+            nd->cy = y;
+            nd->cz = z;
+            nd->total_mass = m;
+            return;
+        }
+        if (depth >= CREF_DEPTH_LIMIT) {
+            /* depth limit: merge into this leaf, first body index kept */
+            cref_absorb(nd, x, y, z, m);
+            return;
+        }
+        if (nd->body_index >= 0) {
+            /* occupied leaf: push the resident body one level down */
+            int r = nd->body_index;
+            nd->body_index = -1;
+            int qr = cref_octant(nd, bodies[r].x, bodies[r].y, bodies[r].z);
+            int c = cref_child(pool, used, v, qr);
+            nd = &pool[v];
+            nd->children[qr] = c;
+            OctreeNode *cn = &pool[c];
+            cn->body_index = r;
+            cn->cx = bodies[r].x;
+            cn->cy = bodies[r].y;
+            cn->cz = bodies[r].z;
+            cn->total_mass = bodies[r].mass;
+        }
+        cref_absorb(nd, x, y, z, m);
+        int q = cref_octant(nd, x, y, z);
+        if (nd->children[q] < 0) {
+            int c = cref_child(pool, used, v, q);
+            pool[v].children[q] = c;
+        }
+        v = pool[v].children[q];
+    }
+}
+
+/* Root cube over live bodies and sequential insertion (src/octree.c:114-169). */
+void cref_build(OctreeNode *pool, int *pool_size, const Body *bodies, int n)
+{
+    int used = 0;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    int live = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!cref_live(&bodies[i]))
+            continue;
+        ++live;
+        const double p[3] = {bodies[i].x, bodies[i].y, bodies[i].z};
+        for (int a = 0; a < 3; ++a) {
+            if (p[a] < lo[a])
+                lo[a] = p[a];
+            if (p[a] > hi[a])
+                hi[a] = p[a];
+        }
+    }
+    if (live == 0) {
+        cref_node(pool, &used, 0.0, 0.0, 0.0, 1.0);
+        *pool_size = used;
+        return;
+    }
+    double ext = hi[0] - lo[0];
+    if (hi[1] - lo[1] > ext)
+        ext = hi[1] - lo[1];
+    if (hi[2] - lo[2] > ext)
+        ext = hi[2] - lo[2];
+    double half = ext * 0.5 * 1.01;
+    if (half < 1e-10)
+        half = 1.0;
+    cref_node(pool, &used, (lo[0] + hi[0]) * 0.5, (lo[1] + hi[1]) * 0.5, (lo[2] + hi[2]) * 0.5,
+              half);
+    for (int i = 0; i < n; ++i)
+        if (cref_live(&bodies[i]))
+            cref_insert(pool, &used, bodies, i);
+    *pool_size = used;
+}
+
+/* Monopole term of a node or body on body i, added in the reference's order
+ * (src/octree.c:192-197, 204-209). */
+static inline void cref_term(Body *bi, double G, double M, double dx, double dy, double dz,
+                             double d2, double eps2)
+{
+    double r2 = d2 + eps2;
+    double w = 1.0 / (r2 * sqrt(r2));
+    bi->ax += G * M * dx * w;
+    bi->ay += G * M * dy * w;
+    bi->az += G * M * dz * w;
+}
+
+/* Depth-first walk for one body with an explicit stack; children in slot order
+ * 0..7 (src/octree.c:171-218).  Returns the number of interactions. */
+static long cref_walk_one(const OctreeNode *pool, int root, Body *bodies, int i, double G,
+                          double eps2, double theta)
+{
+    /* pending[] holds nodes still to visit, pushed in reverse slot order so
+     * they pop in slot order. */
+    int pending[8 * 70 + 8];
+    int top = 0;
+    long count = 0;
+    Body *bi = &bodies[i];
+    const double th2 = theta * theta;
+    pending[top++] = root;
+    while (top > 0) {
+        const OctreeNode *nd = &pool[pending[--top]];
+        if (nd->total_mass == 0.0)
+            continue;
+        double dx = nd->cx - bi->x;
+        double dy = nd->cy - bi->y;
+        double dz = nd->cz - bi->z;
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (nd->body_index >= 0) {
+            if (nd->body_index != i) {
+                cref_term(bi, G, nd->total_mass, dx, dy, dz, d2, eps2);
+                ++count;
+            }
+            continue;
+        }
+        double s = 2.0 * nd->half_size;
+        if (d2 > 0.0 && (s * s) / d2 < th2) {
+            cref_term(bi, G, nd->total_mass, dx, dy, dz, d2, eps2);
+            ++count;
+            continue;
+        }
+        for (int c = 7; c >= 0; --c)
+            if (nd->children[c] >= 0)
+                pending[top++] = nd->children[c];
+    }
+    return count;
+}
+
+/* Zero a for every body, then walk every live body (src/octree.c:220-234).
+ * counts (may be NULL) receives per-body interaction counts. */
+void cref_forces(const OctreeNode *pool, int root, Body *bodies, int n, double G,
+                 double softening, double theta, int *counts)
+{
+    const double eps2 = softening * softening;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        bodies[i].ax = 0.0;
+        bodies[i].ay = 0.0;
+        bodies[i].az = 0.0;
+        long c = 0;
+        if (cref_live(&bodies[i]))
+            c = cref_walk_one(pool, root, bodies, i, G, eps2, theta);
+        if (counts)
+            counts[i] = (int)c;
+    }
+}
+
+/* Half kick (src/integrator.c:45-51). */
+static void cref_half_kick(Body *bodies, int n, double dt)
+{
+    for (int i = 0; i < n; ++i) {
+        Body *b = &bodies[i];
+        if (!cref_live(b))
+            continue;
+        b->vx += 0.5 * b->ax * dt;
+        b->vy += 0.5 * b->ay * dt;
+        b->vz += 0.5 * b->az * dt;
+    }
+}
+
+/* Build + walk once (src/integrator.c:4-10). */
+void cref_init_acc(Body *bodies, int n, double G, double softening, double theta,
+                   OctreeNode *pool)
+{
+    int used = 0;
+    cref_build(pool, &used, bodies, n);
+    cref_forces(pool, 0, bodies, n, G, softening, theta, NULL);
+}
+
+/* One KDK substep without the SPH hooks (src/integrator.c:12-57). */
+void cref_step(Body *bodies, int n, double dt, double G, double softening, double theta,
+               OctreeNode *pool)
+{
+    for (int i = 0; i < n; ++i) {
+        Body *b = &bodies[i];
+        if (!cref_live(b))
+            continue;
+        b->vx += 0.5 * b->ax * dt;
+        b->vy += 0.5 * b->ay * dt;
+        b->vz += 0.5 * b->az * dt;
+        b->x += b->vx * dt;
+        b->y += b->vy * dt;
+        b->z += b->vz * dt;
+    }
+    int used = 0;
+    cref_build(pool, &used, bodies, n);
+    cref_forces(pool, 0, bodies, n, G, softening, theta, NULL);
+    cref_half_kick(bodies, n, dt);
+}
+
+/* Timed pieces for the CPU baseline: the same work as cref_step, split by phase. */
+void cref_kick_drift(Body *bodies, int n, double dt)
+{
+    for (int i = 0; i < n; ++i) {
+        Body *b = &bodies[i];
+        if (!cref_live(b))
+            continue;
+        b->vx += 0.5 * b->ax * dt;
+        b->vy += 0.5 * b->ay * dt;
+        b->vz += 0.5 * b->az * dt;
+        b->x += b->vx * dt;
+        b->y += b->vy * dt;
+        b->z += b->vz * dt;
+    }
+}
+
+void cref_kick(Body *bodies, int n, double dt) { cref_half_kick(bodies, n, dt); }
+
+/* O(N^2) softened direct sum over all other bodies in index order
+ * (tests/test_physics.c:54-71).  Only the k targets idx[] are computed; their
+ * accelerations go to acc[k][3]. */
+void cref_direct(const Body *bodies, int n, const int *idx, int k, double G, double softening,
+                 double *acc)
+{
+    const double eps2 = softening * softening;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int t = 0; t < k; ++t) {
+        const int i = idx[t];
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i)
+                continue;
+            double dx = bodies[j].x - bodies[i].x;
+            double dy = bodies[j].y - bodies[i].y;
+            double dz = bodies[j].z - bodies[i].z;
+            double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            double w = 1.0 / (r2 * sqrt(r2));
+            ax += G * bodies[j].mass * dx * w;
+            ay += G * bodies[j].mass * dy * w;
+            az += G * bodies[j].mass * dz * w;
+        }
+        acc[3 * t + 0] = ax;
+        acc[3 * t + 1] = ay;
+        acc[3 * t + 2] = az;
+    }
+}
+
+/* Softened total energy KE + PE with PE over pairs i<j
+ * (tests/test_physics.c:34-52).  out[0] = KE, out[1] = PE. */
+void cref_energy(const Body *bodies, int n, double G, double softening, double *out)
+{
+    double ke = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const Body *b = &bodies[i];
+        ke += 0.5 * b->mass * (b->vx * b->vx + b->vy * b->vy + b->vz * b->vz);
+    }
+    double pe = 0.0;
+    for (int i = 0; i < n; ++i) {
+        for (int j = i + 1; j < n; ++j) {
+            double dx = bodies[j].x - bodies[i].x;
+            double dy = bodies[j].y - bodies[i].y;
+            double dz = bodies[j].z - bodies[i].z;
+            double r = sqrt(dx * dx + dy * dy + dz * dz + softening * softening);
+            pe -= G * bodies[i].mass * bodies[j].mass / r;
+        }
+    }
+    out[0] = ke;
+    out[1] = pe;
+}
+
+/* Sphere query: cells whose box misses the sphere are pruned, leaves report
+ * live bodies inside the sphere, children in slot order, at most cap results
+ * (src/octree.c:236-318). */
+static void cref_nb_visit(const OctreeNode *pool, int v, const Body *bodies, double qx, double qy,
+                          double qz, double r2, int *buf, int *count, int cap)
+{
+    const OctreeNode *nd = &pool[v];
+    double ex = fmax(0.0, fabs(qx - nd->center_x) - nd->half_size);
+    double ey = fmax(0.0, fabs(qy - nd->center_y) - nd->half_size);
+    double ez = fmax(0.0, fabs(qz - nd->center_z) - nd->half_size);
+    if (ex * ex + ey * ey + ez * ez > r2)
+        return;
+    if (nd->body_index >= 0) {
+        if (*count >= cap)
+            return;
+        const Body *b = &bodies[nd->body_index];
+        if (b->mass <= 0.0)
+            return;
+        double bx = b->x - qx, by = b->y - qy, bz = b->z - qz;
+        if (bx * bx + by * by + bz * bz <= r2)
+            buf[(*count)++] = nd->body_index;
+        return;
+    }
+    for (int c = 0; c < 8; ++c)
+        if (nd->children[c] >= 0 && *count < cap)
+            cref_nb_visit(pool, nd->children[c], bodies, qx, qy, qz, r2, buf, count, cap);
+}
+
+int cref_find_neighbors(const OctreeNode *pool, int root, const Body *bodies, int n, double qx,
+                        double qy, double qz, double radius, int *buf, int cap)
+{
+    (void)n;
+    int count = 0;
+    cref_nb_visit(pool, root, bodies, qx, qy, qz, radius * radius, buf, &count, cap);
+    return count;
+}
+
+/* Reference-layout pools differ between builders only in node numbering when
+ * the trees are equal.  cref_canonical renumbers a pool in depth-first
+ * pre-order (children in slot order) into out[], so two pools can be compared
+ * field by field.  Returns the number of nodes reached from root. */
+int cref_canonical(const OctreeNode *pool, int root, OctreeNode *out, int *map, int pool_size)
+{
+    for (int i = 0; i < pool_size; ++i)
+        map[i] = -1;
+    int *stack = (int *)malloc(sizeof(int) * (size_t)(pool_size + 8));
+    int top = 0, next = 0;
+    stack[top++] = root;
+    while (top > 0) {
+        int v = stack[--top];
+        map[v] = next++;
+        for (int c = 7; c >= 0; --c)
+            if (pool[v].children[c] >= 0)
+                stack[top++] = pool[v].children[c];
+    }
+    for (int v = 0; v < pool_size; ++v) {
+        if (map[v] < 0)
+            continue;
+        OctreeNode nd = pool[v];
+        for (int c = 0; c < 8; ++c)
+            if (nd.children[c] >= 0)
+                nd.children[c] = map[nd.children[c]];
+        out[map[v]] = nd;
+    }
+    free(stack);
+    return next;
+}
