@@ -1,0 +1,221 @@
+"""Benchmark: substeps/s of the 1M-body merger (FP32 walk vs FP64 walk) on one GPU.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--thetas 0.4,0.5,0.7]
+
+A step is one full KDK substep on the device (kick+drift, tree build, walk,
+kick) of the seeded 1,000,000-body two-disk merger (seed 3, eps 0.005,
+dt 5e-4), with every input resident in HBM before the clock starts.  The
+headline value is the FP32 walk at theta 0.5; the FP64 walks (strict reference
+arithmetic and refined-rsqrt FP64) are timed alongside at each theta.  Prints
+one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "substeps/s @1M-body merger: FP32 walk vs FP64, at RMS accel err; adjoint step/s"
+N_BODIES = 1_000_000
+SEED = 3
+EPS = 0.005
+DT = 5e-4
+FLOP_PER_INTERACTION = 20       # 3 sub, 5 for d^2, +eps^2, sqrt, mul, div, G*M, 6 scale-add
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+SUBSET = 4096
+
+
+def rel_err(a, b):
+    d = np.linalg.norm(a - b, axis=1)
+    s = np.linalg.norm(b, axis=1)
+    s[s == 0] = 1.0
+    e = d / s
+    return float(np.sqrt(np.mean(e ** 2))), float(e.max())
+
+
+def timed_run(ca, bodies, theta, precision, steps, warmup, energy=False):
+    sim = ca.Simulation(bodies, softening=EPS, theta=theta, dt=DT, precision=precision)
+    sim.advance(warmup)
+    e0 = sim.energy() if energy else None
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.advance(steps)                       # returns after the stream has drained
+    t1 = time.perf_counter()
+    st = sim.stats()
+    out = {
+        "substeps_per_s": steps / (t1 - t0),
+        "ms_per_step": 1e3 * (t1 - t0) / steps,
+        "walk_ms": st.ms_walk / max(st.timed_walks, 1),
+        "build_ms": st.ms_build / steps,
+        "kdk_ms": st.ms_kdk / steps,
+        "interactions": int(st.interactions),
+        "nodes": int(st.n_nodes),
+        "max_depth": int(st.max_depth),
+    }
+    out["interactions_per_s"] = out["interactions"] * out["substeps_per_s"]
+    if energy:
+        e1 = sim.energy()
+        E0, E1 = e0[0] + e0[1], e1[0] + e1[1]
+        out["energy_drift"] = abs((E1 - E0) / E0)
+        out["momentum_drift"] = float(np.linalg.norm(e1[2] - e0[2]))
+    return sim, out
+
+
+def accuracy(ca, sim, idx):
+    """Three errors on one state: FP32 vs FP64 on the same tree (precision),
+    FP64 vs FP64 direct sum (method), FP32 vs direct sum."""
+    sim.set_params(precision="fp64_strict")
+    sim.compute_forces()
+    a64 = sim.accelerations()
+    sim.set_params(precision="fp32")
+    sim.compute_forces()
+    a32 = sim.accelerations()
+    direct = sim.direct_forces(idx)
+    r = {}
+    r["fp32_vs_fp64_rms"], r["fp32_vs_fp64_max"] = rel_err(a32[idx], a64[idx])
+    r["fp32_vs_fp64_rms_all"], r["fp32_vs_fp64_max_all"] = rel_err(a32, a64)
+    r["fp64_vs_direct_rms"], r["fp64_vs_direct_max"] = rel_err(a64[idx], direct)
+    r["fp32_vs_direct_rms"], r["fp32_vs_direct_max"] = rel_err(a32[idx], direct)
+    return r
+
+
+def cpu_baseline(ca, bodies, theta):
+    """The reference's own C (oracle/_ref) when it was built here, else the
+    oracle's restatement, on the host cores: median of 3 KDK substeps."""
+    from oracle.cpu_ref import CpuRef, RefBuild
+    impl = RefBuild() if RefBuild.available() else CpuRef()
+    b = bodies.copy()
+    impl.init_acc(b, 1.0, EPS, theta)
+    pool = np.zeros(8 * len(b) + 16, dtype=ca.NODE_DTYPE)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        impl.step(b, DT, 1.0, EPS, theta, pool)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": 1.0 / med, "unit": "substeps/s", "cores": threads, "kind": impl.kind,
+        "sample": (f"3 KDK substeps (median) of the same 1M-body merger at theta {theta}, "
+                   f"eps {EPS}: serial build, OpenMP walk ({threads} threads), serial KDK"),
+        "seconds_per_substep": med,
+    }
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "walk_fp32_1M_merger_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--thetas", default="0.4,0.5,0.7")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-energy", action="store_true")
+    ap.add_argument("--headline-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import cosmosim_amd as ca
+    dev_name, cus = ca.device_info(0)
+    bodies = ca.generate(ca.IC_MERGER, N_BODIES, SEED)
+    rng = np.random.default_rng(12345)
+    idx = np.sort(rng.choice(N_BODIES, SUBSET, replace=False)).astype(np.int32)
+    thetas = [float(t) for t in args.thetas.split(",")]
+    if 0.5 not in thetas:
+        thetas.append(0.5)
+    precisions = ["fp32"] if args.headline_only else ["fp32", "fp64", "fp64_strict"]
+
+    results = {}
+    head = None
+    for th in thetas:
+        for prec in precisions:
+            headline = (th == 0.5 and prec == "fp32")
+            sim, r = timed_run(ca, bodies, th, prec, args.steps, args.warmup,
+                               energy=(not args.no_energy and th == 0.5 and prec != "fp64"))
+            if prec == "fp32":
+                results[f"theta{th}_accuracy_end"] = accuracy(ca, sim, idx)
+            sim.close()
+            results[f"theta{th}_{prec}"] = r
+            if headline:
+                head = r
+        s0 = ca.Simulation(bodies, softening=EPS, theta=th, dt=DT, precision="fp64_strict")
+        results[f"theta{th}_accuracy_step0"] = accuracy(ca, s0, idx)
+        s0.close()
+
+    walk_flop = head["interactions"] * FLOP_PER_INTERACTION
+    achieved = walk_flop / (head["walk_ms"] * 1e-3) / 1e12
+    traffic = load_traffic()
+    line = {
+        "metric": METRIC,
+        "value": round(head["substeps_per_s"], 3),
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "1M-body two-disk merger (seed 3), theta 0.5, eps 0.005, dt 5e-4, "
+                        "FP32 walk on FP64-formed offsets, FP64 state and KDK",
+            "bodies": N_BODIES, "theta": 0.5, "softening": EPS, "dt": DT,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "k_walk<WALK_FP32>",
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+            "traffic": None if traffic is None else traffic.get("bytes_per_launch"),
+            "per_launch": {"interactions": head["interactions"],
+                           "flop_per_interaction": FLOP_PER_INTERACTION,
+                           "avg_ms": round(head["walk_ms"], 4)},
+        },
+        "device": {"name": dev_name, "compute_units": cus},
+    }
+    fp64 = results.get("theta0.5_fp64")
+    strict = results.get("theta0.5_fp64_strict")
+    if fp64:
+        line["fp32_speedup_vs_fp64"] = round(head["substeps_per_s"] / fp64["substeps_per_s"], 3)
+    if strict:
+        line["fp32_speedup_vs_fp64_strict"] = round(
+            head["substeps_per_s"] / strict["substeps_per_s"], 3)
+    acc = results["theta0.5_accuracy_step0"]
+    line["accuracy_theta0.5"] = {k: float(f"{v:.3e}") for k, v in acc.items()}
+    if not args.no_cpu:
+        cb = cpu_baseline(ca, bodies, 0.5)
+        line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v)
+                                for k, v in cb.items()}
+        line["speedup_vs_cpu"] = round(head["substeps_per_s"] / cb["value"], 1)
+    line["results"] = {k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv)
+                           for kk, vv in v.items() if not isinstance(vv, dict)}
+                       for k, v in results.items() if "accuracy" not in k}
+    line["accuracy"] = {k: {kk: float(f"{vv:.3e}") for kk, vv in v.items()}
+                        for k, v in results.items() if "accuracy" in k}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -82,7 +82,7 @@ class ExtStats(ctypes.Structure):
         ("n_nodes", ctypes.c_int),
         ("max_depth", ctypes.c_int),
         ("collision_runs", ctypes.c_int),
-        ("pad0", ctypes.c_int),
+        ("timed_walks", ctypes.c_int),
         ("interactions", ctypes.c_longlong),
         ("root_center", ctypes.c_double * 3),
         ("root_half", ctypes.c_double),

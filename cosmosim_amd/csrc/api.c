@@ -499,6 +499,7 @@ COSMOSIM_API int cosmosim_ext_get_stats(SimHandle h, CosmosimExtStats *out)
     out->ms_build = cs.ms_build;
     out->ms_walk = cs.ms_walk;
     out->ms_kdk = cs.ms_kdk;
+    out->timed_walks = cs.timed_walks;
     return COSMOSIM_OK;
 }
 

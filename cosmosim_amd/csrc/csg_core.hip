@@ -10,6 +10,9 @@
 #include "cosmosim_ext.h"
 #include "csg.h"
 
+#define CSG_EV_MAX 4096
+enum { PH_BUILD = 0, PH_WALK = 1, PH_KDK = 2 };
+
 struct csg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -41,10 +44,14 @@ struct csg_ctx {
     int *err_dev = nullptr;
     void *diag = nullptr;
     size_t diag_cap = 0;
-    // timing
+    // timing: event pairs recorded on the stream without host waits and read
+    // back in collect_timers(); totals accumulate until timing is re-enabled
     int timing = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t *ev = nullptr;  // 2 * CSG_EV_MAX events, created on first use
+    int ev_phase[CSG_EV_MAX];
+    int ev_count = 0, ev_open = -1;
     double ms_build = 0, ms_walk = 0, ms_kdk = 0;
+    int timed_walks = 0;
 };
 
 #define CSG_CHECK(expr)                                                                      \
@@ -146,15 +153,29 @@ static int ensure_diag(csg_ctx *c, size_t bytes) {
 }
 
 static void timer_start(csg_ctx *c) {
-    if (c->timing) (void)hipEventRecord(c->ev0, c->stream);
+    if (!c->timing || !c->ev || c->ev_count >= CSG_EV_MAX) return;
+    c->ev_open = c->ev_count;
+    (void)hipEventRecord(c->ev[2 * c->ev_open], c->stream);
 }
-static double timer_stop(csg_ctx *c) {
-    if (!c->timing) return 0.0;
-    (void)hipEventRecord(c->ev1, c->stream);
-    (void)hipEventSynchronize(c->ev1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    return ms;
+static void timer_stop(csg_ctx *c, int phase) {
+    if (c->ev_open < 0) return;
+    (void)hipEventRecord(c->ev[2 * c->ev_open + 1], c->stream);
+    c->ev_phase[c->ev_open] = phase;
+    c->ev_count = c->ev_open + 1;
+    c->ev_open = -1;
+}
+// Wait for the recorded events and add their durations to the phase totals.
+static void collect_timers(csg_ctx *c) {
+    if (!c->ev || c->ev_count == 0) return;
+    (void)hipEventSynchronize(c->ev[2 * c->ev_count - 1]);
+    for (int k = 0; k < c->ev_count; ++k) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]);
+        if (c->ev_phase[k] == PH_BUILD) c->ms_build += ms;
+        else if (c->ev_phase[k] == PH_WALK) { c->ms_walk += ms; c->timed_walks += 1; }
+        else c->ms_kdk += ms;
+    }
+    c->ev_count = 0;
 }
 
 static int read_params(csg_ctx *c) {
@@ -208,8 +229,7 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     if (hipMalloc((void **)&c->bp, sizeof(BuildParams)) != hipSuccess ||
         hipHostMalloc((void **)&c->hbp, sizeof(BuildParams), hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&c->itotal, sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc((void **)&c->err_dev, sizeof(int)) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipMalloc((void **)&c->err_dev, sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
         csg_destroy(c);
         return COSMOSIM_E_ALLOC;
@@ -235,8 +255,11 @@ void csg_destroy(csg_ctx *c) {
     dfree(c->pool_dev);
     if (c->diag) (void)hipFree(c->diag);
     if (c->hbp) (void)hipHostFree(c->hbp);
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev) {
+        for (int k = 0; k < 2 * CSG_EV_MAX; ++k)
+            if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+        free(c->ev);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     (void)hipGetLastError();
     delete c;
@@ -494,7 +517,7 @@ int csg_kick_drift(csg_ctx *c, double dt) {
     timer_start(c);
     int s = launch_kick_drift(c, dt, 1);
     if (s) return s;
-    c->ms_kdk = timer_stop(c);
+    timer_stop(c, PH_KDK);
     CSG_CHECK(hipStreamSynchronize(c->stream));
     return COSMOSIM_OK;
 }
@@ -504,7 +527,7 @@ int csg_kick(csg_ctx *c, double dt) {
     timer_start(c);
     int s = launch_kick(c, dt);
     if (s) return s;
-    c->ms_kdk += timer_stop(c);
+    timer_stop(c, PH_KDK);
     CSG_CHECK(hipStreamSynchronize(c->stream));
     return COSMOSIM_OK;
 }
@@ -512,6 +535,7 @@ int csg_kick(csg_ctx *c, double dt) {
 static int check_overflow(csg_ctx *c) {
     int s = read_params(c);
     if (s) return s;
+    collect_timers(c);
     if (c->hbp->overflow) {
         c->tree_valid = 0;
         return COSMOSIM_E_CAPACITY;
@@ -524,7 +548,7 @@ int csg_build(csg_ctx *c) {
     timer_start(c);
     int s = launch_build(c);
     if (s) return s;
-    c->ms_build = timer_stop(c);
+    timer_stop(c, PH_BUILD);
     return check_overflow(c);
 }
 
@@ -535,7 +559,7 @@ int csg_walk(csg_ctx *c, int precision, double G, double softening, double theta
     timer_start(c);
     int s = launch_walk(c, precision, G, softening, theta);
     if (s) return s;
-    c->ms_walk = timer_stop(c);
+    timer_stop(c, PH_WALK);
     CSG_CHECK(hipStreamSynchronize(c->stream));
     return COSMOSIM_OK;
 }
@@ -546,22 +570,21 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
     if (precision < 0 || precision > 2 || k < 0) return COSMOSIM_E_ARG;
     if (k == 0) return COSMOSIM_OK;
     int s;
-    c->ms_build = c->ms_walk = c->ms_kdk = 0.0;
     timer_start(c);
     if ((s = launch_kick_drift(c, dt, 1))) return s;
-    c->ms_kdk += timer_stop(c);
+    timer_stop(c, PH_KDK);
     for (int sub = 0; sub < k; ++sub) {
         timer_start(c);
         if ((s = launch_build(c))) return s;
-        c->ms_build += timer_stop(c);
+        timer_stop(c, PH_BUILD);
         timer_start(c);
         if ((s = launch_walk(c, precision, G, softening, theta))) return s;
-        c->ms_walk += timer_stop(c);
+        timer_stop(c, PH_WALK);
         timer_start(c);
         // end-of-substep half kick fused with the next substep's kick + drift
         s = (sub + 1 < k) ? launch_kick_drift(c, dt, 2) : launch_kick(c, dt);
         if (s) return s;
-        c->ms_kdk += timer_stop(c);
+        timer_stop(c, PH_KDK);
     }
     return check_overflow(c);
 }
@@ -714,6 +737,7 @@ int csg_get_stats(csg_ctx *c, csg_stats *o) {
     unsigned long long it = 0;
     CSG_CHECK(hipMemcpyAsync(&it, c->itotal, sizeof(it), hipMemcpyDeviceToHost, c->stream));
     CSG_CHECK(hipStreamSynchronize(c->stream));
+    collect_timers(c);
     memset(o, 0, sizeof(*o));
     o->n = c->n;
     o->n_live = c->hbp->n_live;
@@ -726,12 +750,23 @@ int csg_get_stats(csg_ctx *c, csg_stats *o) {
     o->ms_build = c->ms_build;
     o->ms_walk = c->ms_walk;
     o->ms_kdk = c->ms_kdk;
+    o->timed_walks = c->timed_walks;
     return COSMOSIM_OK;
 }
 
 int csg_set_timing(csg_ctx *c, int enabled) {
-    if (!c) return COSMOSIM_E_ARG;
+    CSG_USE(c);
+    if (enabled && !c->ev) {
+        c->ev = (hipEvent_t *)calloc(2 * CSG_EV_MAX, sizeof(hipEvent_t));
+        if (!c->ev) return COSMOSIM_E_ALLOC;
+        for (int k = 0; k < 2 * CSG_EV_MAX; ++k) CSG_CHECK(hipEventCreate(&c->ev[k]));
+    }
+    CSG_CHECK(hipStreamSynchronize(c->stream));
     c->timing = enabled ? 1 : 0;
+    c->ev_count = 0;
+    c->ev_open = -1;
+    c->ms_build = c->ms_walk = c->ms_kdk = 0.0;
+    c->timed_walks = 0;
     return COSMOSIM_OK;
 }
 

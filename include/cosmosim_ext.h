@@ -53,11 +53,11 @@ typedef struct {
     int n_nodes;           /* nodes of the last tree (reference-layout count) */
     int max_depth;         /* deepest leaf of the last tree */
     int collision_runs;    /* groups of bodies sharing a 21-level cell path in the last build */
-    int pad0;
+    int timed_walks;       /* walk launches covered by ms_walk */
     long long interactions;/* body-node interactions of the last walk */
     double root_center[3]; /* root cube of the last build */
     double root_half;
-    double ms_build;       /* GPU time of the last build / walk / KDK (0 unless timing is on) */
+    double ms_build;       /* GPU time in build / walk / KDK since timing was enabled (0 if off) */
     double ms_walk;
     double ms_kdk;
 } CosmosimExtStats;
@@ -104,7 +104,8 @@ int cosmosim_ext_energy(SimHandle h, double *kinetic, double *potential, double 
 /* Per-body interaction counts of the last walk (original body order). */
 int cosmosim_ext_interaction_counts(SimHandle h, int *counts, int n);
 int cosmosim_ext_get_stats(SimHandle h, CosmosimExtStats *out);
-/* Enable per-phase GPU event timing (adds a synchronisation per phase). */
+/* Enable and reset per-phase GPU event timing (events on the core's stream,
+ * no host waits inside the substep loop). */
 int cosmosim_ext_set_timing(SimHandle h, int enabled);
 /* Write the last tree in reference layout (root 0); capacity in nodes. */
 int cosmosim_ext_export_tree(SimHandle h, OctreeNode *pool, int capacity, int *pool_size);

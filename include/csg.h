@@ -27,7 +27,9 @@ typedef struct {
     int n, n_live, n_nodes, max_depth, collision_runs, overflow;
     long long interactions;
     double root[4];                 /* centre x, y, z and half size of the root cube */
-    double ms_build, ms_walk, ms_kdk;
+    double ms_build, ms_walk, ms_kdk; /* GPU time per phase since timing was (re)enabled */
+    int timed_walks;                /* walk launches covered by ms_walk */
+    int pad;
 } csg_stats;
 
 int  csg_device_count(int *count);
@@ -73,6 +75,8 @@ int  csg_energy(csg_ctx *ctx, double G, double softening, double *kinetic,
                 double *potential, double momentum[3]);
 int  csg_interaction_counts(csg_ctx *ctx, int *counts, int n);
 int  csg_get_stats(csg_ctx *ctx, csg_stats *out);
+/* Enable (and reset) per-phase GPU timing: hipEvents on the context's stream,
+ * recorded without host waits and collected by csg_get_stats. */
 int  csg_set_timing(csg_ctx *ctx, int enabled);
 int  csg_synchronize(csg_ctx *ctx);
 
