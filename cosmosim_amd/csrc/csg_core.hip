@@ -35,6 +35,7 @@ struct csg_ctx {
     int *cnt = nullptr, *nbase = nullptr, *scan_part = nullptr;
     DD *pref = nullptr, *dd_part = nullptr;
     WNode *nodes = nullptr;
+    WNode32 *nodes32 = nullptr;
     int *icount = nullptr;
     unsigned long long *itotal = nullptr;
     int tree_valid = 0;
@@ -52,6 +53,7 @@ struct csg_ctx {
     int ev_count = 0, ev_open = -1;
     double ms_build = 0, ms_walk = 0, ms_kdk = 0;
     int timed_walks = 0;
+    int walk_group = 8;  // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
 };
 
 #define CSG_CHECK(expr)                                                                      \
@@ -94,7 +96,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->bbox_part); dfree(c->key); dfree(c->key2); dfree(c->ext); dfree(c->ext_tmp);
     dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_base);
     dfree(c->runs); dfree(c->lcp); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
-    dfree(c->pref); dfree(c->dd_part); dfree(c->nodes); dfree(c->icount);
+    dfree(c->pref); dfree(c->dd_part); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
     c->cap = 0;
     c->node_cap = 0;
 }
@@ -115,7 +117,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->rs_tot, 256) && dalloc(c->rs_base, 256) && dalloc(c->runs, C) &&
               dalloc(c->lcp, C1) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
-              dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->nodes, 8 * C + 4096) &&
+              dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
               dalloc(c->icount, C);
     if (!ok) {
         (void)hipGetLastError();
@@ -220,6 +222,10 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     csg_ctx *c = new (std::nothrow) csg_ctx();
     if (!c) return COSMOSIM_E_ALLOC;
     c->device = device;
+    if (const char *g = getenv("COSMOSIM_WALK_GROUP")) {
+        const int v = atoi(g);
+        if (v == 8 || v == 16 || v == 32 || v == 64) c->walk_group = v;
+    }
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
@@ -474,11 +480,27 @@ static int launch_build(csg_ctx *c) {
                                       c->cap + 1);
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
         c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->x, c->y, c->z, c->m,
-        c->bp, c->nodes);
+        c->bp, c->nodes, c->nodes32);
     CSG_LAUNCHED();
     c->tree_valid = 1;
     return COSMOSIM_OK;
 }
+
+extern "C++" {
+template <int GS>
+static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, double th2, int nb8) {
+    hipStream_t st = c->stream;
+    if (precision == COSMOSIM_PREC_FP64_STRICT)
+        k_walk<WALK_FP64_STRICT, GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y,
+            c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8);
+    else if (precision == COSMOSIM_PREC_FP64)
+        k_walk<WALK_FP64, GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
+            c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8);
+    else
+        k_walk32<<<nb8, CSG_TPB, 0, st>>>(c->nodes32, c->bp, c->idx, c->x, c->y, c->z, c->ax, c->ay,
+                                          c->az, c->icount, c->itotal, G, eps2, th2, nb8);
+}
+}  // extern "C++"
 
 static int launch_walk(csg_ctx *c, int precision, double G, double softening, double theta) {
     const int n = c->n;
@@ -489,24 +511,12 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, c->icount);
     int nb8 = blocks_for(n, CSG_TPB);
     nb8 = (nb8 + 7) & ~7;
-    switch (precision) {
-    case COSMOSIM_PREC_FP64_STRICT:
-        k_walk<WALK_FP64_STRICT><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
-                                                         c->ax, c->ay, c->az, c->icount, c->itotal,
-                                                         G, eps2, th2, nb8);
-        break;
-    case COSMOSIM_PREC_FP64:
-        k_walk<WALK_FP64><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z, c->ax,
-                                                  c->ay, c->az, c->icount, c->itotal, G, eps2, th2,
-                                                  nb8);
-        break;
-    case COSMOSIM_PREC_FP32:
-        k_walk<WALK_FP32><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z, c->ax,
-                                                  c->ay, c->az, c->icount, c->itotal, G, eps2, th2,
-                                                  nb8);
-        break;
-    default:
-        return COSMOSIM_E_ARG;
+    if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
+    switch (c->walk_group) {
+    case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, nb8); break;
+    case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, nb8); break;
+    case 32: launch_walk_gs<32>(c, precision, G, eps2, th2, nb8); break;
+    default: launch_walk_gs<64>(c, precision, G, eps2, th2, nb8); break;
     }
     CSG_LAUNCHED();
     return COSMOSIM_OK;

@@ -58,6 +58,42 @@ struct __align__(64) WNode {
 };
 static_assert(sizeof(WNode) == 64, "WNode is one 64-byte record");
 
+// The same node for the FP32 walk (48 bytes).  The centre of mass relative to
+// the root-cube centre R is formed in FP64 and stored as an FP32 pair
+// (h = fp32(C - R), l = fp32((C - R) - h)), which carries it to ~1e-14; a lane
+// forms its body-node offset as (h - xh) + (l - xl), exact up to one FP32
+// rounding of the offset itself.
+struct __align__(16) WNode32 {
+    float hx, hy, hz;  // high part of C - R
+    float m;           // total mass
+    float lx, ly, lz;  // low part of C - R
+    float s2;          // (2h)^2; -1 for a leaf
+    int skip;
+    int body;
+    int pad0, pad1;
+};
+static_assert(sizeof(WNode32) == 48, "WNode32 is one 48-byte record");
+
+__device__ __forceinline__ void split_f32(double v, float &h, float &l) {
+    h = (float)v;
+    l = (float)(v - (double)h);
+}
+
+__device__ __forceinline__ void store_nodes(WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
+                                            int id, const WNode &w, const double *root) {
+    nodes[id] = w;
+    WNode32 f;
+    split_f32(w.cx - root[0], f.hx, f.lx);
+    split_f32(w.cy - root[1], f.hy, f.ly);
+    split_f32(w.cz - root[2], f.hz, f.lz);
+    f.m = (float)w.m;
+    f.s2 = w.body >= 0 ? -1.0f : (float)w.s2;  // a leaf passes every opening test
+    f.skip = w.skip;
+    f.body = w.body;
+    f.pad0 = f.pad1 = 0;
+    n32[id] = f;
+}
+
 // ------------------------------------------------------------------ helpers
 
 __device__ __forceinline__ bool live_mass(double m) { return !(m <= 0.0); }
@@ -706,11 +742,12 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const int *__restrict__ idx, const signed char *__restrict__ L, const int *__restrict__ nbase,
     const DD *__restrict__ pref, int cap1, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
-    const BuildParams *__restrict__ bp, WNode *__restrict__ nodes) {
+    const BuildParams *__restrict__ bp, WNode *__restrict__ nodes, WNode32 *__restrict__ n32) {
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
     if (bp->overflow) return;
     const double H = bp->root[3];
+    const double root[3] = {bp->root[0], bp->root[1], bp->root[2]};
     if (nl == 0) {
         if (b == 0) {
             WNode r;
@@ -719,7 +756,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
             const double s = 2.0 * H;
             r.s2 = s * s;
             r.skip = 1; r.body = -1; r.depth = 0; r.first = 0; r.last = -1; r.pad = 0;
-            nodes[0] = r;
+            store_nodes(nodes, n32, 0, r, root);
         }
         return;
     }
@@ -740,7 +777,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
         w.s2 = s * s;
         w.skip = nbase[e + 1];
         w.body = -1; w.depth = d; w.first = b; w.last = e; w.pad = 0;
-        nodes[id] = w;
+        store_nodes(nodes, n32, id, w, root);
     }
     if (lp < 64) {
         WNode w;
@@ -764,7 +801,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
         w.s2 = s * s;
         w.skip = id + 1;
         w.body = i; w.depth = depth; w.first = b; w.last = e; w.pad = 0;
-        nodes[id] = w;
+        store_nodes(nodes, n32, id, w, root);
     }
 }
 
@@ -790,7 +827,74 @@ __device__ __forceinline__ int xcd_tile(int nb8) {
     return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
+// Per-lane state of the FP64 walks.  A lane holds one body; p is the next
+// pre-order node that this body must look at (its own DFS position).
+struct WalkLane {
+    int me;              // original body index (-1 for an idle lane)
+    double x, y, z;      // body position
+    int p;
+    int cnt;
+    double sx, sy, sz;   // accumulated acceleration (without G in the fast walk)
+};
+
+// Body-node step at node i (src/octree.c:182-217): skip an empty cell, a leaf
+// interacts unless it is the body itself, an internal node is accepted
+// (interacts) or opened.  Returns whether this lane opens the node.
 template <int PREC>
+__device__ __forceinline__ bool walk_visit(WalkLane &L, const WNode &nd, int i, double G,
+                                           double eps2, double theta2) {
+    bool desc = false;
+    const double dx = nd.cx - L.x, dy = nd.cy - L.y, dz = nd.cz - L.z;
+    if constexpr (PREC == WALK_FP64_STRICT) {
+        // reference arithmetic and evaluation order, no contraction
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        bool inter;
+        if (nd.body >= 0) inter = nd.body != L.me;
+        else { inter = open_accept_exact(nd.s2, d2, theta2); desc = !inter; }
+        if (nd.m == 0.0) { inter = false; desc = false; }
+        if (inter) {
+            const double r2 = d2 + eps2;
+            const double w = 1.0 / (r2 * sqrt(r2));
+            const double gm = G * nd.m;
+            L.sx += gm * dx * w;
+            L.sy += gm * dy * w;
+            L.sz += gm * dz * w;
+            ++L.cnt;
+        }
+    } else {
+        // refined hardware rsqrt, FMA, G applied once at the end; every cell of
+        // a tree that has walkers has mass > 0
+        const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        bool inter;
+        if (nd.body >= 0) inter = nd.body != L.me;
+        else { inter = open_accept_exact(nd.s2, d2, theta2); desc = !inter; }
+        if (inter) {
+            const double r2 = d2 + eps2;
+            double yv = __builtin_amdgcn_rsq(r2);
+            double e = fma(-r2 * yv, yv, 1.0);
+            yv = fma(0.5 * yv, e, yv);
+            e = fma(-r2 * yv, yv, 1.0);
+            yv = fma(0.5 * yv, e, yv);
+            const double f = nd.m * (yv * yv * yv);
+            L.sx = fma(dx, f, L.sx);
+            L.sy = fma(dy, f, L.sy);
+            L.sz = fma(dz, f, L.sz);
+            ++L.cnt;
+        }
+    }
+    L.p = desc ? i + 1 : nd.skip;
+    return desc;
+}
+
+// FP64 walks.  One wavefront per 64 consecutive sorted (Morton-adjacent)
+// bodies, split into 64/GS traversal groups of GS lanes.  A group walks the
+// pre-order node array with one shared index i: it visits node i when any of
+// its lanes has p == i, and moves to i+1 when any lane opened the node, else to
+// skip[i].  Lanes never visit a node outside their own DFS sequence, so each
+// body's interactions and their summation order are the reference's.  GS = 64
+// keeps i in a scalar register and fetches nodes with scalar loads; smaller
+// groups keep several fetches in flight per wave.
+template <int PREC, int GS>
 __global__ __launch_bounds__(CSG_TPB) void k_walk(
     const WNode *__restrict__ nodes, const BuildParams *__restrict__ bp,
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
@@ -806,92 +910,125 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk(
     if (b0 >= n_live) return;
     const int b = b0 + lane;
     const bool act = b < n_live;
+    WalkLane L;
+    L.me = act ? sidx[b] : -1;
+    L.x = act ? px[L.me] : 0.0;
+    L.y = act ? py[L.me] : 0.0;
+    L.z = act ? pz[L.me] : 0.0;
+    L.p = act ? 0 : 0x7fffffff;
+    L.cnt = 0;
+    L.sx = L.sy = L.sz = 0.0;
+    const unsigned long long gmask =
+        (GS == 64) ? ~0ull : (((1ull << (GS & 63)) - 1ull) << (lane & ~(GS - 1)));
+    int i = 0;
+    while (true) {
+        int ii = i;
+        if constexpr (GS == 64) ii = __builtin_amdgcn_readfirstlane(i);
+        const bool alive = ii < nnodes;
+        if (!__any(alive)) break;
+        bool desc = false;
+        int skip = ii + 1;
+        if (alive) {
+            const WNode nd = nodes[ii];
+            skip = nd.skip;
+            if (L.p == ii) desc = walk_visit<PREC>(L, nd, ii, G, eps2, theta2);
+        }
+        const bool gdesc = (__ballot(desc) & gmask) != 0ull;
+        const int nxt = gdesc ? ii + 1 : skip;
+        i = nxt > ii ? nxt : ii + 1;  // strictly increasing: the loop always ends
+    }
+    if (act) {
+        if constexpr (PREC == WALK_FP64_STRICT) {
+            ax[L.me] = L.sx; ay[L.me] = L.sy; az[L.me] = L.sz;
+        } else {
+            ax[L.me] = G * L.sx; ay[L.me] = G * L.sy; az[L.me] = G * L.sz;
+        }
+        if (icount) icount[L.me] = L.cnt;
+    }
+    unsigned long long c = (unsigned long long)L.cnt;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) atomicAdd(itotal, c);
+}
+
+// Kahan step: add t to (s, k), k holding the negated running error.
+__device__ __forceinline__ void kahan_add(float &s, float &k, float t) {
+    const float y = t - k;
+    const float u = s + y;
+    k = (u - s) - y;
+    s = u;
+}
+
+#define CSG_FOLD_EVERY 16
+
+// FP32 walk over WNode32 records, same traversal as k_walk with GS = 64.
+// Offsets: the body's position relative to the root centre is split into an
+// FP32 pair once (formed in FP64), so d = (h_node - h_body) + (l_node - l_body).
+// Accumulation is compensated in blocks: each lane adds its terms in FP32 into
+// a block sum, and every CSG_FOLD_EVERY node steps (a wave-uniform count) the
+// block sums are Kahan-added into the running sums.  G multiplies the result.
+__global__ __launch_bounds__(CSG_TPB) void k_walk32(
+    const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
+    const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
+    double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
+    double G, double eps2, double theta2, int nb8) {
+    const int n_live = bp->n_live;
+    const int nnodes = bp->n_nodes;
+    if (bp->overflow) return;
+    const int lane = threadIdx.x & 63;
+    const int tile = xcd_tile(nb8);
+    const int b0 = (tile * CSG_TPB + (int)(threadIdx.x & ~63));
+    if (b0 >= n_live) return;
+    const int b = b0 + lane;
+    const bool act = b < n_live;
     const int me = act ? sidx[b] : -1;
-    const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
+    float xh, xl, yh, yl, zh, zl;
+    split_f32(act ? px[me] - bp->root[0] : 0.0, xh, xl);
+    split_f32(act ? py[me] - bp->root[1] : 0.0, yh, yl);
+    split_f32(act ? pz[me] - bp->root[2] : 0.0, zh, zl);
+    const float eps2f = (float)eps2, th2f = (float)theta2;
     int p = act ? 0 : 0x7fffffff;
     int cnt = 0;
-    int i = 0;
-    if constexpr (PREC == WALK_FP32) {
-        // FP32 offsets from a per-wave anchor formed in FP64.
-        const double x0 = __shfl(x, 0, 64), y0 = __shfl(y, 0, 64), z0 = __shfl(z, 0, 64);
-        const float xr = (float)(x - x0), yr = (float)(y - y0), zr = (float)(z - z0);
-        const float eps2f = (float)eps2, th2f = (float)theta2;
-        float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-        while (i < nnodes) {
-            const WNode nd = nodes[i];
-            bool desc = false;
-            if (p == i) {
-                const float dx = (float)(nd.cx - x0) - xr;
-                const float dy = (float)(nd.cy - y0) - yr;
-                const float dz = (float)(nd.cz - z0) - zr;
-                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                bool inter;
-                if (nd.body >= 0) inter = nd.body != me;
-                else { inter = (d2 > 0.f) && ((float)nd.s2 < th2f * d2); desc = !inter; }
-                if (nd.m == 0.0) { inter = false; desc = false; }
-                if (inter) {
-                    const float r2 = d2 + eps2f;
-                    const float yv = __builtin_amdgcn_rsqf(r2);
-                    const float f = (float)(G * nd.m) * (yv * yv * yv);
-                    const float tx = dx * f - cx, ty = dy * f - cy, tz = dz * f - cz;
-                    const float ux = sx + tx, uy = sy + ty, uz = sz + tz;
-                    cx = (ux - sx) - tx; cy = (uy - sy) - ty; cz = (uz - sz) - tz;
-                    sx = ux; sy = uy; sz = uz;
-                    ++cnt;
-                }
-                p = desc ? i + 1 : nd.skip;
-            }
-            const int nxt = __any(desc) ? i + 1 : nd.skip;
-            i = nxt > i ? nxt : i + 1;  // strictly increasing: the loop always ends
+    float bx = 0.f, by = 0.f, bz = 0.f;               // block sums
+    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+    int i = 0, steps = 0;
+    while (true) {
+        const int ii = __builtin_amdgcn_readfirstlane(i);
+        if (ii >= nnodes) break;
+        const WNode32 nd = nodes[ii];
+        // branch-free step: lanes whose DFS position is not ii compute and discard
+        const bool here = p == ii;
+        const float dx = (nd.hx - xh) + (nd.lx - xl);
+        const float dy = (nd.hy - yh) + (nd.ly - yl);
+        const float dz = (nd.hz - zh) + (nd.lz - zl);
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        // leaves carry s2 = -1, so crit holds for them and only self is skipped
+        const bool crit = nd.s2 < th2f * d2;
+        const bool inter = here && crit && nd.body != me;
+        const bool desc = here && !crit;
+        const float yv = __builtin_amdgcn_rsqf(d2 + eps2f);
+        const float f = inter ? nd.m * (yv * yv * yv) : 0.0f;
+        bx = fmaf(dx, f, bx);
+        by = fmaf(dy, f, by);
+        bz = fmaf(dz, f, bz);
+        cnt += inter ? 1 : 0;
+        if (here) p = desc ? ii + 1 : nd.skip;
+        if (++steps == CSG_FOLD_EVERY) {
+            steps = 0;
+            kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
+            bx = by = bz = 0.f;
         }
-        if (act) {
-            ax[me] = (double)sx - (double)cx;
-            ay[me] = (double)sy - (double)cy;
-            az[me] = (double)sz - (double)cz;
-        }
-    } else {
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        while (i < nnodes) {
-            const WNode nd = nodes[i];
-            bool desc = false;
-            if (p == i) {
-                const double dx = nd.cx - x, dy = nd.cy - y, dz = nd.cz - z;
-                double d2;
-                if constexpr (PREC == WALK_FP64_STRICT) d2 = dx * dx + dy * dy + dz * dz;
-                else d2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                bool inter;
-                if (nd.body >= 0) inter = nd.body != me;
-                else { inter = open_accept_exact(nd.s2, d2, theta2); desc = !inter; }
-                if (nd.m == 0.0) { inter = false; desc = false; }
-                if (inter) {
-                    const double r2 = d2 + eps2;
-                    const double gm = G * nd.m;
-                    if constexpr (PREC == WALK_FP64_STRICT) {
-                        const double w = 1.0 / (r2 * sqrt(r2));
-                        sx += gm * dx * w;
-                        sy += gm * dy * w;
-                        sz += gm * dz * w;
-                    } else {
-                        double yv = __builtin_amdgcn_rsq(r2);
-                        double e = fma(-r2 * yv, yv, 1.0);
-                        yv = fma(0.5 * yv, e, yv);
-                        e = fma(-r2 * yv, yv, 1.0);
-                        yv = fma(0.5 * yv, e, yv);
-                        const double f = gm * (yv * yv * yv);
-                        sx = fma(dx, f, sx);
-                        sy = fma(dy, f, sy);
-                        sz = fma(dz, f, sz);
-                    }
-                    ++cnt;
-                }
-                p = desc ? i + 1 : nd.skip;
-            }
-            const int nxt = __any(desc) ? i + 1 : nd.skip;
-            i = nxt > i ? nxt : i + 1;  // strictly increasing: the loop always ends
-        }
-        if (act) { ax[me] = sx; ay[me] = sy; az[me] = sz; }
+        const int nxt = __any(desc) ? ii + 1 : nd.skip;
+        i = nxt > ii ? nxt : ii + 1;  // strictly increasing: the loop always ends
     }
-    if (act && icount) icount[me] = cnt;
+    kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
+    if (act) {
+        ax[me] = G * ((double)sx - (double)kx);
+        ay[me] = G * ((double)sy - (double)ky);
+        az[me] = G * ((double)sz - (double)kz);
+        if (icount) icount[me] = cnt;
+    }
     unsigned long long c = (unsigned long long)cnt;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
