@@ -29,7 +29,9 @@ struct csg_ctx {
     double *bbox_part = nullptr;
     uint64_t *key = nullptr, *key2 = nullptr, *ext = nullptr, *ext_tmp = nullptr;
     int *idx = nullptr, *idx2 = nullptr;
-    int *rs_hist = nullptr, *rs_tot = nullptr, *rs_base = nullptr;
+    int *rs_hist = nullptr, *rs_tot = nullptr;
+    double *xs = nullptr, *ys = nullptr, *zs = nullptr, *ms = nullptr;  // sorted copies
+    int *depth_part = nullptr;
     int *runs = nullptr;
     signed char *lcp = nullptr;
     int *cnt = nullptr, *nbase = nullptr, *scan_part = nullptr;
@@ -53,7 +55,9 @@ struct csg_ctx {
     int ev_count = 0, ev_open = -1;
     double ms_build = 0, ms_walk = 0, ms_kdk = 0;
     int timed_walks = 0;
-    int walk_group = 8;  // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
+    int walk_group = 8;     // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
+    int walk_group32 = 64;  // FP32 walk
+    int xcd_chunk = 16;     // walk tiles per XCD chunk (0: one contiguous range per XCD)
 };
 
 #define CSG_CHECK(expr)                                                                      \
@@ -94,7 +98,8 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->vx); dfree(c->vy); dfree(c->vz);
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->m); dfree(c->aos);
     dfree(c->bbox_part); dfree(c->key); dfree(c->key2); dfree(c->ext); dfree(c->ext_tmp);
-    dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_base);
+    dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
+    dfree(c->depth_part);
     dfree(c->runs); dfree(c->lcp); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
     c->cap = 0;
@@ -114,7 +119,8 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->bbox_part, (size_t)CSG_BBOX_BLOCKS * 8) && dalloc(c->key, C) &&
               dalloc(c->key2, C) && dalloc(c->ext, 3 * C) && dalloc(c->ext_tmp, 3 * C) &&
               dalloc(c->idx, C) && dalloc(c->idx2, C) && dalloc(c->rs_hist, (size_t)256 * nb) &&
-              dalloc(c->rs_tot, 256) && dalloc(c->rs_base, 256) && dalloc(c->runs, C) &&
+              dalloc(c->rs_tot, 256) && dalloc(c->xs, C) && dalloc(c->ys, C) && dalloc(c->zs, C) &&
+              dalloc(c->ms, C) && dalloc(c->depth_part, (size_t)blocks_for((long long)C1, CSG_TPB)) && dalloc(c->runs, C) &&
               dalloc(c->lcp, C1) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
               dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
@@ -225,6 +231,11 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     if (const char *g = getenv("COSMOSIM_WALK_GROUP")) {
         const int v = atoi(g);
         if (v == 8 || v == 16 || v == 32 || v == 64) c->walk_group = v;
+    }
+    if (const char *g = getenv("COSMOSIM_XCD_CHUNK")) c->xcd_chunk = atoi(g);
+    if (const char *g = getenv("COSMOSIM_WALK_GROUP32")) {
+        const int v = atoi(g);
+        if (v == 8 || v == 16 || v == 32 || v == 64) c->walk_group32 = v;
     }
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -455,9 +466,8 @@ static int launch_build(csg_ctx *c) {
             const int shift = 8 * pass;
             k_rs_hist<<<nb, CSG_TPB, 0, st>>>(kin, n, shift, c->rs_hist, nb);
             k_rs_rowscan<<<256, CSG_TPB, 0, st>>>(c->rs_hist, nb, c->rs_tot);
-            k_rs_binscan<<<1, CSG_TPB, 0, st>>>(c->rs_tot, c->rs_base);
             k_rs_scatter<<<nb, CSG_TPB, 0, st>>>(kin, vin, kout, vout, n, shift, c->rs_hist,
-                                                 c->rs_base, nb);
+                                                 c->rs_tot, nb);
             uint64_t *tk = kin; kin = kout; kout = tk;
             int *tv = vin; vin = vout; vout = tv;
         }
@@ -469,17 +479,22 @@ static int launch_build(csg_ctx *c) {
         CSG_LAUNCHED();
     }
     const int n1 = n + 1;
-    k_lcp<<<blocks_for(n1, CSG_TPB), CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt);
+    if (n > 0)
+        k_gather_sorted<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, c->x, c->y, c->z, c->m,
+                                                                   c->bp, c->xs, c->ys, c->zs,
+                                                                   c->ms);
+    const int nlb = blocks_for(n1, CSG_TPB);
+    k_lcp<<<nlb, CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt, c->depth_part);
     const int nt = blocks_for(n1, CSG_TILE);
     k_iscan_reduce<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->bp, c->scan_part);
-    k_iscan_top<<<1, CSG_TPB, 0, st>>>(c->scan_part, nt);
+    k_iscan_top<<<1, CSG_TPB, 0, st>>>(c->scan_part, nt, c->depth_part, nlb, c->bp);
     k_iscan_down<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->scan_part, c->bp, c->nbase);
-    k_dd_reduce<<<nt, CSG_TPB, 0, st>>>(c->idx, c->x, c->y, c->z, c->m, c->bp, c->dd_part);
+    k_dd_reduce<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part);
     k_dd_top<<<1, CSG_TPB, 0, st>>>(c->dd_part, nt);
-    k_dd_down<<<nt, CSG_TPB, 0, st>>>(c->idx, c->x, c->y, c->z, c->m, c->bp, c->dd_part, c->pref,
+    k_dd_down<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part, c->pref,
                                       c->cap + 1);
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
-        c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->x, c->y, c->z, c->m,
+        c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->xs, c->ys, c->zs, c->ms,
         c->bp, c->nodes, c->nodes32);
     CSG_LAUNCHED();
     c->tree_valid = 1;
@@ -492,13 +507,13 @@ static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, dou
     hipStream_t st = c->stream;
     if (precision == COSMOSIM_PREC_FP64_STRICT)
         k_walk<WALK_FP64_STRICT, GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y,
-            c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8);
+            c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk);
     else if (precision == COSMOSIM_PREC_FP64)
         k_walk<WALK_FP64, GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
-            c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8);
+            c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk);
     else
-        k_walk32<<<nb8, CSG_TPB, 0, st>>>(c->nodes32, c->bp, c->idx, c->x, c->y, c->z, c->ax, c->ay,
-                                          c->az, c->icount, c->itotal, G, eps2, th2, nb8);
+        k_walk32<GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes32, c->bp, c->idx, c->x, c->y, c->z, c->ax, c->ay,
+                                          c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk);
 }
 }  // extern "C++"
 
@@ -510,9 +525,11 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     if (n == 0) return COSMOSIM_OK;
     k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, c->icount);
     int nb8 = blocks_for(n, CSG_TPB);
-    nb8 = (nb8 + 7) & ~7;
+    const int gran = c->xcd_chunk > 0 ? 8 * c->xcd_chunk : 8;
+    nb8 = (nb8 + gran - 1) / gran * gran;
     if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
-    switch (c->walk_group) {
+    const int gs = precision == COSMOSIM_PREC_FP32 ? c->walk_group32 : c->walk_group;
+    switch (gs) {
     case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, nb8); break;
     case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, nb8); break;
     case 32: launch_walk_gs<32>(c, precision, G, eps2, th2, nb8); break;

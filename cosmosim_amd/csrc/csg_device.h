@@ -397,21 +397,19 @@ __global__ __launch_bounds__(CSG_TPB) void k_rs_rowscan(int *__restrict__ hist, 
     if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(CSG_TPB) void k_rs_binscan(const int *__restrict__ tot,
-                                                        int *__restrict__ binbase) {
-    __shared__ int lds4[4];
-    int total;
-    const int ex = block_excl_scan(tot[threadIdx.x], lds4, total);
-    binbase[threadIdx.x] = ex;
-}
-
 __global__ __launch_bounds__(CSG_TPB) void k_rs_scatter(
     const uint64_t *__restrict__ kin, const int *__restrict__ vin, uint64_t *__restrict__ kout,
     int *__restrict__ vout, int n, int shift, const int *__restrict__ hist,
-    const int *__restrict__ binbase, int nb) {
+    const int *__restrict__ tot, int nb) {
     __shared__ int wcnt[4][256];
+    __shared__ int binbase[256];
+    __shared__ int lds4[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int k = threadIdx.x; k < 4 * 256; k += CSG_TPB) (&wcnt[0][0])[k] = 0;
+    {
+        int total;
+        binbase[threadIdx.x] = block_excl_scan(tot[threadIdx.x], lds4, total);
+    }
     __syncthreads();
     const int base = blockIdx.x * CSG_TILE + w * (64 * CSG_ITEMS);
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -539,9 +537,12 @@ __global__ __launch_bounds__(CSG_TPB) void k_runs_fix(
 // opens the internal nodes of depths (min(L[b-1],63), min(L[b],63)] and starts
 // a leaf unless it shares all 64 levels with b-1.
 __global__ __launch_bounds__(CSG_TPB) void k_lcp(const uint64_t *__restrict__ key,
-                                                 const uint64_t *__restrict__ ext, BuildParams *bp,
+                                                 const uint64_t *__restrict__ ext,
+                                                 const BuildParams *__restrict__ bp,
                                                  signed char *__restrict__ L,
-                                                 int *__restrict__ cnt) {
+                                                 int *__restrict__ cnt,
+                                                 int *__restrict__ depth_part) {
+    __shared__ int wmax[4];
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
     int leafdepth = 0;
@@ -557,7 +558,22 @@ __global__ __launch_bounds__(CSG_TPB) void k_lcp(const uint64_t *__restrict__ ke
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) leafdepth = max(leafdepth, __shfl_xor(leafdepth, o, 64));
-    if ((threadIdx.x & 63) == 0 && leafdepth > 0) atomicMax(&bp->max_depth, leafdepth);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = leafdepth;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        depth_part[blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+}
+
+// Sorted copies of position and mass (coalesced reads for the scans and emit).
+__global__ __launch_bounds__(CSG_TPB) void k_gather_sorted(
+    const int *__restrict__ idx, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, const BuildParams *__restrict__ bp,
+    double *__restrict__ xs, double *__restrict__ ys, double *__restrict__ zs,
+    double *__restrict__ ms) {
+    const int b = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (b >= bp->n_live) return;
+    const int i = idx[b];
+    xs[b] = x[i]; ys[b] = y[i]; zs[b] = z[i]; ms[b] = m[i];
 }
 
 // ============================================================ int scan
@@ -576,7 +592,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_iscan_reduce(const int *__restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(CSG_TPB) void k_iscan_top(int *__restrict__ part, int nb) {
+__global__ __launch_bounds__(CSG_TPB) void k_iscan_top(int *__restrict__ part, int nb,
+                                                       const int *__restrict__ depth_part,
+                                                       int ndp, BuildParams *bp) {
     __shared__ int lds4[4];
     int carry = 0;
     for (int c0 = 0; c0 < nb; c0 += CSG_TPB) {
@@ -587,6 +605,15 @@ __global__ __launch_bounds__(CSG_TPB) void k_iscan_top(int *__restrict__ part, i
         if (i < nb) part[i] = carry + ex;
         carry += total;
     }
+    // deepest leaf of the tree
+    int d = 0;
+    for (int k = threadIdx.x; k < ndp; k += CSG_TPB) d = max(d, depth_part[k]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) d = max(d, __shfl_xor(d, o, 64));
+    __shared__ int wmax[4];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) bp->max_depth = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
 }
 
 __global__ __launch_bounds__(CSG_TPB) void k_iscan_down(const int *__restrict__ in,
@@ -617,114 +644,125 @@ __global__ __launch_bounds__(CSG_TPB) void k_iscan_down(const int *__restrict__ 
 
 // ====================================================== double-double scan
 // Exclusive prefix P[b] over sorted bodies of (m, m*x, m*y, m*z), each a DD;
-// P[n_live] is the total.  Layout: pref[c*(cap1) + b], c = 0..3.
-__device__ __forceinline__ void dd_elem(const int *__restrict__ idx, const double *__restrict__ x,
-                                        const double *__restrict__ y,
-                                        const double *__restrict__ z,
-                                        const double *__restrict__ m, int b, int nl, DD v[4]) {
+// P[n_live] is the total.  Layout: pref[c*(cap1) + b], c = 0..3.  Every sum is
+// formed in a fixed order (thread-serial over 8 elements, then a wave shuffle
+// scan, then the 4 waves in order), so the result is reproducible.
+__device__ __forceinline__ void dd_elem(const double *__restrict__ xs,
+                                        const double *__restrict__ ys,
+                                        const double *__restrict__ zs,
+                                        const double *__restrict__ ms, int b, int nl, DD v[4]) {
     if (b < nl) {
-        const int i = idx[b];
-        const double mm = m[i];
+        const double mm = ms[b];
         v[0] = DD{mm, 0.0};
-        v[1] = dd_prod(mm, x[i]);
-        v[2] = dd_prod(mm, y[i]);
-        v[3] = dd_prod(mm, z[i]);
+        v[1] = dd_prod(mm, xs[b]);
+        v[2] = dd_prod(mm, ys[b]);
+        v[3] = dd_prod(mm, zs[b]);
     } else {
         for (int c = 0; c < 4; ++c) v[c] = DD{0.0, 0.0};
     }
 }
 
-// Block-wide inclusive DD scan of one value per thread (Hillis-Steele, fixed order).
-__device__ void block_dd_incl_scan(DD v[4], DD (*lds)[CSG_TPB]) {
-    for (int o = 1; o < CSG_TPB; o <<= 1) {
-        for (int c = 0; c < 4; ++c) lds[c][threadIdx.x] = v[c];
-        __syncthreads();
-        if ((int)threadIdx.x >= o)
-            for (int c = 0; c < 4; ++c) v[c] = dd_add(lds[c][threadIdx.x - o], v[c]);
-        __syncthreads();
+// Block-wide inclusive DD scan of one value per thread; `wtot` is LDS for the
+// four wave totals.
+__device__ __forceinline__ void block_dd_incl_scan(DD v[4], DD (*wtot)[4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const DD t{__shfl_up(v[c].h, o, 64), __shfl_up(v[c].l, o, 64)};
+            if (lane >= o) v[c] = dd_add(t, v[c]);
+        }
+    }
+    if (lane == 63)
+        for (int c = 0; c < 4; ++c) wtot[w][c] = v[c];
+    __syncthreads();
+    if (w > 0) {
+        for (int c = 0; c < 4; ++c) {
+            DD off = wtot[0][c];
+            for (int k = 1; k < w; ++k) off = dd_add(off, wtot[k][c]);
+            v[c] = dd_add(off, v[c]);
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void dd_thread_sum(const double *__restrict__ xs,
+                                              const double *__restrict__ ys,
+                                              const double *__restrict__ zs,
+                                              const double *__restrict__ ms, int base, int nl,
+                                              DD s[4]) {
+    for (int c = 0; c < 4; ++c) s[c] = DD{0.0, 0.0};
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        DD e[4];
+        dd_elem(xs, ys, zs, ms, base + r, nl, e);
+        for (int c = 0; c < 4; ++c) s[c] = dd_add(s[c], e[c]);
     }
 }
 
-__global__ __launch_bounds__(CSG_TPB) void k_dd_reduce(const int *__restrict__ idx,
-                                                       const double *__restrict__ x,
-                                                       const double *__restrict__ y,
-                                                       const double *__restrict__ z,
-                                                       const double *__restrict__ m,
+__global__ __launch_bounds__(CSG_TPB) void k_dd_reduce(const double *__restrict__ xs,
+                                                       const double *__restrict__ ys,
+                                                       const double *__restrict__ zs,
+                                                       const double *__restrict__ ms,
                                                        const BuildParams *__restrict__ bp,
                                                        DD *__restrict__ part) {
-    __shared__ DD lds[4][CSG_TPB];
+    __shared__ DD wtot[4][4];
     const int nl = bp->n_live;
-    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
-    DD s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    for (int r = 0; r < CSG_ITEMS; ++r) {
-        DD e[4];
-        dd_elem(idx, x, y, z, m, base + r, nl, e);
-        for (int c = 0; c < 4; ++c) s[c] = dd_add(s[c], e[c]);
-    }
-    block_dd_incl_scan(s, lds);
+    DD s[4];
+    dd_thread_sum(xs, ys, zs, ms, blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS, nl, s);
+    block_dd_incl_scan(s, wtot);
     if (threadIdx.x == CSG_TPB - 1)
         for (int c = 0; c < 4; ++c) part[blockIdx.x * 4 + c] = s[c];
 }
 
 // Exclusive scan of the per-block DD totals (one block, chunks of 256).
 __global__ __launch_bounds__(CSG_TPB) void k_dd_top(DD *__restrict__ part, int nb) {
-    __shared__ DD lds[4][CSG_TPB];
+    __shared__ DD wtot[4][4];
+    __shared__ DD prev[CSG_TPB][4];
     DD carry[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     for (int c0 = 0; c0 < nb; c0 += CSG_TPB) {
         const int i = c0 + threadIdx.x;
-        DD v[4], orig[4];
-        for (int c = 0; c < 4; ++c) v[c] = orig[c] = (i < nb) ? part[i * 4 + c] : DD{0, 0};
-        block_dd_incl_scan(v, lds);
-        // exclusive = carry + (inclusive - own) computed as carry + inclusive of previous thread
-        __shared__ DD last[4];
-        for (int c = 0; c < 4; ++c) lds[c][threadIdx.x] = v[c];
-        __syncthreads();
-        DD ex[4];
-        for (int c = 0; c < 4; ++c)
-            ex[c] = dd_add(carry[c], threadIdx.x > 0 ? lds[c][threadIdx.x - 1] : DD{0, 0});
-        if (threadIdx.x == CSG_TPB - 1)
-            for (int c = 0; c < 4; ++c) last[c] = v[c];
+        DD v[4];
+        for (int c = 0; c < 4; ++c) v[c] = (i < nb) ? part[i * 4 + c] : DD{0, 0};
+        block_dd_incl_scan(v, wtot);
+        for (int c = 0; c < 4; ++c) prev[threadIdx.x][c] = v[c];
         __syncthreads();
         if (i < nb)
-            for (int c = 0; c < 4; ++c) part[i * 4 + c] = ex[c];
-        for (int c = 0; c < 4; ++c) carry[c] = dd_add(carry[c], last[c]);
+            for (int c = 0; c < 4; ++c)
+                part[i * 4 + c] =
+                    threadIdx.x > 0 ? dd_add(carry[c], prev[threadIdx.x - 1][c]) : carry[c];
+        for (int c = 0; c < 4; ++c) carry[c] = dd_add(carry[c], prev[CSG_TPB - 1][c]);
         __syncthreads();
-        (void)orig;
     }
 }
 
-__global__ __launch_bounds__(CSG_TPB) void k_dd_down(const int *__restrict__ idx,
-                                                     const double *__restrict__ x,
-                                                     const double *__restrict__ y,
-                                                     const double *__restrict__ z,
-                                                     const double *__restrict__ m,
+__global__ __launch_bounds__(CSG_TPB) void k_dd_down(const double *__restrict__ xs,
+                                                     const double *__restrict__ ys,
+                                                     const double *__restrict__ zs,
+                                                     const double *__restrict__ ms,
                                                      const BuildParams *__restrict__ bp,
                                                      const DD *__restrict__ part,
                                                      DD *__restrict__ pref, int cap1) {
-    __shared__ DD lds[4][CSG_TPB];
+    __shared__ DD wtot[4][4];
+    __shared__ DD prev[CSG_TPB][4];
     const int nl = bp->n_live;
     const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
-    DD s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    for (int r = 0; r < CSG_ITEMS; ++r) {
-        DD e[4];
-        dd_elem(idx, x, y, z, m, base + r, nl, e);
-        for (int c = 0; c < 4; ++c) s[c] = dd_add(s[c], e[c]);
-    }
-    DD inc[4] = {s[0], s[1], s[2], s[3]};
-    block_dd_incl_scan(inc, lds);
-    for (int c = 0; c < 4; ++c) lds[c][threadIdx.x] = inc[c];
+    DD s[4];
+    dd_thread_sum(xs, ys, zs, ms, base, nl, s);
+    block_dd_incl_scan(s, wtot);
+    for (int c = 0; c < 4; ++c) prev[threadIdx.x][c] = s[c];
     __syncthreads();
     DD run[4];
     for (int c = 0; c < 4; ++c)
-        run[c] = dd_add(part[blockIdx.x * 4 + c],
-                        threadIdx.x > 0 ? lds[c][threadIdx.x - 1] : DD{0, 0});
+        run[c] = threadIdx.x > 0 ? dd_add(part[blockIdx.x * 4 + c], prev[threadIdx.x - 1][c])
+                                 : part[blockIdx.x * 4 + c];
     for (int r = 0; r < CSG_ITEMS; ++r) {
         const int b = base + r;
         if (b <= nl)
             for (int c = 0; c < 4; ++c) pref[(size_t)c * cap1 + b] = run[c];
         if (b < nl) {
             DD e[4];
-            dd_elem(idx, x, y, z, m, b, nl, e);
+            dd_elem(xs, ys, zs, ms, b, nl, e);
             for (int c = 0; c < 4; ++c) run[c] = dd_add(run[c], e[c]);
         }
     }
@@ -740,8 +778,8 @@ __device__ __forceinline__ double dd_range(const DD *__restrict__ pref, int cap1
 __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext,
     const int *__restrict__ idx, const signed char *__restrict__ L, const int *__restrict__ nbase,
-    const DD *__restrict__ pref, int cap1, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
+    const DD *__restrict__ pref, int cap1, const double *__restrict__ xs,
+    const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ms,
     const BuildParams *__restrict__ bp, WNode *__restrict__ nodes, WNode32 *__restrict__ n32) {
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
@@ -794,8 +832,8 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
             w.m = M;
         } else {
             depth = max(lp, lc) + 1;
-            w.cx = x[i]; w.cy = y[i]; w.cz = z[i];
-            w.m = m[i];
+            w.cx = xs[b]; w.cy = ys[b]; w.cz = zs[b];
+            w.m = ms[b];
         }
         const double s = 2.0 * ldexp(H, -depth);
         w.s2 = s * s;
@@ -820,11 +858,18 @@ __device__ __forceinline__ bool open_accept_exact(double s2, double d2, double t
     return (s2 / d2) < theta2;
 }
 
-// Work-group -> tile mapping that keeps consecutive tiles on one XCD (work
-// groups are dealt round-robin over the 8 XCDs, each with its own L2).
-__device__ __forceinline__ int xcd_tile(int nb8) {
-    const int per = nb8 >> 3;
-    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+// Work-group -> tile mapping.  Work groups are dealt round-robin over the 8
+// XCDs (each with its own L2): block b runs on XCD b % 8.  Consecutive tiles
+// are kept on one XCD in chunks of `chunk` tiles, so neighbouring waves share
+// that XCD's L2, while the chunks of one XCD are spread over the whole body
+// range to balance dense and sparse regions.  chunk == 0: one contiguous range
+// per XCD; chunk < 0: tile = block.  The grid is a multiple of 8 * chunk;
+// surplus tiles exit.
+__device__ __forceinline__ int xcd_tile(int grid, int chunk) {
+    const int x = blockIdx.x & 7, q = blockIdx.x >> 3;
+    if (chunk < 0) return blockIdx.x;
+    if (chunk == 0) return x * (grid >> 3) + q;
+    return (q / chunk) * (8 * chunk) + x * chunk + (q % chunk);
 }
 
 // Per-lane state of the FP64 walks.  A lane holds one body; p is the next
@@ -900,12 +945,12 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk(
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
     const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
     double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
-    double G, double eps2, double theta2, int nb8) {
+    double G, double eps2, double theta2, int nb8, int chunk) {
     const int n_live = bp->n_live;
     const int nnodes = bp->n_nodes;
     if (bp->overflow) return;
     const int lane = threadIdx.x & 63;
-    const int tile = xcd_tile(nb8);
+    const int tile = xcd_tile(nb8, chunk);
     const int b0 = (tile * CSG_TPB + (int)(threadIdx.x & ~63));
     if (b0 >= n_live) return;
     const int b = b0 + lane;
@@ -961,23 +1006,24 @@ __device__ __forceinline__ void kahan_add(float &s, float &k, float t) {
 
 #define CSG_FOLD_EVERY 16
 
-// FP32 walk over WNode32 records, same traversal as k_walk with GS = 64.
+// FP32 walk over WNode32 records, same traversal as k_walk (groups of GS lanes).
 // Offsets: the body's position relative to the root centre is split into an
 // FP32 pair once (formed in FP64), so d = (h_node - h_body) + (l_node - l_body).
 // Accumulation is compensated in blocks: each lane adds its terms in FP32 into
 // a block sum, and every CSG_FOLD_EVERY node steps (a wave-uniform count) the
 // block sums are Kahan-added into the running sums.  G multiplies the result.
+template <int GS>
 __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
     const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
     double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
-    double G, double eps2, double theta2, int nb8) {
+    double G, double eps2, double theta2, int nb8, int chunk) {
     const int n_live = bp->n_live;
     const int nnodes = bp->n_nodes;
     if (bp->overflow) return;
     const int lane = threadIdx.x & 63;
-    const int tile = xcd_tile(nb8);
+    const int tile = xcd_tile(nb8, chunk);
     const int b0 = (tile * CSG_TPB + (int)(threadIdx.x & ~63));
     if (b0 >= n_live) return;
     const int b = b0 + lane;
@@ -988,17 +1034,25 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     split_f32(act ? py[me] - bp->root[1] : 0.0, yh, yl);
     split_f32(act ? pz[me] - bp->root[2] : 0.0, zh, zl);
     const float eps2f = (float)eps2, th2f = (float)theta2;
+    const unsigned long long gmask =
+        (GS == 64) ? ~0ull : (((1ull << (GS & 63)) - 1ull) << (lane & ~(GS - 1)));
     int p = act ? 0 : 0x7fffffff;
     int cnt = 0;
     float bx = 0.f, by = 0.f, bz = 0.f;               // block sums
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     int i = 0, steps = 0;
     while (true) {
-        const int ii = __builtin_amdgcn_readfirstlane(i);
-        if (ii >= nnodes) break;
-        const WNode32 nd = nodes[ii];
+        int ii = i;
+        if constexpr (GS == 64) {
+            ii = __builtin_amdgcn_readfirstlane(i);
+            if (ii >= nnodes) break;
+        } else {
+            if (!__any(ii < nnodes)) break;
+        }
+        const bool alive = ii < nnodes;
+        const WNode32 nd = nodes[alive ? ii : 0];
         // branch-free step: lanes whose DFS position is not ii compute and discard
-        const bool here = p == ii;
+        const bool here = alive && p == ii;
         const float dx = (nd.hx - xh) + (nd.lx - xl);
         const float dy = (nd.hy - yh) + (nd.ly - yl);
         const float dz = (nd.hz - zh) + (nd.lz - zl);
@@ -1019,8 +1073,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
             kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
             bx = by = bz = 0.f;
         }
-        const int nxt = __any(desc) ? ii + 1 : nd.skip;
-        i = nxt > ii ? nxt : ii + 1;  // strictly increasing: the loop always ends
+        const bool gdesc = (__ballot(desc) & gmask) != 0ull;
+        const int nxt = gdesc ? ii + 1 : nd.skip;
+        i = (nxt > ii) ? nxt : ii + 1;  // strictly increasing: the loop always ends
     }
     kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
     if (act) {
