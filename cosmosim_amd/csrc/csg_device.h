@@ -436,23 +436,40 @@ __global__ __launch_bounds__(CSG_TPB) void k_rs_scatter(
         rk[r] = old + below;
     }
     __syncthreads();
-    // per-digit offsets of each wave inside the tile
+    // per-digit offsets of each wave inside the tile, and where each digit's
+    // run starts in the tile's digit-sorted order
     __shared__ int woff[4][256];
+    __shared__ int tstart[256];
     {
         const int d = threadIdx.x;
         int s = 0;
         for (int k = 0; k < 4; ++k) { woff[k][d] = s; s += wcnt[k][d]; }
+        int total;
+        tstart[d] = block_excl_scan(s, lds4, total);
     }
     __syncthreads();
+    // stage the tile in digit-sorted order, then write each digit run with
+    // consecutive lanes (coalesced)
+    __shared__ uint64_t skey[CSG_TILE];
+    __shared__ int sval[CSG_TILE];
 #pragma unroll
     for (int r = 0; r < CSG_ITEMS; ++r) {
         const int i = base + r * 64 + lane;
         if (i < n) {
             const int d = dg[r];
-            const int pos = binbase[d] + hist[d * nb + blockIdx.x] + woff[w][d] + rk[r];
-            kout[pos] = kk[r];
-            vout[pos] = vv[r];
+            const int lp = tstart[d] + woff[w][d] + rk[r];
+            skey[lp] = kk[r];
+            sval[lp] = vv[r];
         }
+    }
+    __syncthreads();
+    const int cnt = min(CSG_TILE, n - blockIdx.x * CSG_TILE);
+    for (int j = threadIdx.x; j < cnt; j += CSG_TPB) {
+        const uint64_t k = skey[j];
+        const int d = (int)((k >> shift) & 255);
+        const int pos = binbase[d] + hist[d * nb + blockIdx.x] + (j - tstart[d]);
+        kout[pos] = k;
+        vout[pos] = sval[j];
     }
 }
 
@@ -644,7 +661,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_iscan_down(const int *__restrict__ 
 
 // ====================================================== double-double scan
 // Exclusive prefix P[b] over sorted bodies of (m, m*x, m*y, m*z), each a DD;
-// P[n_live] is the total.  Layout: pref[c*(cap1) + b], c = 0..3.  Every sum is
+// P[n_live] is the total.  Layout: pref[b*4 + c], c = 0..3 (one 64-byte line per b).  Every sum is
 // formed in a fixed order (thread-serial over 8 elements, then a wave shuffle
 // scan, then the 4 waves in order), so the result is reproducible.
 __device__ __forceinline__ void dd_elem(const double *__restrict__ xs,
@@ -759,7 +776,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_dd_down(const double *__restrict__ 
     for (int r = 0; r < CSG_ITEMS; ++r) {
         const int b = base + r;
         if (b <= nl)
-            for (int c = 0; c < 4; ++c) pref[(size_t)c * cap1 + b] = run[c];
+            for (int c = 0; c < 4; ++c) pref[(size_t)b * 4 + c] = run[c];
         if (b < nl) {
             DD e[4];
             dd_elem(xs, ys, zs, ms, b, nl, e);
@@ -770,7 +787,8 @@ __global__ __launch_bounds__(CSG_TPB) void k_dd_down(const double *__restrict__ 
 
 __device__ __forceinline__ double dd_range(const DD *__restrict__ pref, int cap1, int c, int b,
                                            int e) {
-    const DD hi = pref[(size_t)c * cap1 + e + 1], lo = pref[(size_t)c * cap1 + b];
+    (void)cap1;
+    const DD hi = pref[(size_t)(e + 1) * 4 + c], lo = pref[(size_t)b * 4 + c];
     return dd_add(hi, dd_neg(lo)).h;
 }
 
