@@ -72,6 +72,12 @@ _SIGS = {
     "cosmosim_ext_clear_error": (None, []),
     "cosmosim_ext_error_string": (ctypes.c_char_p, [_I]),
     "cosmosim_ext_device_info": (_I, [_I, ctypes.c_char_p, _I, _VP]),
+    "cosmosim_ext_force_vjp": (_I, [_VP, _VP, _VP]),
+    "cosmosim_ext_forces_frozen": (_I, [_VP, _VP, _VP]),
+    "cosmosim_ext_direct_vjp": (_I, [_VP, _VP, _VP]),
+    "cosmosim_ext_record": (_I, [_VP, _I, _I, _I]),
+    "cosmosim_ext_backward": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP]),
+    "cosmosim_ext_replay": (_I, [_VP, _VP, _VP, _VP, _VP]),
     # csg.h (device-core C-ABI)
     "csg_device_count": (_I, [_VP]),
     "csg_device_info": (_I, [_I, ctypes.c_char_p, _I, _VP]),
@@ -97,6 +103,14 @@ _SIGS = {
     "csg_get_stats": (_I, [_VP, _VP]),
     "csg_set_timing": (_I, [_VP, _I]),
     "csg_synchronize": (_I, [_VP]),
+    "csg_walk_taped": (_I, [_VP, _I, _D, _D, _D]),
+    "csg_force_vjp": (_I, [_VP, _VP, _VP]),
+    "csg_forces_frozen": (_I, [_VP, _VP, _VP]),
+    "csg_direct_vjp": (_I, [_VP, _VP, _VP, _D, _D]),
+    "csg_record": (_I, [_VP, _I, _D, _D, _D, _D, _I, _I, _I]),
+    "csg_backward": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP]),
+    "csg_replay": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "csg_set_tape": (_I, [_VP, _I]),
     "csg_host_alloc": (_VP, [ctypes.c_size_t]),
     "csg_host_free": (None, [_VP]),
 }
@@ -333,3 +347,44 @@ class Simulation(Handle):
 
     def sync(self):
         _check(lib().cosmosim_ext_sync(self.h), "sync")
+
+    # ------------------------------------------------------------ adjoint
+    def force_vjp(self, w):
+        """(d a / d x)^T w at the current positions (tree and decisions frozen)."""
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.n, 3)
+        out = np.zeros((self.n, 3))
+        _check(lib().cosmosim_ext_force_vjp(self.h, ptr(w), ptr(out)), "force_vjp")
+        return out
+
+    def forces_frozen(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
+        out = np.zeros((self.n, 3))
+        _check(lib().cosmosim_ext_forces_frozen(self.h, ptr(pos), ptr(out)), "forces_frozen")
+        return out
+
+    def direct_vjp(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.n, 3)
+        out = np.zeros((self.n, 3))
+        _check(lib().cosmosim_ext_direct_vjp(self.h, ptr(w), ptr(out)), "direct_vjp")
+        return out
+
+    def record(self, substeps, checkpoint_every=10, keep_trees=False):
+        _check(lib().cosmosim_ext_record(self.h, substeps, checkpoint_every, int(keep_trees)),
+               "record")
+
+    def backward(self, gpos, gvel=None, mode=0):
+        """(dL/dx0, dL/dv0, stats) from dL/dx_k (and dL/dv_k) of the recorded run."""
+        gp = np.ascontiguousarray(gpos, dtype=np.float64).reshape(self.n, 3)
+        gv = None if gvel is None else np.ascontiguousarray(gvel, dtype=np.float64).reshape(self.n, 3)
+        xb, vb, st = np.zeros((self.n, 3)), np.zeros((self.n, 3)), np.zeros(2)
+        _check(lib().cosmosim_ext_backward(self.h, ptr(gp), ptr(gv), mode, ptr(xb), ptr(vb),
+                                           ptr(st)), "backward")
+        return xb, vb, st
+
+    def replay(self, pos0, vel0):
+        p0 = np.ascontiguousarray(pos0, dtype=np.float64).reshape(self.n, 3)
+        v0 = np.ascontiguousarray(vel0, dtype=np.float64).reshape(self.n, 3)
+        p, v = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        _check(lib().cosmosim_ext_replay(self.h, ptr(p0), ptr(v0), ptr(p), ptr(v)), "replay")
+        return p, v
+

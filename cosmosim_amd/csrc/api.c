@@ -527,3 +527,76 @@ COSMOSIM_API int cosmosim_ext_device_info(int device, char *name, int len, int *
 {
     return csg_device_info(device, name, len, compute_units);
 }
+
+/* ---------------------------------------------------------------- adjoint */
+COSMOSIM_API int cosmosim_ext_force_vjp(SimHandle h, const double *w, double *xbar)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    int st = csg_build(s->ctx);
+    if (!st)
+        st = csg_walk_taped(s->ctx, s->params.precision, s->params.G, s->params.softening,
+                            s->params.theta);
+    if (!st)
+        st = csg_force_vjp(s->ctx, w, xbar);
+    if (!st)
+        s->host_stale = 1;
+    return st;
+}
+
+COSMOSIM_API int cosmosim_ext_forces_frozen(SimHandle h, const double *pos, double *acc)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    s->host_stale = 1;
+    return csg_forces_frozen(s->ctx, pos, acc);
+}
+
+COSMOSIM_API int cosmosim_ext_direct_vjp(SimHandle h, const double *w, double *xbar)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    return csg_direct_vjp(s->ctx, w, xbar, s->params.G, s->params.softening);
+}
+
+COSMOSIM_API int cosmosim_ext_record(SimHandle h, int substeps, int checkpoint_every,
+                                     int keep_trees)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    if (s->config.quasar)
+        return COSMOSIM_E_STATE;
+    int st = csg_record(s->ctx, substeps, s->params.dt, s->params.G, s->params.softening,
+                        s->params.theta, s->params.precision, checkpoint_every, keep_trees);
+    if (st == COSMOSIM_OK) {
+        s->host_stale = 1;
+        s->sim_time += s->params.dt * substeps;
+    }
+    return st;
+}
+
+COSMOSIM_API int cosmosim_ext_backward(SimHandle h, const double *gpos, const double *gvel,
+                                       int mode, double *pos0_bar, double *vel0_bar,
+                                       double stats[2])
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    s->host_stale = 1;
+    return csg_backward(s->ctx, gpos, gvel, mode, pos0_bar, vel0_bar, stats);
+}
+
+COSMOSIM_API int cosmosim_ext_replay(SimHandle h, const double *pos0, const double *vel0,
+                                     double *pos, double *vel)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    s->host_stale = 1;
+    return csg_replay(s->ctx, pos0, vel0, pos, vel);
+}
+

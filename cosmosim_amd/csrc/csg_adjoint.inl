@@ -1,0 +1,927 @@
+// csg_adjoint.inl - adjoint of the force walk and of the KDK leapfrog.
+// Included at the end of csg_core.hip (same translation unit, so it shares the
+// context and the build kernels).  No counterpart in the reference.
+//
+// Force VJP.  For a frozen tree and frozen opening decisions (a walk tape),
+// a_i = G sum_J M_J K(d_iJ) d_iJ with d_iJ = c_J - x_i and K = (|d|^2+eps^2)^-3/2.
+// For weights w (the adjoint of a), the gradient of sum_i w_i . a_i is
+//   xbar_i  = - sum_J T_iJ w_i                 (the body's own offset)
+//   cbar_J  = + sum_{i interacting with J} T_iJ w_i   (the node's centre of mass)
+//   xbar_k += (m_k / M_J) cbar_J for every node J containing k (c_J = sum m x / M)
+// with T = G M r^-3 (I - 3 d d^T / r^2), r^2 = |d|^2 + eps^2.
+//   1. k_adj_walk replays each wave's tape: per-lane -T w, and per tape record the
+//      wave's lane sum of T w (a fixed butterfly order).
+//   2. The records are stable-radix-sorted by node id: the transposed
+//      interaction lists.  k_adj_node_reduce sums each node's records in that
+//      order: cbar without atomics, reproducibly.
+//   3. The push-down is a difference array over each node's sorted member range
+//      [first, last], prefix-summed in double-double (k_adj_diff + k_dd_*).
+// Masses are not differentiated.
+
+// T w for one interaction (FP64).
+__device__ __forceinline__ void tidal_apply(double dx, double dy, double dz, double wx, double wy,
+                                            double wz, double gm, double eps2, double &gx,
+                                            double &gy, double &gz) {
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz)) + eps2;
+    const double ri = 1.0 / sqrt(r2);
+    const double ri2 = ri * ri;
+    const double s = gm * ri2 * ri;
+    const double k = 3.0 * fma(dx, wx, fma(dy, wy, dz * wz)) * ri2;
+    gx = s * (wx - k * dx);
+    gy = s * (wy - k * dy);
+    gz = s * (wz - k * dz);
+}
+
+// Weights in sorted order: ws[b] = w[idx[b]].
+__global__ __launch_bounds__(CSG_TPB) void k_adj_gather_w(const int *__restrict__ idx,
+                                                          const double *__restrict__ w,
+                                                          const BuildParams *__restrict__ bp,
+                                                          double *__restrict__ ws) {
+    const int b = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (b >= bp->n_live) return;
+    const size_t i = (size_t)idx[b];
+    ws[3 * (size_t)b] = w[3 * i];
+    ws[3 * (size_t)b + 1] = w[3 * i + 1];
+    ws[3 * (size_t)b + 2] = w[3 * i + 2];
+}
+
+// Exclusive scan of the per-wave tape counts (one block); total -> *total.
+__global__ __launch_bounds__(CSG_TPB) void k_tape_scan(const int *__restrict__ count, int nw,
+                                                       int *__restrict__ off, int *__restrict__ total) {
+    __shared__ int lds4[4];
+    int carry = 0;
+    for (int c0 = 0; c0 < nw; c0 += CSG_TPB) {
+        const int i = c0 + threadIdx.x;
+        const int v = i < nw ? count[i] : 0;
+        int t;
+        const int ex = block_excl_scan(v, lds4, t);
+        if (i < nw) off[i] = carry + ex;
+        carry += t;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Step 1: per-lane -T w and per-record lane sums (one wave per 64 sorted bodies).
+__global__ __launch_bounds__(CSG_TPB) void k_adj_walk(
+    const WNode *__restrict__ nodes, const TapeRec *__restrict__ tape,
+    const int *__restrict__ tcount, const int *__restrict__ toff, int cap,
+    const BuildParams *__restrict__ bp, const double *__restrict__ xs,
+    const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ws,
+    double G, double eps2, double *__restrict__ xbar_s, uint64_t *__restrict__ rkey,
+    int *__restrict__ rval, double *__restrict__ rsum) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * CSG_TPB + threadIdx.x) >> 6;
+    const int nl = bp->n_live;
+    const int b0 = wave * 64;
+    if (b0 >= nl) return;
+    const int b = b0 + lane;
+    const bool act = b < nl;
+    const double x = act ? xs[b] : 0.0, y = act ? ys[b] : 0.0, z = act ? zs[b] : 0.0;
+    const double wx = act ? ws[3 * (size_t)b] : 0.0, wy = act ? ws[3 * (size_t)b + 1] : 0.0,
+                 wz = act ? ws[3 * (size_t)b + 2] : 0.0;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int cnt = tcount[wave], off = toff[wave];
+    const TapeRec *t = tape + (size_t)wave * cap;
+    for (int r = 0; r < cnt; ++r) {
+        const TapeRec rec = t[r];
+        const WNode nd = nodes[rec.node];
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        if ((rec.imask >> lane) & 1ull)
+            tidal_apply(nd.cx - x, nd.cy - y, nd.cz - z, wx, wy, wz, G * nd.m, eps2, gx, gy, gz);
+        ax -= gx; ay -= gy; az -= gz;
+        gx = wave_sum(gx); gy = wave_sum(gy); gz = wave_sum(gz);
+        if (lane == 0) {
+            const size_t q = (size_t)off + r;
+            rkey[q] = (uint64_t)rec.node;
+            rval[q] = (int)q;
+            rsum[3 * q] = gx; rsum[3 * q + 1] = gy; rsum[3 * q + 2] = gz;
+        }
+    }
+    if (act) {
+        xbar_s[3 * (size_t)b] = ax;
+        xbar_s[3 * (size_t)b + 1] = ay;
+        xbar_s[3 * (size_t)b + 2] = az;
+    }
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_fill_f64(double *__restrict__ p, size_t n, double v) {
+    for (size_t i = blockIdx.x * (size_t)CSG_TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * CSG_TPB)
+        p[i] = v;
+}
+
+// Step 2: sum each node's records in sorted (node, wave, visit) order.
+__global__ __launch_bounds__(CSG_TPB) void k_adj_node_reduce(const uint64_t *__restrict__ key,
+                                                             const int *__restrict__ val,
+                                                             const double *__restrict__ rsum,
+                                                             const int *__restrict__ total,
+                                                             double *__restrict__ cbar) {
+    const int R = *total;
+    const int r = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (r >= R) return;
+    const uint64_t k = key[r];
+    if (r > 0 && key[r - 1] == k) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int q = r; q < R && key[q] == k; ++q) {
+        const size_t v = (size_t)val[q];
+        sx += rsum[3 * v];
+        sy += rsum[3 * v + 1];
+        sz += rsum[3 * v + 2];
+    }
+    cbar[3 * k] = sx;
+    cbar[3 * k + 1] = sy;
+    cbar[3 * k + 2] = sz;
+}
+
+// Step 3a: difference array over sorted bodies.  D[b] = sum of cbar/M over the
+// nodes whose member range starts at b minus the same over those ending at b-1.
+__device__ __forceinline__ void add_node_val(const WNode *__restrict__ nodes,
+                                             const double *__restrict__ cbar, int id, double sgn,
+                                             double &dx, double &dy, double &dz) {
+    const double f = sgn / nodes[id].m;
+    dx += f * cbar[3 * (size_t)id];
+    dy += f * cbar[3 * (size_t)id + 1];
+    dz += f * cbar[3 * (size_t)id + 2];
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_adj_diff(
+    const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext,
+    const signed char *__restrict__ L, const int *__restrict__ nbase,
+    const WNode *__restrict__ nodes, const double *__restrict__ cbar,
+    const BuildParams *__restrict__ bp, double *__restrict__ D) {
+    const int b = blockIdx.x * CSG_TPB + threadIdx.x;
+    const int nl = bp->n_live;
+    if (b >= nl) return;
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    for (int id = nbase[b]; id < nbase[b + 1]; ++id) add_node_val(nodes, cbar, id, 1.0, dx, dy, dz);
+    if (b > 0) {
+        const int e = b - 1;
+        const int le = L[e];
+        const int lpe = (e > 0) ? L[e - 1] : -1;
+        // internal nodes whose last member is e: depths (L[e], L[e-1]] (<= 63)
+        for (int d = min(le, 63) + 1; d <= min(lpe, 63); ++d) {
+            const int s0 = range_begin(key, ext, e, d);
+            const int lps = (s0 > 0) ? L[s0 - 1] : -1;
+            add_node_val(nodes, cbar, nbase[s0] + d - (min(lps, 63) + 1), -1.0, dx, dy, dz);
+        }
+        // the leaf whose last member is e
+        if (le < 64) {
+            const int s0 = (lpe >= 64) ? range_begin(key, ext, e, 64) : e;
+            add_node_val(nodes, cbar, nbase[s0 + 1] - 1, -1.0, dx, dy, dz);
+        }
+    }
+    D[3 * (size_t)b] = dx;
+    D[3 * (size_t)b + 1] = dy;
+    D[3 * (size_t)b + 2] = dz;
+}
+
+// Inclusive double-double prefix of D (3 components) over sorted bodies, in the
+// fixed order of the position scans (thread-serial, wave shuffle, 4 waves).
+__device__ __forceinline__ void v3_elem(const double *__restrict__ D, int b, int nl, DD v[4]) {
+    if (b < nl) {
+        v[0] = DD{D[3 * (size_t)b], 0.0};
+        v[1] = DD{D[3 * (size_t)b + 1], 0.0};
+        v[2] = DD{D[3 * (size_t)b + 2], 0.0};
+    } else {
+        v[0] = v[1] = v[2] = DD{0.0, 0.0};
+    }
+    v[3] = DD{0.0, 0.0};
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_v3_reduce(const double *__restrict__ D,
+                                                       const BuildParams *__restrict__ bp,
+                                                       DD *__restrict__ part) {
+    __shared__ DD wtot[4][4];
+    const int nl = bp->n_live;
+    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
+    DD s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        DD e[4];
+        v3_elem(D, base + r, nl, e);
+        for (int c = 0; c < 4; ++c) s[c] = dd_add(s[c], e[c]);
+    }
+    block_dd_incl_scan(s, wtot);
+    if (threadIdx.x == CSG_TPB - 1)
+        for (int c = 0; c < 4; ++c) part[blockIdx.x * 4 + c] = s[c];
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_v3_down(const double *__restrict__ D,
+                                                     const BuildParams *__restrict__ bp,
+                                                     const DD *__restrict__ part,
+                                                     double *__restrict__ U) {
+    __shared__ DD wtot[4][4];
+    __shared__ DD prev[CSG_TPB][4];
+    const int nl = bp->n_live;
+    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
+    DD s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        DD e[4];
+        v3_elem(D, base + r, nl, e);
+        for (int c = 0; c < 4; ++c) s[c] = dd_add(s[c], e[c]);
+    }
+    block_dd_incl_scan(s, wtot);
+    for (int c = 0; c < 4; ++c) prev[threadIdx.x][c] = s[c];
+    __syncthreads();
+    DD run[4];
+    for (int c = 0; c < 4; ++c)
+        run[c] = threadIdx.x > 0 ? dd_add(part[blockIdx.x * 4 + c], prev[threadIdx.x - 1][c])
+                                 : part[blockIdx.x * 4 + c];
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        const int b = base + r;
+        if (b >= nl) break;
+        DD e[4];
+        v3_elem(D, b, nl, e);
+        for (int c = 0; c < 3; ++c) {
+            run[c] = dd_add(run[c], e[c]);
+            U[3 * (size_t)b + c] = run[c].h;
+        }
+    }
+}
+
+// Step 3b: xbar[idx[b]] (+)= xbar_s[b] + m_b U[b]; dead bodies untouched.
+__global__ __launch_bounds__(CSG_TPB) void k_adj_finish(const int *__restrict__ idx,
+                                                        const double *__restrict__ ms,
+                                                        const double *__restrict__ xbar_s,
+                                                        const double *__restrict__ U,
+                                                        const BuildParams *__restrict__ bp,
+                                                        double *__restrict__ xbar, int accumulate) {
+    const int b = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (b >= bp->n_live) return;
+    const size_t i = (size_t)idx[b];
+    for (int c = 0; c < 3; ++c) {
+        const double v = xbar_s[3 * (size_t)b + c] + ms[b] * U[3 * (size_t)b + c];
+        xbar[3 * i + c] = accumulate ? xbar[3 * i + c] + v : v;
+    }
+}
+
+// Frozen-tree forces: the node records get centres of mass and masses from the
+// current positions over their recorded member ranges; the tape supplies the
+// interactions.  FP64, G applied at the end.
+__global__ __launch_bounds__(CSG_TPB) void k_refresh_nodes(const WNode *__restrict__ nodes,
+                                                           const DD *__restrict__ pref, int cap1,
+                                                           const double *__restrict__ xs,
+                                                           const double *__restrict__ ys,
+                                                           const double *__restrict__ zs,
+                                                           const double *__restrict__ ms,
+                                                           const BuildParams *__restrict__ bp,
+                                                           WNode *__restrict__ out) {
+    const int v = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (v >= bp->n_nodes || bp->n_live == 0) return;
+    WNode w = nodes[v];
+    if (w.first == w.last) {
+        w.cx = xs[w.first]; w.cy = ys[w.first]; w.cz = zs[w.first];
+        w.m = ms[w.first];
+    } else {
+        const double M = dd_range(pref, cap1, 0, w.first, w.last);
+        w.cx = dd_range(pref, cap1, 1, w.first, w.last) / M;
+        w.cy = dd_range(pref, cap1, 2, w.first, w.last) / M;
+        w.cz = dd_range(pref, cap1, 3, w.first, w.last) / M;
+        w.m = M;
+    }
+    out[v] = w;
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_tape_forces(
+    const WNode *__restrict__ nodes, const TapeRec *__restrict__ tape,
+    const int *__restrict__ tcount, int cap, const BuildParams *__restrict__ bp,
+    const int *__restrict__ idx, const double *__restrict__ xs, const double *__restrict__ ys,
+    const double *__restrict__ zs, double G, double eps2, double *__restrict__ ax,
+    double *__restrict__ ay, double *__restrict__ az) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * CSG_TPB + threadIdx.x) >> 6;
+    const int nl = bp->n_live;
+    const int b0 = wave * 64;
+    if (b0 >= nl) return;
+    const int b = b0 + lane;
+    const bool act = b < nl;
+    const double x = act ? xs[b] : 0.0, y = act ? ys[b] : 0.0, z = act ? zs[b] : 0.0;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const int cnt = tcount[wave];
+    const TapeRec *t = tape + (size_t)wave * cap;
+    for (int r = 0; r < cnt; ++r) {
+        const TapeRec rec = t[r];
+        if (!((rec.imask >> lane) & 1ull)) continue;
+        const WNode nd = nodes[rec.node];
+        const double dx = nd.cx - x, dy = nd.cy - y, dz = nd.cz - z;
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz)) + eps2;
+        const double ri = 1.0 / sqrt(r2);
+        const double f = nd.m * ri * ri * ri;
+        sx = fma(dx, f, sx); sy = fma(dy, f, sy); sz = fma(dz, f, sz);
+    }
+    if (act) {
+        const int i = idx[b];
+        ax[i] = G * sx; ay[i] = G * sy; az[i] = G * sz;
+    }
+}
+
+// FP64 direct-sum VJP (the judge for the tree adjoint): for weights w,
+// xbar_k = G sum_{j != k} S(x_j - x_k) (m_k w_j - m_j w_k), S = r^-3 (I - 3 d d^T / r^2).
+__global__ __launch_bounds__(CSG_TPB) void k_direct_vjp(const double *__restrict__ x,
+                                                        const double *__restrict__ y,
+                                                        const double *__restrict__ z,
+                                                        const double *__restrict__ m,
+                                                        const double *__restrict__ w, int n,
+                                                        double G, double eps2,
+                                                        double *__restrict__ xbar) {
+    const int k = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (k >= n) return;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    const double xk = x[k], yk = y[k], zk = z[k], mk = m[k];
+    const double wkx = w[3 * (size_t)k], wky = w[3 * (size_t)k + 1], wkz = w[3 * (size_t)k + 2];
+    if (live_mass(mk)) {
+        for (int j = 0; j < n; ++j) {
+            if (j == k || !live_mass(m[j])) continue;
+            const double ux = mk * w[3 * (size_t)j] - m[j] * wkx;
+            const double uy = mk * w[3 * (size_t)j + 1] - m[j] * wky;
+            const double uz = mk * w[3 * (size_t)j + 2] - m[j] * wkz;
+            double tx, ty, tz;
+            tidal_apply(x[j] - xk, y[j] - yk, z[j] - zk, ux, uy, uz, G, eps2, tx, ty, tz);
+            gx += tx; gy += ty; gz += tz;
+        }
+    }
+    xbar[3 * (size_t)k] = gx;
+    xbar[3 * (size_t)k + 1] = gy;
+    xbar[3 * (size_t)k + 2] = gz;
+}
+
+// Reverse KDK pieces (live bodies only, mirroring k_kick_drift / k_kick).
+// Before the force VJP of substep s: abar_{s+1} += (dt/2) vbar_{s+1}.
+// After it: vbar_h = vbar + dt xbar, abar_s = (dt/2) vbar_h, vbar_s = vbar_h.
+__global__ __launch_bounds__(CSG_TPB) void k_adj_kick(const double *__restrict__ m, int n, double dt,
+                                                      int phase, const double *__restrict__ xb,
+                                                      double *__restrict__ vb,
+                                                      double *__restrict__ ab) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n || !live_mass(m[i])) return;
+    for (int c = 0; c < 3; ++c) {
+        const size_t q = 3 * (size_t)i + c;
+        if (phase == 0) {
+            ab[q] = ab[q] + 0.5 * dt * vb[q];
+        } else {
+            const double v = vb[q] + dt * xb[q];
+            vb[q] = v;
+            ab[q] = 0.5 * dt * v;
+        }
+    }
+}
+
+// Time reversal of one KDK substep: v_h = v - (dt/2) a, x -= dt v_h (phase 0);
+// after the forces at the earlier positions: v = v_h - (dt/2) a (phase 1).
+__global__ __launch_bounds__(CSG_TPB) void k_unstep(double *__restrict__ x, double *__restrict__ y,
+                                                    double *__restrict__ z, double *__restrict__ vx,
+                                                    double *__restrict__ vy,
+                                                    double *__restrict__ vz,
+                                                    const double *__restrict__ ax,
+                                                    const double *__restrict__ ay,
+                                                    const double *__restrict__ az,
+                                                    const double *__restrict__ m, int n, double dt,
+                                                    int phase) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i >= n || !live_mass(m[i])) return;
+    const double hx = 0.5 * ax[i], hy = 0.5 * ay[i], hz = 0.5 * az[i];
+    if (phase == 0) {
+        const double ux = vx[i] - hx * dt, uy = vy[i] - hy * dt, uz = vz[i] - hz * dt;
+        vx[i] = ux; vy[i] = uy; vz[i] = uz;
+        x[i] = x[i] - ux * dt;
+        y[i] = y[i] - uy * dt;
+        z[i] = z[i] - uz * dt;
+    } else {
+        vx[i] = vx[i] - hx * dt;
+        vy[i] = vy[i] - hy * dt;
+        vz[i] = vz[i] - hz * dt;
+    }
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_absdiff_max(const double *__restrict__ a,
+                                                         const double *__restrict__ b, int n,
+                                                         unsigned long long *__restrict__ out) {
+    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    double d = (i < n) ? fabs(a[i] - b[i]) : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) d = fmax(d, __shfl_xor(d, o, 64));
+    // non-negative doubles order like their bit patterns
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(d));
+}
+
+// ---------------------------------------------------------------- host side
+static void adj_free(csg_ctx *c) {
+    dfree(c->tape); dfree(c->tcount); dfree(c->toff); dfree(c->tape_total); dfree(c->tape_ovf);
+    dfree(c->rkey); dfree(c->rkey2); dfree(c->rval); dfree(c->rval2); dfree(c->rsum);
+    dfree(c->rhist); dfree(c->rtot); dfree(c->ws); dfree(c->xbar_s); dfree(c->Dv); dfree(c->Uv);
+    dfree(c->cbar); dfree(c->vpart); dfree(c->nodes_alt);
+    dfree(c->xb); dfree(c->vb); dfree(c->ab); dfree(c->gtmp); dfree(c->ckpt); dfree(c->seg);
+    dfree(c->u64tmp);
+    c->tape_cap = 0;
+    c->tape_waves = 0;
+    c->rec_cap = 0;
+    c->ckpt_slots = 0;
+    c->seg_slots = 0;
+    c->tape_valid = 0;
+}
+
+// Tape and per-body adjoint buffers for the current capacity.
+static int adj_reserve(csg_ctx *c, int cap_per_wave) {
+    const int nw = at_least1(blocks_for(c->cap, 64));
+    if (c->tape && c->tape_cap >= cap_per_wave && c->tape_waves >= nw) return COSMOSIM_OK;
+    dfree(c->tape); dfree(c->tcount); dfree(c->toff); dfree(c->tape_total); dfree(c->tape_ovf);
+    const size_t C = (size_t)c->cap + 1;
+    bool ok = dalloc(c->tape, (size_t)nw * cap_per_wave) && dalloc(c->tcount, (size_t)nw) &&
+              dalloc(c->toff, (size_t)nw + 1) && dalloc(c->tape_total, 1) &&
+              dalloc(c->tape_ovf, 1);
+    if (ok && !c->ws)
+        ok = dalloc(c->ws, 3 * C) && dalloc(c->xbar_s, 3 * C) && dalloc(c->Dv, 3 * C) &&
+             dalloc(c->Uv, 3 * C) && dalloc(c->cbar, 3 * (size_t)c->node_cap) &&
+             dalloc(c->vpart, 4 * (size_t)at_least1(blocks_for((long long)C, CSG_TILE))) &&
+             dalloc(c->nodes_alt, (size_t)c->node_cap) && dalloc(c->xb, 3 * C) &&
+             dalloc(c->vb, 3 * C) && dalloc(c->ab, 3 * C) && dalloc(c->gtmp, 3 * C) &&
+             dalloc(c->u64tmp, 1);
+    if (!ok) {
+        (void)hipGetLastError();
+        adj_free(c);
+        return COSMOSIM_E_ALLOC;
+    }
+    c->tape_cap = cap_per_wave;
+    c->tape_waves = nw;
+    c->tape_valid = 0;
+    return COSMOSIM_OK;
+}
+
+static int adj_reserve_records(csg_ctx *c, size_t R) {
+    if (R <= c->rec_cap && c->rkey) return COSMOSIM_OK;
+    dfree(c->rkey); dfree(c->rkey2); dfree(c->rval); dfree(c->rval2); dfree(c->rsum);
+    dfree(c->rhist); dfree(c->rtot);
+    const size_t cap = R + R / 4 + 1024;
+    const int nb = at_least1(blocks_for((long long)cap, CSG_TILE));
+    bool ok = dalloc(c->rkey, cap) && dalloc(c->rkey2, cap) && dalloc(c->rval, cap) &&
+              dalloc(c->rval2, cap) && dalloc(c->rsum, 3 * cap) &&
+              dalloc(c->rhist, (size_t)256 * nb) && dalloc(c->rtot, 256);
+    if (!ok) {
+        (void)hipGetLastError();
+        c->rec_cap = 0;
+        return COSMOSIM_E_ALLOC;
+    }
+    c->rec_cap = cap;
+    return COSMOSIM_OK;
+}
+
+// Walk that records the tape; grows the per-wave tape when a wave overflows it.
+static int walk_taped(csg_ctx *c, int precision, double G, double softening, double theta) {
+    int cap = c->tape_cap > 0 ? c->tape_cap : 2048;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        int s = adj_reserve(c, cap);
+        if (s) return s;
+        const int nw = at_least1(blocks_for(c->n, 64));
+        k_set_i32<<<1, 1, 0, c->stream>>>(c->tape_ovf, 0);
+        hipMemsetD32Async((hipDeviceptr_t)c->tcount, 0, (size_t)nw, c->stream);
+        TapeOut t{c->tape, c->tcount, c->tape_cap, c->tape_ovf};
+        if ((s = launch_walk(c, precision, G, softening, theta, t))) return s;
+        int ovf = 0;
+        CSG_CHECK(hipMemcpyAsync(&ovf, c->tape_ovf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        CSG_CHECK(hipStreamSynchronize(c->stream));
+        if (!ovf) {
+            c->tape_valid = 1;
+            c->tape_G = G;
+            c->tape_eps2 = softening * softening;
+            return COSMOSIM_OK;
+        }
+        cap *= 2;
+        if (cap > c->node_cap + 64) cap = c->node_cap + 64;  // a wave visits a node at most once
+    }
+    return COSMOSIM_E_CAPACITY;
+}
+
+// xbar (+)= J^T w at the tree + tape of the last taped walk (device arrays in the
+// caller's body order, [n][3]).
+static int force_vjp_dev(csg_ctx *c, const double *w_dev, double *xbar_dev, int accumulate) {
+    if (!c->tree_valid || !c->tape_valid) return COSMOSIM_E_STATE;
+    hipStream_t st = c->stream;
+    const int n = c->n;
+    int s = read_params(c);
+    if (s) return s;
+    const int nl = c->hbp->n_live, nn = c->hbp->n_nodes;
+    if (nl == 0) {
+        if (!accumulate) CSG_CHECK(hipMemsetAsync(xbar_dev, 0, sizeof(double) * 3 * n, st));
+        return COSMOSIM_OK;
+    }
+    if (!accumulate) CSG_CHECK(hipMemsetAsync(xbar_dev, 0, sizeof(double) * 3 * n, st));
+    const int nw = blocks_for(nl, 64);
+    k_adj_gather_w<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, w_dev, c->bp, c->ws);
+    k_tape_scan<<<1, CSG_TPB, 0, st>>>(c->tcount, nw, c->toff, c->tape_total);
+    int R = 0;
+    CSG_CHECK(hipMemcpyAsync(&R, c->tape_total, sizeof(int), hipMemcpyDeviceToHost, st));
+    CSG_CHECK(hipStreamSynchronize(st));
+    if ((s = adj_reserve_records(c, (size_t)R + 1))) return s;
+    k_adj_walk<<<blocks_for((long long)nw * 64, CSG_TPB), CSG_TPB, 0, st>>>(
+        c->nodes, c->tape, c->tcount, c->toff, c->tape_cap, c->bp, c->xs, c->ys, c->zs, c->ws,
+        c->tape_G, c->tape_eps2, c->xbar_s, c->rkey, c->rval, c->rsum);
+    CSG_LAUNCHED();
+    // stable radix sort of the records by node id: the transposed lists
+    int bits = 1;
+    while ((1ll << bits) < (long long)nn) ++bits;
+    const int passes = (bits + 7) / 8;
+    uint64_t *kin = c->rkey, *kout = c->rkey2;
+    int *vin = c->rval, *vout = c->rval2;
+    if (R > 0) {
+        const int nb = blocks_for(R, CSG_TILE);
+        for (int pass = 0; pass < passes; ++pass) {
+            k_rs_hist<<<nb, CSG_TPB, 0, st>>>(kin, R, 8 * pass, c->rhist, nb);
+            k_rs_rowscan<<<256, CSG_TPB, 0, st>>>(c->rhist, nb, c->rtot);
+            k_rs_scatter<<<nb, CSG_TPB, 0, st>>>(kin, vin, kout, vout, R, 8 * pass, c->rhist,
+                                                 c->rtot, nb);
+            uint64_t *tk = kin; kin = kout; kout = tk;
+            int *tv = vin; vin = vout; vout = tv;
+        }
+    }
+    k_fill_f64<<<256, CSG_TPB, 0, st>>>(c->cbar, 3 * (size_t)nn, 0.0);
+    if (R > 0)
+        k_adj_node_reduce<<<blocks_for(R, CSG_TPB), CSG_TPB, 0, st>>>(kin, vin, c->rsum,
+                                                                      c->tape_total, c->cbar);
+    k_adj_diff<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(c->key, c->ext, c->lcp, c->nbase,
+                                                            c->nodes, c->cbar, c->bp, c->Dv);
+    const int nt = blocks_for(nl, CSG_TILE);
+    k_v3_reduce<<<nt, CSG_TPB, 0, st>>>(c->Dv, c->bp, c->vpart);
+    k_dd_top<<<1, CSG_TPB, 0, st>>>(c->vpart, nt);
+    k_v3_down<<<nt, CSG_TPB, 0, st>>>(c->Dv, c->bp, c->vpart, c->Uv);
+    k_adj_finish<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, c->ms, c->xbar_s, c->Uv, c->bp,
+                                                              xbar_dev, 1);
+    CSG_LAUNCHED();
+    return COSMOSIM_OK;
+}
+
+// Forces at the current positions through the frozen tree (recorded member
+// ranges) and frozen interactions (the tape of the last taped walk).
+static int forces_frozen_dev(csg_ctx *c) {
+    if (!c->tree_valid || !c->tape_valid) return COSMOSIM_E_STATE;
+    hipStream_t st = c->stream;
+    const int n = c->n;
+    if (n == 0) return COSMOSIM_OK;
+    int s = read_params(c);
+    if (s) return s;
+    const int nl = c->hbp->n_live, nn = c->hbp->n_nodes;
+    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, nullptr);
+    if (nl == 0) return COSMOSIM_OK;
+    k_gather_sorted<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, c->x, c->y, c->z, c->m,
+                                                                c->bp, c->xs, c->ys, c->zs, c->ms);
+    const int nt = blocks_for(nl + 1, CSG_TILE);
+    k_dd_reduce<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part);
+    k_dd_top<<<1, CSG_TPB, 0, st>>>(c->dd_part, nt);
+    k_dd_down<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part, c->pref,
+                                      c->cap + 1);
+    k_refresh_nodes<<<blocks_for(nn, CSG_TPB), CSG_TPB, 0, st>>>(c->nodes, c->pref, c->cap + 1,
+                                                                c->xs, c->ys, c->zs, c->ms, c->bp,
+                                                                c->nodes_alt);
+    const int nw = blocks_for(nl, 64);
+    k_tape_forces<<<blocks_for((long long)nw * 64, CSG_TPB), CSG_TPB, 0, st>>>(
+        c->nodes_alt, c->tape, c->tcount, c->tape_cap, c->bp, c->idx, c->xs, c->ys, c->zs,
+        c->tape_G, c->tape_eps2, c->ax, c->ay, c->az);
+    CSG_LAUNCHED();
+    return COSMOSIM_OK;
+}
+
+static int upload3(csg_ctx *c, double *dst, const double *host) {
+    CSG_CHECK(hipMemcpyAsync(dst, host, sizeof(double) * 3 * (size_t)c->n, hipMemcpyHostToDevice,
+                             c->stream));
+    return COSMOSIM_OK;
+}
+static int download3(csg_ctx *c, double *host, const double *src) {
+    CSG_CHECK(hipMemcpyAsync(host, src, sizeof(double) * 3 * (size_t)c->n, hipMemcpyDeviceToHost,
+                             c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
+// Device copy of the whole kinematic state (x, v, a: 9 doubles per body).
+static int state_copy(csg_ctx *c, double *dst, int to_slot) {
+    const size_t n = (size_t)c->n;
+    double *arr[9] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az};
+    for (int a = 0; a < 9; ++a) {
+        if (to_slot)
+            CSG_CHECK(hipMemcpyAsync(dst + a * n, arr[a], sizeof(double) * n,
+                                     hipMemcpyDeviceToDevice, c->stream));
+        else
+            CSG_CHECK(hipMemcpyAsync(arr[a], dst + a * n, sizeof(double) * n,
+                                     hipMemcpyDeviceToDevice, c->stream));
+    }
+    return COSMOSIM_OK;
+}
+
+// Frozen structures of one force evaluation of a recorded run: sorted order,
+// build parameters, node records and tape.
+struct FrozenStep {
+    int *idx = nullptr;
+    BuildParams *bp = nullptr;
+    WNode *nodes = nullptr;
+    int *tcount = nullptr;
+    TapeRec *tape = nullptr;
+    int cap = 0;
+};
+
+static void frozen_free(csg_ctx *c) {
+    if (!c->frozen) return;
+    for (int e = 0; e < c->n_frozen; ++e) {
+        FrozenStep &f = c->frozen[e];
+        dfree(f.idx); dfree(f.bp); dfree(f.nodes); dfree(f.tcount); dfree(f.tape);
+    }
+    delete[] c->frozen;
+    c->frozen = nullptr;
+    c->n_frozen = 0;
+}
+
+// Copy the current tree + tape into frozen entry e.
+static int frozen_store(csg_ctx *c, int e) {
+    int s = read_params(c);
+    if (s) return s;
+    FrozenStep &f = c->frozen[e];
+    const int n = c->n > 0 ? c->n : 1, nn = c->hbp->n_nodes;
+    const int nw = at_least1(blocks_for(c->n, 64));
+    if (!dalloc(f.idx, (size_t)n) || !dalloc(f.bp, 1) || !dalloc(f.nodes, (size_t)nn) ||
+        !dalloc(f.tcount, (size_t)nw) || !dalloc(f.tape, (size_t)nw * c->tape_cap)) {
+        (void)hipGetLastError();
+        return COSMOSIM_E_ALLOC;
+    }
+    f.cap = c->tape_cap;
+    hipStream_t st = c->stream;
+    CSG_CHECK(hipMemcpyAsync(f.idx, c->idx, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+    CSG_CHECK(hipMemcpyAsync(f.bp, c->bp, sizeof(BuildParams), hipMemcpyDeviceToDevice, st));
+    CSG_CHECK(hipMemcpyAsync(f.nodes, c->nodes, sizeof(WNode) * nn, hipMemcpyDeviceToDevice, st));
+    CSG_CHECK(hipMemcpyAsync(f.tcount, c->tcount, sizeof(int) * nw, hipMemcpyDeviceToDevice, st));
+    CSG_CHECK(hipMemcpyAsync(f.tape, c->tape, sizeof(TapeRec) * (size_t)nw * c->tape_cap,
+                             hipMemcpyDeviceToDevice, st));
+    return COSMOSIM_OK;
+}
+
+// a = forces at the current positions through frozen entry e.
+static int frozen_forces(csg_ctx *c, int e) {
+    const FrozenStep &f = c->frozen[e];
+    hipStream_t st = c->stream;
+    const int n = c->n;
+    BuildParams hb;
+    CSG_CHECK(hipMemcpyAsync(&hb, f.bp, sizeof(BuildParams), hipMemcpyDeviceToHost, st));
+    CSG_CHECK(hipStreamSynchronize(st));
+    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, nullptr);
+    const int nl = hb.n_live, nn = hb.n_nodes;
+    if (nl == 0) return COSMOSIM_OK;
+    k_gather_sorted<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(f.idx, c->x, c->y, c->z, c->m, f.bp,
+                                                                c->xs, c->ys, c->zs, c->ms);
+    const int nt = blocks_for(nl + 1, CSG_TILE);
+    k_dd_reduce<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, f.bp, c->dd_part);
+    k_dd_top<<<1, CSG_TPB, 0, st>>>(c->dd_part, nt);
+    k_dd_down<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, f.bp, c->dd_part, c->pref,
+                                      c->cap + 1);
+    k_refresh_nodes<<<blocks_for(nn, CSG_TPB), CSG_TPB, 0, st>>>(f.nodes, c->pref, c->cap + 1, c->xs,
+                                                                c->ys, c->zs, c->ms, f.bp,
+                                                                c->nodes_alt);
+    const int nw = blocks_for(nl, 64);
+    k_tape_forces<<<blocks_for((long long)nw * 64, CSG_TPB), CSG_TPB, 0, st>>>(
+        c->nodes_alt, f.tape, f.tcount, f.cap, f.bp, f.idx, c->xs, c->ys, c->zs, c->rec_G,
+        c->rec_eps * c->rec_eps, c->ax, c->ay, c->az);
+    CSG_LAUNCHED();
+    c->tree_valid = 0;  // the sorted copies now hold replay positions
+    return COSMOSIM_OK;
+}
+
+extern "C" {
+
+int csg_set_tape(csg_ctx *c, int enabled) {
+    CSG_USE(c);
+    c->tape_on = enabled ? 1 : 0;
+    return COSMOSIM_OK;
+}
+
+int csg_walk_taped(csg_ctx *c, int precision, double G, double softening, double theta) {
+    CSG_USE(c);
+    if (!c->tree_valid) return COSMOSIM_E_STATE;
+    if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
+    return walk_taped(c, precision, G, softening, theta);
+}
+
+int csg_force_vjp(csg_ctx *c, const double *w, double *xbar) {
+    CSG_USE(c);
+    if (!w || !xbar) return COSMOSIM_E_ARG;
+    if (c->n == 0) return COSMOSIM_OK;
+    int s = adj_reserve(c, c->tape_cap > 0 ? c->tape_cap : 2048);
+    if (s) return s;
+    if ((s = upload3(c, c->gtmp, w))) return s;
+    if ((s = force_vjp_dev(c, c->gtmp, c->xb, 0))) return s;
+    return download3(c, xbar, c->xb);
+}
+
+// Forces at positions `pos` ([n][3], may be NULL for the current ones) through
+// the tree and tape of the last taped walk.  The positions become the state's.
+int csg_forces_frozen(csg_ctx *c, const double *pos, double *acc) {
+    CSG_USE(c);
+    int s = ensure_diag(c, sizeof(double) * 3 * (size_t)(c->n > 0 ? c->n : 1));
+    if (s) return s;
+    if (pos && c->n > 0 && (s = set3(c, c->x, c->y, c->z, pos))) return s;
+    if ((s = forces_frozen_dev(c))) return s;
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    if (acc && c->n > 0) return get3(c, c->ax, c->ay, c->az, acc);
+    return COSMOSIM_OK;
+}
+
+int csg_direct_vjp(csg_ctx *c, const double *w, double *xbar, double G, double softening) {
+    CSG_USE(c);
+    if (!w || !xbar) return COSMOSIM_E_ARG;
+    if (c->n == 0) return COSMOSIM_OK;
+    int s = adj_reserve(c, c->tape_cap > 0 ? c->tape_cap : 2048);
+    if (s) return s;
+    if ((s = upload3(c, c->gtmp, w))) return s;
+    k_direct_vjp<<<blocks_for(c->n, CSG_TPB), CSG_TPB, 0, c->stream>>>(
+        c->x, c->y, c->z, c->m, c->gtmp, c->n, G, softening * softening, c->xb);
+    CSG_LAUNCHED();
+    return download3(c, xbar, c->xb);
+}
+
+// Forward k substeps from the current state (whose accelerations must be the
+// forces at its positions), keeping the state every `every` substeps and the
+// final state on the device for csg_backward.
+int csg_record(csg_ctx *c, int k, double dt, double G, double softening, double theta,
+               int precision, int every, int keep_trees) {
+    CSG_USE(c);
+    if (k < 0 || every <= 0 || precision < 0 || precision > 2) return COSMOSIM_E_ARG;
+    int s = adj_reserve(c, c->tape_cap > 0 ? c->tape_cap : 2048);
+    if (s) return s;
+    frozen_free(c);
+    c->rec_valid = 0;
+    const int nck = k / every + 2;  // checkpoints 0, K, 2K, ... plus the final state
+    const size_t slot = 9 * (size_t)c->n;
+    if (c->ckpt_slots < nck) {
+        dfree(c->ckpt);
+        c->ckpt_slots = 0;
+        if (!dalloc(c->ckpt, slot * nck)) {
+            (void)hipGetLastError();
+            return COSMOSIM_E_ALLOC;
+        }
+        c->ckpt_slots = nck;
+    }
+    c->rec_k = k; c->rec_every = every; c->rec_dt = dt; c->rec_G = G; c->rec_eps = softening;
+    c->rec_theta = theta; c->rec_prec = precision;
+    if (keep_trees) {
+        // entry 0: the forces of the initial state; entry s+1: those of substep s
+        c->frozen = new (std::nothrow) FrozenStep[k + 1];
+        if (!c->frozen) return COSMOSIM_E_ALLOC;
+        c->n_frozen = k + 1;
+        if ((s = launch_build(c)) || (s = walk_taped(c, precision, G, softening, theta)) ||
+            (s = frozen_store(c, 0)))
+            return s;
+    }
+    for (int step = 0; step < k; ++step) {
+        if (step % every == 0 && (s = state_copy(c, c->ckpt + slot * (step / every), 1))) return s;
+        if (keep_trees) {
+            if ((s = launch_kick_drift(c, dt, 1)) || (s = launch_build(c)) ||
+                (s = walk_taped(c, precision, G, softening, theta)) || (s = launch_kick(c, dt)) ||
+                (s = frozen_store(c, step + 1)))
+                return s;
+        } else if ((s = csg_substeps(c, 1, dt, G, softening, theta, precision))) {
+            return s;
+        }
+    }
+    if (k % every == 0 && (s = state_copy(c, c->ckpt + slot * (k / every), 1))) return s;
+    if ((s = state_copy(c, c->ckpt + slot * (nck - 1), 1))) return s;
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    c->rec_valid = 1;
+    return COSMOSIM_OK;
+}
+
+// Reverse pass of the recorded run.  gx, gv: dL/dx_k, dL/dv_k ([n][3], gv may be
+// NULL); writes dL/dx_0, dL/dv_0.  mode 0 recomputes each segment forward from
+// its checkpoint; mode 1 steps the leapfrog backwards from the final state.
+// stats (may be NULL): [0] max |x_s(reversed) - x_s(checkpoint)| over the
+// checkpoints (mode 1), [1] device bytes held by the adjoint.
+int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, double *x0bar,
+                 double *v0bar, double *stats) {
+    CSG_USE(c);
+    if (!gx || !x0bar || !v0bar || (mode != 0 && mode != 1)) return COSMOSIM_E_ARG;
+    if (!c->rec_valid) return COSMOSIM_E_STATE;
+    const int n = c->n, k = c->rec_k, every = c->rec_every;
+    const double dt = c->rec_dt, G = c->rec_G, eps = c->rec_eps, th = c->rec_theta;
+    const int prec = c->rec_prec;
+    const size_t slot = 9 * (size_t)n;
+    const int nck = k / every + 2;
+    hipStream_t st = c->stream;
+    int s;
+    if ((s = upload3(c, c->xb, gx))) return s;
+    if (gv) { if ((s = upload3(c, c->vb, gv))) return s; }
+    else CSG_CHECK(hipMemsetAsync(c->vb, 0, sizeof(double) * 3 * n, st));
+    CSG_CHECK(hipMemsetAsync(c->ab, 0, sizeof(double) * 3 * n, st));
+    CSG_CHECK(hipMemsetAsync(c->u64tmp, 0, sizeof(unsigned long long), st));
+    const int g = blocks_for(n, CSG_TPB);
+    // positions x_{s0+1} .. x_{s1} of one segment (mode 0)
+    if (mode == 0 && c->seg_slots < every) {
+        dfree(c->seg);
+        c->seg_slots = 0;
+        if (!dalloc(c->seg, 3 * (size_t)n * every)) {
+            (void)hipGetLastError();
+            return COSMOSIM_E_ALLOC;
+        }
+        c->seg_slots = every;
+    }
+    // one reverse substep with the positions x_{s+1} in the state arrays
+    auto reverse_substep = [&]() -> int {
+        int r;
+        k_adj_kick<<<g, CSG_TPB, 0, st>>>(c->m, n, dt, 0, c->xb, c->vb, c->ab);
+        if ((r = launch_build(c))) return r;
+        if ((r = walk_taped(c, prec, G, eps, th))) return r;
+        if ((r = force_vjp_dev(c, c->ab, c->xb, 1))) return r;
+        k_adj_kick<<<g, CSG_TPB, 0, st>>>(c->m, n, dt, 1, c->xb, c->vb, c->ab);
+        CSG_LAUNCHED();
+        return COSMOSIM_OK;
+    };
+    double xs_dev = 0.0;
+    if (mode == 0) {
+        const int nseg = (k + every - 1) / every;
+        for (int seg = nseg - 1; seg >= 0; --seg) {
+            const int s0 = seg * every, s1 = (s0 + every < k) ? s0 + every : k;
+            if ((s = state_copy(c, c->ckpt + slot * seg, 0))) return s;
+            for (int step = s0; step < s1; ++step) {
+                if ((s = csg_substeps(c, 1, dt, G, eps, th, prec))) return s;
+                double *dstp = c->seg + 3 * (size_t)n * (step - s0);
+                CSG_CHECK(hipMemcpyAsync(dstp, c->x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                CSG_CHECK(hipMemcpyAsync(dstp + n, c->y, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                CSG_CHECK(hipMemcpyAsync(dstp + 2 * n, c->z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+            }
+            for (int step = s1 - 1; step >= s0; --step) {
+                const double *srcp = c->seg + 3 * (size_t)n * (step - s0);
+                CSG_CHECK(hipMemcpyAsync(c->x, srcp, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                CSG_CHECK(hipMemcpyAsync(c->y, srcp + n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                CSG_CHECK(hipMemcpyAsync(c->z, srcp + 2 * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                if ((s = reverse_substep())) return s;
+            }
+        }
+        if ((s = state_copy(c, c->ckpt, 0))) return s;   // x_0
+    } else {
+        if ((s = state_copy(c, c->ckpt + slot * (nck - 1), 0))) return s;  // final state
+        for (int step = k - 1; step >= 0; --step) {
+            if ((s = reverse_substep())) return s;
+            // reconstruct the state of substep `step`
+            k_unstep<<<g, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay,
+                                            c->az, c->m, n, dt, 0);
+            if ((s = launch_build(c))) return s;
+            if ((s = launch_walk(c, prec, G, eps, th))) return s;
+            k_unstep<<<g, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay,
+                                            c->az, c->m, n, dt, 1);
+            CSG_LAUNCHED();
+            if (step % every == 0) {
+                const double *ck = c->ckpt + slot * (step / every);
+                k_absdiff_max<<<g, CSG_TPB, 0, st>>>(c->x, ck, n, c->u64tmp);
+                k_absdiff_max<<<g, CSG_TPB, 0, st>>>(c->y, ck + n, n, c->u64tmp);
+                k_absdiff_max<<<g, CSG_TPB, 0, st>>>(c->z, ck + 2 * n, n, c->u64tmp);
+            }
+        }
+        unsigned long long bits = 0;
+        CSG_CHECK(hipMemcpyAsync(&bits, c->u64tmp, sizeof(bits), hipMemcpyDeviceToHost, st));
+        CSG_CHECK(hipStreamSynchronize(st));
+        memcpy(&xs_dev, &bits, sizeof(double));
+    }
+    // a_0 = A(x_0): x0bar += J^T abar_0 at the initial positions
+    if ((s = launch_build(c))) return s;
+    if ((s = walk_taped(c, prec, G, eps, th))) return s;
+    if ((s = force_vjp_dev(c, c->ab, c->xb, 1))) return s;
+    if ((s = download3(c, x0bar, c->xb))) return s;
+    if ((s = download3(c, v0bar, c->vb))) return s;
+    // leave the handle at the recorded final state
+    if ((s = state_copy(c, c->ckpt + slot * (nck - 1), 0))) return s;
+    if ((s = launch_build(c))) return s;
+    CSG_CHECK(hipStreamSynchronize(st));
+    if (stats) {
+        stats[0] = xs_dev;
+        size_t bytes = (size_t)c->tape_waves * c->tape_cap * sizeof(TapeRec) +
+                       c->rec_cap * (2 * sizeof(uint64_t) + 2 * sizeof(int) + 3 * sizeof(double)) +
+                       (size_t)c->ckpt_slots * slot * sizeof(double) +
+                       (size_t)c->seg_slots * 3 * n * sizeof(double) +
+                       (size_t)c->node_cap * (sizeof(WNode) + 3 * sizeof(double)) +
+                       (size_t)(c->cap + 1) * 3 * sizeof(double) * 8;
+        stats[1] = (double)bytes;
+    }
+    return COSMOSIM_OK;
+}
+
+// Frozen-tree forward of the recorded run (recorded with keep_trees): from
+// (pos0, vel0) run the same substeps with every force evaluation using the
+// recorded tree (member ranges) and interactions, centres of mass taken from the
+// current positions.  Writes the final positions and velocities.  The handle's
+// state is left at the replayed final state (without a valid tree).
+int csg_replay(csg_ctx *c, const double *pos0, const double *vel0, double *pos, double *vel) {
+    CSG_USE(c);
+    if (!pos0 || !vel0 || !pos || !vel) return COSMOSIM_E_ARG;
+    if (!c->rec_valid || !c->frozen || c->n_frozen != c->rec_k + 1) return COSMOSIM_E_STATE;
+    int s = ensure_diag(c, sizeof(double) * 3 * (size_t)(c->n > 0 ? c->n : 1));
+    if (s) return s;
+    if ((s = set3(c, c->x, c->y, c->z, pos0)) || (s = set3(c, c->vx, c->vy, c->vz, vel0))) return s;
+    if ((s = frozen_forces(c, 0))) return s;
+    for (int step = 0; step < c->rec_k; ++step) {
+        if ((s = launch_kick_drift(c, c->rec_dt, 1)) || (s = frozen_forces(c, step + 1)) ||
+            (s = launch_kick(c, c->rec_dt)))
+            return s;
+    }
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    if ((s = get3(c, c->x, c->y, c->z, pos))) return s;
+    return get3(c, c->vx, c->vy, c->vz, vel);
+}
+
+}  // extern "C"

@@ -55,6 +55,25 @@ struct csg_ctx {
     int ev_count = 0, ev_open = -1;
     double ms_build = 0, ms_walk = 0, ms_kdk = 0;
     int timed_walks = 0;
+    // adjoint: walk tape, transposed records, per-body adjoint buffers, checkpoints
+    int tape_on = 0, tape_cap = 0, tape_waves = 0, tape_valid = 0;
+    double tape_G = 1.0, tape_eps2 = 0.0;
+    TapeRec *tape = nullptr;
+    int *tcount = nullptr, *toff = nullptr, *tape_total = nullptr, *tape_ovf = nullptr;
+    size_t rec_cap = 0;
+    uint64_t *rkey = nullptr, *rkey2 = nullptr;
+    int *rval = nullptr, *rval2 = nullptr, *rhist = nullptr, *rtot = nullptr;
+    double *rsum = nullptr, *ws = nullptr, *xbar_s = nullptr, *Dv = nullptr, *Uv = nullptr;
+    double *cbar = nullptr, *xb = nullptr, *vb = nullptr, *ab = nullptr, *gtmp = nullptr;
+    DD *vpart = nullptr;
+    WNode *nodes_alt = nullptr;
+    unsigned long long *u64tmp = nullptr;
+    double *ckpt = nullptr, *seg = nullptr;
+    int ckpt_slots = 0, seg_slots = 0;
+    int rec_valid = 0, rec_k = 0, rec_every = 1, rec_prec = 0;
+    struct FrozenStep *frozen = nullptr;
+    int n_frozen = 0;
+    double rec_dt = 0, rec_G = 1, rec_eps = 0, rec_theta = 0;
     int walk_group = 8;     // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
     int walk_group32 = 64;  // FP32 walk
     int xcd_chunk = 16;     // walk tiles per XCD chunk (0: one contiguous range per XCD)
@@ -77,6 +96,9 @@ struct csg_ctx {
             return COSMOSIM_E_HIP;                                                           \
         }                                                                                    \
     } while (0)
+
+static void adj_free(csg_ctx *c);  // csg_adjoint.inl
+static void frozen_free(csg_ctx *c);
 
 static inline int blocks_for(long long n, int per) { return (int)((n + per - 1) / per); }
 static inline int at_least1(int v) { return v < 1 ? 1 : v; }
@@ -266,6 +288,8 @@ void csg_destroy(csg_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_buffers(c);
+    adj_free(c);
+    frozen_free(c);
     dfree(c->bp);
     dfree(c->itotal);
     dfree(c->err_dev);
@@ -503,21 +527,23 @@ static int launch_build(csg_ctx *c) {
 
 extern "C++" {
 template <int GS>
-static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, double th2, int nb8) {
+static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, double th2, int nb8,
+                           TapeOut tape) {
     hipStream_t st = c->stream;
     if (precision == COSMOSIM_PREC_FP64_STRICT)
         k_walk<WALK_FP64_STRICT, GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y,
-            c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk);
+            c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
     else if (precision == COSMOSIM_PREC_FP64)
         k_walk<WALK_FP64, GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
-            c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk);
+            c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
     else
         k_walk32<GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes32, c->bp, c->idx, c->x, c->y, c->z, c->ax, c->ay,
-                                          c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk);
+                                          c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
 }
 }  // extern "C++"
 
-static int launch_walk(csg_ctx *c, int precision, double G, double softening, double theta) {
+static int launch_walk(csg_ctx *c, int precision, double G, double softening, double theta,
+                       TapeOut tape = TapeOut{nullptr, nullptr, 0, nullptr}) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const double eps2 = softening * softening, th2 = theta * theta;
@@ -528,12 +554,13 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     const int gran = c->xcd_chunk > 0 ? 8 * c->xcd_chunk : 8;
     nb8 = (nb8 + gran - 1) / gran * gran;
     if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
-    const int gs = precision == COSMOSIM_PREC_FP32 ? c->walk_group32 : c->walk_group;
+    // a taped walk runs with whole-wave traversal (the tape is per wavefront)
+    const int gs = tape.rec ? 64 : (precision == COSMOSIM_PREC_FP32 ? c->walk_group32 : c->walk_group);
     switch (gs) {
-    case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, nb8); break;
-    case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, nb8); break;
-    case 32: launch_walk_gs<32>(c, precision, G, eps2, th2, nb8); break;
-    default: launch_walk_gs<64>(c, precision, G, eps2, th2, nb8); break;
+    case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, nb8, tape); break;
+    case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, nb8, tape); break;
+    case 32: launch_walk_gs<32>(c, precision, G, eps2, th2, nb8, tape); break;
+    default: launch_walk_gs<64>(c, precision, G, eps2, th2, nb8, tape); break;
     }
     CSG_LAUNCHED();
     return COSMOSIM_OK;
@@ -819,3 +846,5 @@ void csg_host_free(void *p) {
 }
 
 }  // extern "C"
+
+#include "csg_adjoint.inl"

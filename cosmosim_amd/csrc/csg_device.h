@@ -74,6 +74,43 @@ struct __align__(16) WNode32 {
 };
 static_assert(sizeof(WNode32) == 48, "WNode32 is one 48-byte record");
 
+// One record of a walk tape: a node visited by a wavefront and the lanes
+// (bodies) that interacted with it there.  A tape is the frozen list of
+// interactions of one force evaluation, in each wave's visiting order.
+struct TapeRec {
+    unsigned long long imask;
+    int node;
+    int pad;
+};
+static_assert(sizeof(TapeRec) == 16, "TapeRec is 16 bytes");
+
+// Where a walk writes its tape: wave w owns rec[w*cap, w*cap + count[w]).
+struct TapeOut {
+    TapeRec *rec;
+    int *count;
+    int cap;
+    int *overflow;
+};
+
+__device__ __forceinline__ void tape_push(const TapeOut &t, int wave, int lane, int &nrec,
+                                          unsigned long long imask, int node) {
+    if (imask == 0ull) return;
+    if (lane == 0 && nrec < t.cap) {
+        TapeRec r;
+        r.imask = imask;
+        r.node = node;
+        r.pad = 0;
+        t.rec[(size_t)wave * t.cap + nrec] = r;
+    }
+    ++nrec;
+}
+
+__device__ __forceinline__ void tape_close(const TapeOut &t, int wave, int lane, int nrec) {
+    if (lane != 0) return;
+    t.count[wave] = nrec < t.cap ? nrec : t.cap;
+    if (nrec > t.cap) atomicOr(t.overflow, 1);
+}
+
 __device__ __forceinline__ void split_f32(double v, float &h, float &l) {
     h = (float)v;
     l = (float)(v - (double)h);
@@ -897,6 +934,7 @@ struct WalkLane {
     double x, y, z;      // body position
     int p;
     int cnt;
+    bool inter;          // whether the last visited node interacted
     double sx, sy, sz;   // accumulated acceleration (without G in the fast walk)
 };
 
@@ -915,6 +953,7 @@ __device__ __forceinline__ bool walk_visit(WalkLane &L, const WNode &nd, int i, 
         if (nd.body >= 0) inter = nd.body != L.me;
         else { inter = open_accept_exact(nd.s2, d2, theta2); desc = !inter; }
         if (nd.m == 0.0) { inter = false; desc = false; }
+        L.inter = inter;
         if (inter) {
             const double r2 = d2 + eps2;
             const double w = 1.0 / (r2 * sqrt(r2));
@@ -931,6 +970,7 @@ __device__ __forceinline__ bool walk_visit(WalkLane &L, const WNode &nd, int i, 
         bool inter;
         if (nd.body >= 0) inter = nd.body != L.me;
         else { inter = open_accept_exact(nd.s2, d2, theta2); desc = !inter; }
+        L.inter = inter;
         if (inter) {
             const double r2 = d2 + eps2;
             double yv = __builtin_amdgcn_rsq(r2);
@@ -963,7 +1003,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk(
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
     const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
     double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
-    double G, double eps2, double theta2, int nb8, int chunk) {
+    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape) {
     const int n_live = bp->n_live;
     const int nnodes = bp->n_nodes;
     if (bp->overflow) return;
@@ -973,6 +1013,8 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk(
     if (b0 >= n_live) return;
     const int b = b0 + lane;
     const bool act = b < n_live;
+    const bool taping = (GS == 64) && tape.rec != nullptr;
+    int nrec = 0;
     WalkLane L;
     L.me = act ? sidx[b] : -1;
     L.x = act ? px[L.me] : 0.0;
@@ -989,13 +1031,17 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk(
         if constexpr (GS == 64) ii = __builtin_amdgcn_readfirstlane(i);
         const bool alive = ii < nnodes;
         if (!__any(alive)) break;
-        bool desc = false;
+        bool desc = false, inter = false;
         int skip = ii + 1;
         if (alive) {
             const WNode nd = nodes[ii];
             skip = nd.skip;
-            if (L.p == ii) desc = walk_visit<PREC>(L, nd, ii, G, eps2, theta2);
+            if (L.p == ii) {
+                desc = walk_visit<PREC>(L, nd, ii, G, eps2, theta2);
+                inter = L.inter;
+            }
         }
+        if (taping) tape_push(tape, b0 >> 6, lane, nrec, __ballot(inter), ii);
         const bool gdesc = (__ballot(desc) & gmask) != 0ull;
         const int nxt = gdesc ? ii + 1 : skip;
         i = nxt > ii ? nxt : ii + 1;  // strictly increasing: the loop always ends
@@ -1008,6 +1054,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk(
         }
         if (icount) icount[L.me] = L.cnt;
     }
+    if (taping) tape_close(tape, b0 >> 6, lane, nrec);
     unsigned long long c = (unsigned long long)L.cnt;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
@@ -1036,7 +1083,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
     const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
     double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
-    double G, double eps2, double theta2, int nb8, int chunk) {
+    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape) {
     const int n_live = bp->n_live;
     const int nnodes = bp->n_nodes;
     if (bp->overflow) return;
@@ -1046,6 +1093,8 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     if (b0 >= n_live) return;
     const int b = b0 + lane;
     const bool act = b < n_live;
+    const bool taping = (GS == 64) && tape.rec != nullptr;
+    int nrec = 0;
     const int me = act ? sidx[b] : -1;
     float xh, xl, yh, yl, zh, zl;
     split_f32(act ? px[me] - bp->root[0] : 0.0, xh, xl);
@@ -1085,6 +1134,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
         by = fmaf(dy, f, by);
         bz = fmaf(dz, f, bz);
         cnt += inter ? 1 : 0;
+        if (taping) tape_push(tape, b0 >> 6, lane, nrec, __ballot(inter), ii);
         if (here) p = desc ? ii + 1 : nd.skip;
         if (++steps == CSG_FOLD_EVERY) {
             steps = 0;
@@ -1102,6 +1152,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
         az[me] = G * ((double)sz - (double)kz);
         if (icount) icount[me] = cnt;
     }
+    if (taping) tape_close(tape, b0 >> 6, lane, nrec);
     unsigned long long c = (unsigned long long)cnt;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);

@@ -115,6 +115,27 @@ int cosmosim_ext_handle_error(SimHandle h);
 int cosmosim_ext_last_error(void);
 void cosmosim_ext_clear_error(void);
 const char *cosmosim_ext_error_string(int code);
+/* Adjoint of the force walk and of the leapfrog (no counterpart in the
+ * reference).  Trees and opening decisions are frozen from the forward pass;
+ * body masses are constants.  All arrays are [n][3] in body order.
+ *  force_vjp:     xbar = (d a / d x)^T w at the current positions (rebuilds the tree).
+ *  forces_frozen: forces at pos through the tree/decisions of the last force_vjp.
+ *  direct_vjp:    the same VJP for the FP64 direct sum.
+ *  record:        advance `substeps` from the current state, checkpointing every
+ *                 `checkpoint_every`; keep_trees also keeps each tree for replay.
+ *  backward:      gradients of a loss L(x_k, v_k) w.r.t. the recorded initial state;
+ *                 mode 0 = recompute from checkpoints, 1 = time-reversed leapfrog;
+ *                 stats (may be NULL) = {max reversed-state deviation, adjoint bytes}.
+ *  replay:        frozen-tree forward of the recorded run from (pos0, vel0). */
+int cosmosim_ext_force_vjp(SimHandle h, const double *w, double *xbar);
+int cosmosim_ext_forces_frozen(SimHandle h, const double *pos, double *acc);
+int cosmosim_ext_direct_vjp(SimHandle h, const double *w, double *xbar);
+int cosmosim_ext_record(SimHandle h, int substeps, int checkpoint_every, int keep_trees);
+int cosmosim_ext_backward(SimHandle h, const double *gpos, const double *gvel, int mode,
+                          double *pos0_bar, double *vel0_bar, double stats[2]);
+int cosmosim_ext_replay(SimHandle h, const double *pos0, const double *vel0, double *pos,
+                        double *vel);
+
 /* Name and compute-unit count of the HIP device (name buffer of len bytes). */
 int cosmosim_ext_device_info(int device, char *name, int len, int *compute_units);
 

@@ -80,6 +80,32 @@ int  csg_get_stats(csg_ctx *ctx, csg_stats *out);
 int  csg_set_timing(csg_ctx *ctx, int enabled);
 int  csg_synchronize(csg_ctx *ctx);
 
+/* Adjoint (no counterpart in the reference).
+ * csg_walk_taped: the walk, also recording its tape (the visited nodes and the
+ *   interacting bodies of each wavefront): frozen tree + opening decisions.
+ * csg_force_vjp: xbar = J^T w for a = forces(x) at the last taped walk, w and
+ *   xbar [n][3] in body order (centre-of-mass dependence included, masses fixed).
+ * csg_forces_frozen: forces at positions pos (NULL: current) through that tree
+ *   and tape, centres of mass from pos; pos becomes the state's positions.
+ * csg_direct_vjp: the same VJP for the FP64 direct sum (the judge).
+ * csg_record: k substeps from the current state keeping a checkpoint every
+ *   `every` substeps (and, with keep_trees, every force evaluation's tree+tape).
+ * csg_backward: dL/dx_0, dL/dv_0 from dL/dx_k, dL/dv_k (gv may be NULL) through
+ *   the recorded run; mode 0 recomputes segments from checkpoints, mode 1
+ *   reverses the leapfrog from the final state.  stats: [0] max |x| deviation of
+ *   the reversed states from the checkpoints (mode 1), [1] adjoint device bytes.
+ * csg_replay: frozen-tree forward of the recorded run from (pos0, vel0). */
+int  csg_walk_taped(csg_ctx *ctx, int precision, double G, double softening, double theta);
+int  csg_force_vjp(csg_ctx *ctx, const double *w, double *xbar);
+int  csg_forces_frozen(csg_ctx *ctx, const double *pos, double *acc);
+int  csg_direct_vjp(csg_ctx *ctx, const double *w, double *xbar, double G, double softening);
+int  csg_record(csg_ctx *ctx, int k, double dt, double G, double softening, double theta,
+                int precision, int every, int keep_trees);
+int  csg_backward(csg_ctx *ctx, const double *gx, const double *gv, int mode, double *x0bar,
+                  double *v0bar, double *stats);
+int  csg_replay(csg_ctx *ctx, const double *pos0, const double *vel0, double *pos, double *vel);
+int  csg_set_tape(csg_ctx *ctx, int enabled);
+
 /* Pinned host memory for the handle-owned Body[] (zero-filled). */
 void *csg_host_alloc(size_t bytes);
 void  csg_host_free(void *p);
