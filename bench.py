@@ -109,6 +109,47 @@ def cpu_baseline(ca, bodies, theta):
     }
 
 
+def adjoint_runs(ca, big_n, big_steps):
+    """Adjoint throughput.  Config 3: 8,192-body merger, 50 substeps, dt 1e-3,
+    eps 0.02, theta 0.5, FP64 walk, checkpoints every 10.  Config 4 (shortened to
+    big_steps substeps): big_n-body merger (seed 5), FP32 walk, theta 0.7,
+    eps 0.003, dt 5e-4, checkpoints every 10; both reverse modes."""
+    out = {}
+    for name, n, seed, steps, eps, th, dt, prec in [
+            ("config3_8192_fp64_theta0.5", 8192, 4, 50, 0.02, 0.5, 1e-3, "fp64"),
+            (f"config4_{big_n}_fp32_theta0.7", big_n, 5, big_steps, 0.003, 0.7, 5e-4, "fp32")]:
+        free0, total = ca.device_memory(0)
+        b = ca.generate(ca.IC_MERGER, n, seed)
+        orbit = ca.default_orbit()
+        target = ca.positions(ca.generate(ca.IC_MERGER, n, seed, orbit=orbit * 1.01))
+        sim = ca.Simulation(b, softening=eps, theta=th, dt=dt, precision=prec)
+        t0 = time.perf_counter()
+        sim.record(steps, 10)
+        t1 = time.perf_counter()
+        xk = sim.state()[0]
+        g = 2 * (xk - target) / n
+        r = {"bodies": n, "substeps": steps, "checkpoint_every": 10,
+             "forward_substeps_per_s": steps / (t1 - t0)}
+        grads = []
+        for mode, label in [(0, "checkpoint"), (1, "reversal")]:
+            t2 = time.perf_counter()
+            xb, vb, st = sim.backward(g, mode=mode)
+            t3 = time.perf_counter()
+            r[f"adjoint_substeps_per_s_{label}"] = steps / (t3 - t2)
+            if mode == 1:
+                r["reversal_max_position_deviation"] = float(st[0])
+            grads.append(xb)
+        r["reversal_vs_checkpoint_grad_rel"] = float(
+            np.max(np.abs(grads[0] - grads[1])) / max(np.max(np.abs(grads[0])), 1e-300))
+        free1, _ = ca.device_memory(0)
+        r["adjoint_device_bytes"] = float(st[1])
+        r["hbm_used_bytes"] = int(total - free1)
+        r["hbm_added_bytes"] = int(free0 - free1)
+        sim.close()
+        out[name] = r
+    return out
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "walk_fp32_1M_merger_traffic.json")
     if not os.path.exists(p):
@@ -126,6 +167,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--headline-only", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-bodies", type=int, default=4_000_000)
+    ap.add_argument("--adjoint-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -204,6 +248,12 @@ def main():
             head["substeps_per_s"] / strict["substeps_per_s"], 3)
     acc = results["theta0.5_accuracy_step0"]
     line["accuracy_theta0.5"] = {k: float(f"{v:.3e}") for k, v in acc.items()}
+    if not args.no_adjoint:
+        adj = adjoint_runs(ca, args.adjoint_bodies, args.adjoint_steps)
+        line["adjoint"] = {k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv)
+                               for kk, vv in v.items()} for k, v in adj.items()}
+        big = adj[f"config4_{args.adjoint_bodies}_fp32_theta0.7"]
+        line["adjoint_substeps_per_s"] = round(big["adjoint_substeps_per_s_checkpoint"], 3)
     if not args.no_cpu:
         cb = cpu_baseline(ca, bodies, 0.5)
         line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v)

@@ -22,7 +22,7 @@ __all__ = [
     "positions", "velocities", "accelerations", "lib", "library_path", "CosmosimError",
     "octree_build", "octree_compute_forces", "octree_find_neighbors", "integrator_step",
     "integrator_init_accelerations", "generate", "default_orbit", "orbit_coefficients",
-    "Simulation", "Handle", "device_info", "EXPORTED_SYMBOLS",
+    "Simulation", "Handle", "device_info", "device_memory", "EXPORTED_SYMBOLS",
 ]
 
 library_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcosmosim.so")
@@ -72,6 +72,7 @@ _SIGS = {
     "cosmosim_ext_clear_error": (None, []),
     "cosmosim_ext_error_string": (ctypes.c_char_p, [_I]),
     "cosmosim_ext_device_info": (_I, [_I, ctypes.c_char_p, _I, _VP]),
+    "cosmosim_ext_device_memory": (_I, [_I, _VP, _VP]),
     "cosmosim_ext_force_vjp": (_I, [_VP, _VP, _VP]),
     "cosmosim_ext_forces_frozen": (_I, [_VP, _VP, _VP]),
     "cosmosim_ext_direct_vjp": (_I, [_VP, _VP, _VP]),
@@ -81,6 +82,7 @@ _SIGS = {
     # csg.h (device-core C-ABI)
     "csg_device_count": (_I, [_VP]),
     "csg_device_info": (_I, [_I, ctypes.c_char_p, _I, _VP]),
+    "csg_device_memory": (_I, [_I, _VP, _VP]),
     "csg_create": (_I, [_I, _I, _VP]),
     "csg_destroy": (None, [_VP]),
     "csg_reserve": (_I, [_VP, _I]),
@@ -159,6 +161,14 @@ def device_info(device=0):
     cu = ctypes.c_int(0)
     _check(lib().cosmosim_ext_device_info(device, name, 256, ctypes.byref(cu)), "device_info")
     return name.value.decode(), cu.value
+
+
+def device_memory(device=0):
+    """(free, total) device memory in bytes."""
+    f, t = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    _check(lib().cosmosim_ext_device_memory(device, ctypes.byref(f), ctypes.byref(t)),
+           "device_memory")
+    return f.value, t.value
 
 
 # ------------------------------------------------------------ legacy (src/octree.h)

@@ -528,6 +528,12 @@ COSMOSIM_API int cosmosim_ext_device_info(int device, char *name, int len, int *
     return csg_device_info(device, name, len, compute_units);
 }
 
+COSMOSIM_API int cosmosim_ext_device_memory(int device, unsigned long long *free_bytes,
+                                            unsigned long long *total_bytes)
+{
+    return csg_device_memory(device, free_bytes, total_bytes);
+}
+
 /* ---------------------------------------------------------------- adjoint */
 COSMOSIM_API int cosmosim_ext_force_vjp(SimHandle h, const double *w, double *xbar)
 {

@@ -241,6 +241,18 @@ int csg_device_info(int device, char *name, int len, int *compute_units) {
     return COSMOSIM_OK;
 }
 
+int csg_device_memory(int device, unsigned long long *free_bytes, unsigned long long *total_bytes) {
+    int k = 0;
+    if (csg_device_count(&k) != COSMOSIM_OK || device < 0 || device >= k)
+        return COSMOSIM_E_NO_DEVICE;
+    CSG_CHECK(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    CSG_CHECK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return COSMOSIM_OK;
+}
+
 int csg_create(int device, int capacity, csg_ctx **out) {
     if (!out || capacity < 0) return COSMOSIM_E_ARG;
     *out = nullptr;

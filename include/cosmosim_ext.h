@@ -138,6 +138,9 @@ int cosmosim_ext_replay(SimHandle h, const double *pos0, const double *vel0, dou
 
 /* Name and compute-unit count of the HIP device (name buffer of len bytes). */
 int cosmosim_ext_device_info(int device, char *name, int len, int *compute_units);
+/* Free and total device memory in bytes. */
+int cosmosim_ext_device_memory(int device, unsigned long long *free_bytes,
+                               unsigned long long *total_bytes);
 
 #ifdef __cplusplus
 }

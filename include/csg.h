@@ -34,6 +34,8 @@ typedef struct {
 
 int  csg_device_count(int *count);
 int  csg_device_info(int device, char *name, int len, int *compute_units);
+int  csg_device_memory(int device, unsigned long long *free_bytes,
+                       unsigned long long *total_bytes);
 int  csg_create(int device, int capacity, csg_ctx **out);
 void csg_destroy(csg_ctx *ctx);
 int  csg_reserve(csg_ctx *ctx, int capacity);
