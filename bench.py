@@ -150,6 +150,20 @@ def adjoint_runs(ca, big_n, big_steps):
     return out
 
 
+def copy_calibration(nbytes=1 << 30, launches=5):
+    """A plain device copy of known size, to scale rocprofv3's FETCH_SIZE /
+    WRITE_SIZE counters: torch's elementwise negation of a float32 array
+    (nbytes read and nbytes written per launch)."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(launches):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"kernel": "neg", "bytes_read": nbytes,
+                                           "bytes_written": nbytes, "launches": launches}}))
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "walk_fp32_1M_merger_traffic.json")
     if not os.path.exists(p):
@@ -170,12 +184,17 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-bodies", type=int, default=4_000_000)
     ap.add_argument("--adjoint-steps", type=int, default=20)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB copy used to scale the traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import cosmosim_amd as ca
     dev_name, cus = ca.device_info(0)
     bodies = ca.generate(ca.IC_MERGER, N_BODIES, SEED)
@@ -226,7 +245,7 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "k_walk<WALK_FP32>",
+            "kernel": "k_walk32w",
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,

@@ -114,6 +114,21 @@ __global__ __launch_bounds__(CSG_TPB) void k_fill_f64(double *__restrict__ p, si
     for (size_t i = blockIdx.x * (size_t)CSG_TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * CSG_TPB)
         p[i] = v;
 }
+__global__ __launch_bounds__(CSG_TPB) void k_fill_i32(int *__restrict__ p, size_t n, int v) {
+    for (size_t i = blockIdx.x * (size_t)CSG_TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * CSG_TPB)
+        p[i] = v;
+}
+// Fills by kernel (not memset nodes), so the sequences stay valid under stream capture.
+static void fill_f64(double *p, size_t n, double v, hipStream_t st) {
+    if (n == 0) return;
+    size_t b = (n + CSG_TPB - 1) / CSG_TPB;
+    k_fill_f64<<<(unsigned)(b < 4096 ? b : 4096), CSG_TPB, 0, st>>>(p, n, v);
+}
+static void fill_i32(int *p, size_t n, int v, hipStream_t st) {
+    if (n == 0) return;
+    size_t b = (n + CSG_TPB - 1) / CSG_TPB;
+    k_fill_i32<<<(unsigned)(b < 4096 ? b : 4096), CSG_TPB, 0, st>>>(p, n, v);
+}
 
 // Step 2: sum each node's records in sorted (node, wave, visit) order.
 __global__ __launch_bounds__(CSG_TPB) void k_adj_node_reduce(const uint64_t *__restrict__ key,
@@ -477,7 +492,7 @@ static int walk_taped(csg_ctx *c, int precision, double G, double softening, dou
         if (s) return s;
         const int nw = at_least1(blocks_for(c->n, 64));
         k_set_i32<<<1, 1, 0, c->stream>>>(c->tape_ovf, 0);
-        hipMemsetD32Async((hipDeviceptr_t)c->tcount, 0, (size_t)nw, c->stream);
+        fill_i32(c->tcount, (size_t)nw, 0, c->stream);
         TapeOut t{c->tape, c->tcount, c->tape_cap, c->tape_ovf};
         if ((s = launch_walk(c, precision, G, softening, theta, t))) return s;
         int ovf = 0;
@@ -505,10 +520,11 @@ static int force_vjp_dev(csg_ctx *c, const double *w_dev, double *xbar_dev, int 
     if (s) return s;
     const int nl = c->hbp->n_live, nn = c->hbp->n_nodes;
     if (nl == 0) {
-        if (!accumulate) CSG_CHECK(hipMemsetAsync(xbar_dev, 0, sizeof(double) * 3 * n, st));
+        if (!accumulate) fill_f64(xbar_dev, 3 * (size_t)n, 0.0, st);
+        CSG_LAUNCHED();
         return COSMOSIM_OK;
     }
-    if (!accumulate) CSG_CHECK(hipMemsetAsync(xbar_dev, 0, sizeof(double) * 3 * n, st));
+    if (!accumulate) fill_f64(xbar_dev, 3 * (size_t)n, 0.0, st);
     const int nw = blocks_for(nl, 64);
     k_adj_gather_w<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, w_dev, c->bp, c->ws);
     k_tape_scan<<<1, CSG_TPB, 0, st>>>(c->tcount, nw, c->toff, c->tape_total);
@@ -807,9 +823,10 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
     int s;
     if ((s = upload3(c, c->xb, gx))) return s;
     if (gv) { if ((s = upload3(c, c->vb, gv))) return s; }
-    else CSG_CHECK(hipMemsetAsync(c->vb, 0, sizeof(double) * 3 * n, st));
-    CSG_CHECK(hipMemsetAsync(c->ab, 0, sizeof(double) * 3 * n, st));
-    CSG_CHECK(hipMemsetAsync(c->u64tmp, 0, sizeof(unsigned long long), st));
+    else fill_f64(c->vb, 3 * (size_t)n, 0.0, st);
+    fill_f64(c->ab, 3 * (size_t)n, 0.0, st);
+    k_set_u64<<<1, 1, 0, st>>>(c->u64tmp, 0ull);
+    CSG_LAUNCHED();
     const int g = blocks_for(n, CSG_TPB);
     // positions x_{s0+1} .. x_{s1} of one segment (mode 0)
     if (mode == 0 && c->seg_slots < every) {

@@ -75,7 +75,8 @@ struct csg_ctx {
     int n_frozen = 0;
     double rec_dt = 0, rec_G = 1, rec_eps = 0, rec_theta = 0;
     int walk_group = 8;     // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
-    int walk_group32 = 64;  // FP32 walk
+    int walk_group32 = 0;   // FP32 walk: 0 = wave-mask traversal, else lanes per group
+    int counts_valid = 0;   // icount holds the per-body counts of the last walk
     int xcd_chunk = 16;     // walk tiles per XCD chunk (0: one contiguous range per XCD)
 };
 
@@ -269,7 +270,7 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     if (const char *g = getenv("COSMOSIM_XCD_CHUNK")) c->xcd_chunk = atoi(g);
     if (const char *g = getenv("COSMOSIM_WALK_GROUP32")) {
         const int v = atoi(g);
-        if (v == 8 || v == 16 || v == 32 || v == 64) c->walk_group32 = v;
+        if (v == 0 || v == 8 || v == 16 || v == 32 || v == 64) c->walk_group32 = v;
     }
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -346,6 +347,7 @@ int csg_reserve(csg_ctx *c, int capacity) {
     for (int a = 0; a < 10; ++a) dfree(old[a]);
     dfree(old_aos);
     c->tree_valid = 0;
+    c->counts_valid = 0;
     if (s != COSMOSIM_OK) c->n = 0;
     return s;
 }
@@ -359,6 +361,7 @@ int csg_set_bodies(csg_ctx *c, const Body *bodies, int n) {
     if (s != COSMOSIM_OK) return s;
     c->n = n;
     c->tree_valid = 0;
+    c->counts_valid = 0;
     if (n == 0) return COSMOSIM_OK;
     CSG_CHECK(hipMemcpyAsync(c->aos, bodies, sizeof(Body) * (size_t)n, hipMemcpyHostToDevice,
                              c->stream));
@@ -552,10 +555,19 @@ static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, dou
         k_walk32<GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes32, c->bp, c->idx, c->x, c->y, c->z, c->ax, c->ay,
                                           c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
 }
+
+template <bool LC, bool TP, bool O32>
+static void launch_walk32w(csg_ctx *c, double G, double eps2, double th2, int nb8, TapeOut tape) {
+    k_walk32w<LC, TP, O32><<<nb8, CSG_TPB, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x, c->y,
+        c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
+}
 }  // extern "C++"
 
+// lane_counts: record per-body interaction counts (c->icount); walks inside the
+// substep loops skip them (the total is always kept).
 static int launch_walk(csg_ctx *c, int precision, double G, double softening, double theta,
-                       TapeOut tape = TapeOut{nullptr, nullptr, 0, nullptr}) {
+                       TapeOut tape = TapeOut{nullptr, nullptr, 0, nullptr},
+                       bool lane_counts = true) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const double eps2 = softening * softening, th2 = theta * theta;
@@ -566,8 +578,29 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     const int gran = c->xcd_chunk > 0 ? 8 * c->xcd_chunk : 8;
     nb8 = (nb8 + gran - 1) / gran * gran;
     if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
+    c->counts_valid = 0;
+    if (precision == COSMOSIM_PREC_FP32 && (c->walk_group32 == 0 || tape.rec)) {
+        // wave-mask traversal (also the taped FP32 walk: the tape is per wavefront)
+        const bool off32 = (size_t)c->node_cap * sizeof(WNode32) < ((size_t)1 << 32);
+        const bool taping = tape.rec != nullptr;
+        const int v = (lane_counts ? 4 : 0) | (taping ? 2 : 0) | (off32 ? 1 : 0);
+        switch (v) {
+        case 0: launch_walk32w<false, false, false>(c, G, eps2, th2, nb8, tape); break;
+        case 1: launch_walk32w<false, false, true>(c, G, eps2, th2, nb8, tape); break;
+        case 2: launch_walk32w<false, true, false>(c, G, eps2, th2, nb8, tape); break;
+        case 3: launch_walk32w<false, true, true>(c, G, eps2, th2, nb8, tape); break;
+        case 4: launch_walk32w<true, false, false>(c, G, eps2, th2, nb8, tape); break;
+        case 5: launch_walk32w<true, false, true>(c, G, eps2, th2, nb8, tape); break;
+        case 6: launch_walk32w<true, true, false>(c, G, eps2, th2, nb8, tape); break;
+        default: launch_walk32w<true, true, true>(c, G, eps2, th2, nb8, tape); break;
+        }
+        CSG_LAUNCHED();
+        c->counts_valid = lane_counts ? 1 : 0;
+        return COSMOSIM_OK;
+    }
     // a taped walk runs with whole-wave traversal (the tape is per wavefront)
     const int gs = tape.rec ? 64 : (precision == COSMOSIM_PREC_FP32 ? c->walk_group32 : c->walk_group);
+    c->counts_valid = 1;
     switch (gs) {
     case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, nb8, tape); break;
     case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, nb8, tape); break;
@@ -604,6 +637,7 @@ static int check_overflow(csg_ctx *c) {
     collect_timers(c);
     if (c->hbp->overflow) {
         c->tree_valid = 0;
+        c->counts_valid = 0;
         return COSMOSIM_E_CAPACITY;
     }
     return COSMOSIM_OK;
@@ -644,7 +678,7 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
         if ((s = launch_build(c))) return s;
         timer_stop(c, PH_BUILD);
         timer_start(c);
-        if ((s = launch_walk(c, precision, G, softening, theta))) return s;
+        if ((s = launch_walk(c, precision, G, softening, theta, TapeOut{}, false))) return s;
         timer_stop(c, PH_WALK);
         timer_start(c);
         // end-of-substep half kick fused with the next substep's kick + drift
@@ -706,6 +740,7 @@ int csg_walk_pool(csg_ctx *c, const OctreeNode *pool, int pool_size, int root, d
             c->icount, G, softening * softening, theta, c->err_dev);
     }
     CSG_LAUNCHED();
+    c->counts_valid = 1;
     int err = 0;
     CSG_CHECK(hipMemcpyAsync(&err, c->err_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     CSG_CHECK(hipStreamSynchronize(c->stream));
@@ -790,6 +825,7 @@ int csg_interaction_counts(csg_ctx *c, int *counts, int n) {
     CSG_USE(c);
     if (n != c->n || (n > 0 && !counts)) return COSMOSIM_E_ARG;
     if (n == 0) return COSMOSIM_OK;
+    if (!c->counts_valid) return COSMOSIM_E_STATE;
     CSG_CHECK(hipMemcpyAsync(counts, c->icount, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
     CSG_CHECK(hipStreamSynchronize(c->stream));
     return COSMOSIM_OK;

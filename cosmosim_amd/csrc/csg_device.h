@@ -58,21 +58,18 @@ struct __align__(64) WNode {
 };
 static_assert(sizeof(WNode) == 64, "WNode is one 64-byte record");
 
-// The same node for the FP32 walk (48 bytes).  The centre of mass relative to
-// the root-cube centre R is formed in FP64 and stored as an FP32 pair
-// (h = fp32(C - R), l = fp32((C - R) - h)), which carries it to ~1e-14; a lane
-// forms its body-node offset as (h - xh) + (l - xl), exact up to one FP32
-// rounding of the offset itself.
-struct __align__(16) WNode32 {
-    float hx, hy, hz;  // high part of C - R
-    float m;           // total mass
-    float lx, ly, lz;  // low part of C - R
-    float s2;          // (2h)^2; -1 for a leaf
+// The same node for the FP32 walk (40 bytes).  The centre of mass stays FP64:
+// a lane forms its body-node offset as fp32(C - x) with the subtraction in FP64
+// (full rate on CDNA4), so the offset carries a single FP32 rounding however far
+// the pair sits from the origin.  Mass and (2h)^2 are FP32.
+struct __align__(8) WNode32 {
+    double cx, cy, cz;  // centre of mass
+    float m;            // total mass
+    float s2;           // (2h)^2; -1 for a leaf
     int skip;
     int body;
-    int pad0, pad1;
 };
-static_assert(sizeof(WNode32) == 48, "WNode32 is one 48-byte record");
+static_assert(sizeof(WNode32) == 40, "WNode32 is one 40-byte record");
 
 // One record of a walk tape: a node visited by a wavefront and the lanes
 // (bodies) that interacted with it there.  A tape is the frozen list of
@@ -111,23 +108,18 @@ __device__ __forceinline__ void tape_close(const TapeOut &t, int wave, int lane,
     if (nrec > t.cap) atomicOr(t.overflow, 1);
 }
 
-__device__ __forceinline__ void split_f32(double v, float &h, float &l) {
-    h = (float)v;
-    l = (float)(v - (double)h);
-}
-
 __device__ __forceinline__ void store_nodes(WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
                                             int id, const WNode &w, const double *root) {
+    (void)root;
     nodes[id] = w;
     WNode32 f;
-    split_f32(w.cx - root[0], f.hx, f.lx);
-    split_f32(w.cy - root[1], f.hy, f.ly);
-    split_f32(w.cz - root[2], f.hz, f.lz);
+    f.cx = w.cx;
+    f.cy = w.cy;
+    f.cz = w.cz;
     f.m = (float)w.m;
     f.s2 = w.body >= 0 ? -1.0f : (float)w.s2;  // a leaf passes every opening test
     f.skip = w.skip;
     f.body = w.body;
-    f.pad0 = f.pad1 = 0;
     n32[id] = f;
 }
 
@@ -1072,11 +1064,12 @@ __device__ __forceinline__ void kahan_add(float &s, float &k, float t) {
 #define CSG_FOLD_EVERY 16
 
 // FP32 walk over WNode32 records, same traversal as k_walk (groups of GS lanes).
-// Offsets: the body's position relative to the root centre is split into an
-// FP32 pair once (formed in FP64), so d = (h_node - h_body) + (l_node - l_body).
-// Accumulation is compensated in blocks: each lane adds its terms in FP32 into
-// a block sum, and every CSG_FOLD_EVERY node steps (a wave-uniform count) the
-// block sums are Kahan-added into the running sums.  G multiplies the result.
+// Offsets: d = fp32(C - x) with the subtraction in FP64 (one FP32 rounding of
+// the offset).  Accumulation is compensated in blocks: each lane adds its terms
+// in FP32 into a block sum, and every CSG_FOLD_EVERY node steps (a wave-uniform
+// count) the block sums are Kahan-added into the running sums.  G multiplies
+// the result.  k_walk32w is the default one-traversal-per-wavefront form; this
+// grouped form serves GS = 8, 16, 32 (and 64 with an explicit DFS position).
 template <int GS>
 __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
@@ -1096,10 +1089,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     const bool taping = (GS == 64) && tape.rec != nullptr;
     int nrec = 0;
     const int me = act ? sidx[b] : -1;
-    float xh, xl, yh, yl, zh, zl;
-    split_f32(act ? px[me] - bp->root[0] : 0.0, xh, xl);
-    split_f32(act ? py[me] - bp->root[1] : 0.0, yh, yl);
-    split_f32(act ? pz[me] - bp->root[2] : 0.0, zh, zl);
+    const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
     const float eps2f = (float)eps2, th2f = (float)theta2;
     const unsigned long long gmask =
         (GS == 64) ? ~0ull : (((1ull << (GS & 63)) - 1ull) << (lane & ~(GS - 1)));
@@ -1120,9 +1110,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
         const WNode32 nd = nodes[alive ? ii : 0];
         // branch-free step: lanes whose DFS position is not ii compute and discard
         const bool here = alive && p == ii;
-        const float dx = (nd.hx - xh) + (nd.lx - xl);
-        const float dy = (nd.hy - yh) + (nd.ly - yl);
-        const float dz = (nd.hz - zh) + (nd.lz - zl);
+        const float dx = (float)(nd.cx - x);
+        const float dy = (float)(nd.cy - y);
+        const float dz = (float)(nd.cz - z);
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         // leaves carry s2 = -1, so crit holds for them and only self is skipped
         const bool crit = nd.s2 < th2f * d2;
@@ -1157,6 +1147,106 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if (lane == 0) atomicAdd(itotal, c);
+}
+
+// Lane select on a wave mask held in SGPRs: m's bit for this lane ? t : f.
+__device__ __forceinline__ float lane_sel_f32(unsigned long long m, float t, float f) {
+    float r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(f), "v"(t), "s"(m));
+    return r;
+}
+__device__ __forceinline__ int lane_sel_i32(unsigned long long m, int t, int f) {
+    int r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(f), "v"(t), "s"(m));
+    return r;
+}
+
+// FP32 walk, one traversal per wavefront (k_walk32<64>'s traversal written on
+// wave masks); same offsets, opening test, interaction and compensated
+// accumulation as k_walk32.  The lane state is one integer p: a lane takes part
+// in node ii while p <= ii, and accepting ii sets p = skip(ii).  A lane that
+// descends keeps p, since the wave then moves to ii + 1 (any descending lane
+// makes the whole wave descend); a waiting lane accepted a node J with ii inside
+// J's subtree, so p = skip(J) >= skip(ii) and the wave cannot jump past it.
+// Hence "p <= ii" holds exactly when the lane's own depth-first walk is at ii,
+// as with the explicit position of k_walk32.
+// Compares produce SGPR lane masks directly (the compare builtins), the
+// participation / acceptance / descent logic runs on the scalar unit, and the
+// two per-lane selects take those masks.
+//   LANE_COUNTS: write per-body interaction counts (the wave total is always
+//                kept, on the scalar unit);
+//   TAPING:      record the walk tape;
+//   OFF32:       the node array spans < 4 GiB, so records are addressed by a
+//                32-bit byte offset (scalar loads with an SGPR offset).
+template <bool LANE_COUNTS, bool TAPING, bool OFF32>
+__global__ __launch_bounds__(CSG_TPB) void k_walk32w(
+    const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
+    const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
+    double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
+    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape) {
+    const int n_live = bp->n_live;
+    const int nnodes = bp->n_nodes;
+    if (bp->overflow) return;
+    const int lane = threadIdx.x & 63;
+    const int tile = xcd_tile(nb8, chunk);
+    const int b0 = (tile * CSG_TPB + (int)(threadIdx.x & ~63));
+    if (b0 >= n_live) return;
+    const int b = b0 + lane;
+    const bool act = b < n_live;
+    int nrec = 0;
+    const int me = act ? sidx[b] : -1;
+    const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
+    const float eps2f = (float)eps2, th2f = (float)theta2;
+    const char *const base = (const char *)nodes;
+    int p = act ? 0 : 0x7fffffff;
+    int cnt = 0;
+    unsigned long long wcnt = 0;
+    float bx = 0.f, by = 0.f, bz = 0.f;               // block sums
+    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+    int ii = 0, steps = 0;
+    while (ii < nnodes) {
+        const WNode32 nd = OFF32
+            ? *(const WNode32 *)(base + (unsigned)ii * (unsigned)sizeof(WNode32))
+            : nodes[ii];
+        const float dx = (float)(nd.cx - x);
+        const float dy = (float)(nd.cy - y);
+        const float dz = (float)(nd.cz - z);
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        // compare predicates: 41 = signed <=, 4 = ordered <, 33 = !=
+        const unsigned long long H = __builtin_amdgcn_sicmp(p, ii, 41);
+        // leaves carry s2 = -1, so the criterion holds for them
+        const unsigned long long C = __builtin_amdgcn_fcmpf(nd.s2, th2f * d2, 4);
+        const unsigned long long S = __builtin_amdgcn_sicmp(nd.body, me, 33);
+        const unsigned long long A = H & C;          // accepted (own leaf included)
+        const unsigned long long I = A & S;          // interacting
+        const unsigned long long D = H & ~C;         // descending
+        const float yv = __builtin_amdgcn_rsqf(d2 + eps2f);
+        const float f = lane_sel_f32(I, nd.m * (yv * yv * yv), 0.0f);
+        bx = fmaf(dx, f, bx);
+        by = fmaf(dy, f, by);
+        bz = fmaf(dz, f, bz);
+        wcnt += (unsigned long long)__builtin_popcountll(I);
+        if constexpr (LANE_COUNTS) cnt += (int)((I >> lane) & 1ull);
+        if constexpr (TAPING) tape_push(tape, b0 >> 6, lane, nrec, I, ii);
+        p = lane_sel_i32(A, nd.skip, p);
+        if (++steps == CSG_FOLD_EVERY) {
+            steps = 0;
+            kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
+            bx = by = bz = 0.f;
+        }
+        const int nxt = D ? ii + 1 : nd.skip;
+        ii = (nxt > ii) ? nxt : ii + 1;  // strictly increasing: the loop always ends
+    }
+    kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
+    if (act) {
+        ax[me] = G * ((double)sx - (double)kx);
+        ay[me] = G * ((double)sy - (double)ky);
+        az[me] = G * ((double)sz - (double)kz);
+        if constexpr (LANE_COUNTS) icount[me] = cnt;
+    }
+    if constexpr (TAPING) tape_close(tape, b0 >> 6, lane, nrec);
+    if (lane == 0) atomicAdd(itotal, wcnt);
 }
 
 // Dead bodies get a = 0 (src/octree.c:227-231).

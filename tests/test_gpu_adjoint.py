@@ -37,11 +37,14 @@ def test_force_vjp_matches_frozen_finite_differences(theta):
         assert abs(fd - np.sum(xb * d)) <= 1e-6 * abs(fd)
 
 
-def test_tree_vjp_equals_direct_vjp_at_theta0():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_tree_vjp_equals_direct_vjp_at_theta0(precision):
+    """At theta = 0 every interaction is body-body; the tape of either walk
+    (FP64 or the FP32 wave-mask walk) then gives the direct-sum adjoint."""
     rng = np.random.default_rng(2)
     b = ca.generate(ca.IC_MERGER, 2048, 5)
     b["mass"][::97] = 0.0
-    sim = ca.Simulation(b, softening=0.02, theta=0.0, precision="fp64")
+    sim = ca.Simulation(b, softening=0.02, theta=0.0, precision=precision)
     w = rng.normal(size=(2048, 3))
     xt = sim.force_vjp(w)
     xd = sim.direct_vjp(w)

@@ -236,6 +236,41 @@ def test_runs_are_deterministic():
         assert same_bits(u, v)
 
 
+def test_fp32_traversal_forms_agree(monkeypatch):
+    """The default FP32 walk (one traversal per wavefront on lane masks) and the
+    grouped walk with an explicit DFS position at 64 lanes visit the same nodes in
+    the same order with the same arithmetic: bit-identical accelerations, equal
+    counts.  Narrower groups fold their compensated sums at other steps: equal
+    counts, accelerations within FP32 rounding."""
+    b = ca.generate(ca.IC_MERGER, 50000, 7)
+    b["mass"][::101] = 0.0
+    res = {}
+    for gs in ["0", "64", "32", "8"]:
+        monkeypatch.setenv("COSMOSIM_WALK_GROUP32", gs)
+        sim = sim_for(b, 0.005, 0.6, precision="fp32")
+        res[gs] = forces_on(sim, 0.6, "fp32")
+        sim.close()
+    assert same_bits(res["64"][0], res["0"][0])
+    for gs in ["64", "32", "8"]:
+        assert np.array_equal(res[gs][1], res["0"][1])
+        assert rel_err(res[gs][0], res["0"][0]).max() <= 1e-5
+    assert np.all(res["0"][0][b["mass"] <= 0] == 0.0)
+
+
+def test_counts_come_from_explicit_walks():
+    """Substep loops keep only the total; per-body counts of a loop's walk are
+    refused rather than returned stale."""
+    b = ca.generate(ca.IC_DISK, 2000, 1)
+    sim = sim_for(b, 0.01, 0.5, precision="fp32")
+    assert sim.interaction_counts().sum() == sim.stats().interactions
+    sim.advance(2)
+    with pytest.raises(ca.CosmosimError):
+        sim.interaction_counts()
+    assert sim.stats().interactions > 0
+    sim.compute_forces()
+    assert sim.interaction_counts().sum() == sim.stats().interactions
+
+
 # ------------------------------------------------------------ handle API
 def test_api_handle_like_reference_tests():
     """tests/test_physics.c:1048-1111 restated."""
