@@ -77,6 +77,10 @@ struct csg_ctx {
     int walk_group = 8;     // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
     int walk_group32 = 0;   // FP32 walk: 0 = wave-mask traversal, else lanes per group
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
+    int *wcost = nullptr;   // FP32 wave-mask walk: node steps of each wave in the last walk
+    int *worder = nullptr;  // dispatch order built from wcost (longest first)
+    int wcost_n = 0;        // number of waves wcost describes (0: none)
+    int wave_order = 1;     // dispatch the wave-mask walk longest first
     int xcd_chunk = 16;     // walk tiles per XCD chunk (0: one contiguous range per XCD)
 };
 
@@ -125,6 +129,8 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->depth_part);
     dfree(c->runs); dfree(c->lcp); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
+    dfree(c->wcost); dfree(c->worder);
+    c->wcost_n = 0;
     c->cap = 0;
     c->node_cap = 0;
 }
@@ -147,7 +153,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->lcp, C1) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
               dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
-              dalloc(c->icount, C);
+              dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, C / 64 + 1);
     if (!ok) {
         (void)hipGetLastError();
         free_buffers(c);
@@ -268,6 +274,7 @@ int csg_create(int device, int capacity, csg_ctx **out) {
         if (v == 8 || v == 16 || v == 32 || v == 64) c->walk_group = v;
     }
     if (const char *g = getenv("COSMOSIM_XCD_CHUNK")) c->xcd_chunk = atoi(g);
+    if (const char *g = getenv("COSMOSIM_WAVE_ORDER")) c->wave_order = atoi(g) != 0;
     if (const char *g = getenv("COSMOSIM_WALK_GROUP32")) {
         const int v = atoi(g);
         if (v == 0 || v == 8 || v == 16 || v == 32 || v == 64) c->walk_group32 = v;
@@ -362,6 +369,7 @@ int csg_set_bodies(csg_ctx *c, const Body *bodies, int n) {
     c->n = n;
     c->tree_valid = 0;
     c->counts_valid = 0;
+    c->wcost_n = 0;
     if (n == 0) return COSMOSIM_OK;
     CSG_CHECK(hipMemcpyAsync(c->aos, bodies, sizeof(Body) * (size_t)n, hipMemcpyHostToDevice,
                              c->stream));
@@ -557,9 +565,11 @@ static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, dou
 }
 
 template <bool LC, bool TP, bool O32>
-static void launch_walk32w(csg_ctx *c, double G, double eps2, double th2, int nb8, TapeOut tape) {
-    k_walk32w<LC, TP, O32><<<nb8, CSG_TPB, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x, c->y,
-        c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
+static void launch_walk32w(csg_ctx *c, double G, double eps2, double th2, int grid, int nb8,
+                           TapeOut tape, WaveOrder wo) {
+    k_walk32w<LC, TP, O32><<<grid, CSG_TPB, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x,
+        c->y, c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk,
+        tape, wo);
 }
 }  // extern "C++"
 
@@ -583,17 +593,27 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         // wave-mask traversal (also the taped FP32 walk: the tape is per wavefront)
         const bool off32 = (size_t)c->node_cap * sizeof(WNode32) < ((size_t)1 << 32);
         const bool taping = tape.rec != nullptr;
+        // longest-first dispatch from the previous walk's per-wave step counts
+        const int nw = blocks_for(n, 64);
+        WaveOrder wo{nullptr, nw, c->wcost};
+        int grid = nb8;
+        if (c->wave_order && c->wcost_n == nw) {
+            k_wave_order<<<1, 1024, 0, st>>>(c->wcost, nw, c->worder);
+            wo.order = c->worder;
+            grid = blocks_for(nw, CSG_TPB / 64);
+        }
         const int v = (lane_counts ? 4 : 0) | (taping ? 2 : 0) | (off32 ? 1 : 0);
         switch (v) {
-        case 0: launch_walk32w<false, false, false>(c, G, eps2, th2, nb8, tape); break;
-        case 1: launch_walk32w<false, false, true>(c, G, eps2, th2, nb8, tape); break;
-        case 2: launch_walk32w<false, true, false>(c, G, eps2, th2, nb8, tape); break;
-        case 3: launch_walk32w<false, true, true>(c, G, eps2, th2, nb8, tape); break;
-        case 4: launch_walk32w<true, false, false>(c, G, eps2, th2, nb8, tape); break;
-        case 5: launch_walk32w<true, false, true>(c, G, eps2, th2, nb8, tape); break;
-        case 6: launch_walk32w<true, true, false>(c, G, eps2, th2, nb8, tape); break;
-        default: launch_walk32w<true, true, true>(c, G, eps2, th2, nb8, tape); break;
+        case 0: launch_walk32w<false, false, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+        case 1: launch_walk32w<false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+        case 2: launch_walk32w<false, true, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+        case 3: launch_walk32w<false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+        case 4: launch_walk32w<true, false, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+        case 5: launch_walk32w<true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+        case 6: launch_walk32w<true, true, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+        default: launch_walk32w<true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
         }
+        c->wcost_n = nw;
         CSG_LAUNCHED();
         c->counts_valid = lane_counts ? 1 : 0;
         return COSMOSIM_OK;

@@ -71,6 +71,14 @@ struct __align__(8) WNode32 {
 };
 static_assert(sizeof(WNode32) == 40, "WNode32 is one 40-byte record");
 
+// Dispatch order of the wave-mask walk: wave slot s walks wave order[s]
+// (s < n); cost (may be null) receives each wave's node-step count.
+struct WaveOrder {
+    const int *order;
+    int n;
+    int *cost;
+};
+
 // One record of a walk tape: a node visited by a wavefront and the lanes
 // (bodies) that interacted with it there.  A tape is the frozen list of
 // interactions of one force evaluation, in each wave's visiting order.
@@ -1149,6 +1157,57 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     if (lane == 0) atomicAdd(itotal, c);
 }
 
+// Longest-first dispatch order for the wave-mask walk, from each wave's node-step
+// count in the previous walk (longest processing time first: long traversals
+// start early and short ones fill the end of the grid, which otherwise drains
+// with few waves per SIMD).  Counting sort on the count quantized to 4096
+// levels, descending; the order inside a level is whatever the LDS atomics give,
+// which changes only the timing.  One block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cost, int nw,
+                                                     int *__restrict__ order) {
+    __shared__ int hist[4096];
+    __shared__ int part[16];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    int mx = 0;
+    for (int i = t; i < nw; i += 1024) mx = max(mx, cost[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) part[wid] = mx;
+    for (int i = t; i < 4096; i += 1024) hist[i] = 0;
+    __syncthreads();
+    mx = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) mx = max(mx, part[k]);
+    int shift = 0;
+    while ((mx >> shift) >= 4096) ++shift;
+    for (int i = t; i < nw; i += 1024)
+        atomicAdd(&hist[4095 - (max(cost[i], 0) >> shift)], 1);
+    __syncthreads();
+    // exclusive scan of the 4096 bins: 4 per thread, then across threads
+    int v[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { v[k] = hist[4 * t + k]; sum += v[k]; }
+    int inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+    }
+    __syncthreads();
+    if (lane == 63) part[wid] = inc;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < wid; ++k) base += part[k];
+    int run = base + inc - sum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { hist[4 * t + k] = run; run += v[k]; }
+    __syncthreads();
+    for (int i = t; i < nw; i += 1024) {
+        const int pos = atomicAdd(&hist[4095 - (max(cost[i], 0) >> shift)], 1);
+        order[pos] = i;
+    }
+}
+
 // Lane select on a wave mask held in SGPRs: m's bit for this lane ? t : f.
 __device__ __forceinline__ float lane_sel_f32(unsigned long long m, float t, float f) {
     float r;
@@ -1178,20 +1237,35 @@ __device__ __forceinline__ int lane_sel_i32(unsigned long long m, int t, int f) 
 //   TAPING:      record the walk tape;
 //   OFF32:       the node array spans < 4 GiB, so records are addressed by a
 //                32-bit byte offset (scalar loads with an SGPR offset).
+// Wave w walks sorted bodies [64 w, 64 w + 64).  Without an order, w follows the
+// XCD-chunked tile mapping; with one (WaveOrder), wave slot s of the grid takes
+// w = order[s] (longest first, see k_wave_order).  Either way the results do not
+// depend on the dispatch order.  With WaveOrder::cost set, every wave stores its
+// node-step count there.
 template <bool LANE_COUNTS, bool TAPING, bool OFF32>
 __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
     const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
     const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
     double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
-    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape) {
+    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape, WaveOrder wo) {
     const int n_live = bp->n_live;
     const int nnodes = bp->n_nodes;
     if (bp->overflow) return;
     const int lane = threadIdx.x & 63;
-    const int tile = xcd_tile(nb8, chunk);
-    const int b0 = (tile * CSG_TPB + (int)(threadIdx.x & ~63));
-    if (b0 >= n_live) return;
+    int wv;
+    if (wo.order) {
+        const int slot = (int)blockIdx.x * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
+        if (slot >= wo.n) return;
+        wv = wo.order[slot];
+    } else {
+        wv = xcd_tile(nb8, chunk) * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
+    }
+    const int b0 = wv * 64;
+    if (b0 >= n_live) {
+        if (wo.cost && lane == 0 && wv < wo.n) wo.cost[wv] = 0;
+        return;
+    }
     const int b = b0 + lane;
     const bool act = b < n_live;
     int nrec = 0;
@@ -1204,7 +1278,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
     unsigned long long wcnt = 0;
     float bx = 0.f, by = 0.f, bz = 0.f;               // block sums
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
-    int ii = 0, steps = 0;
+    int ii = 0, steps = 0, folds = 0;
     while (ii < nnodes) {
         const WNode32 nd = OFF32
             ? *(const WNode32 *)(base + (unsigned)ii * (unsigned)sizeof(WNode32))
@@ -1232,12 +1306,14 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         p = lane_sel_i32(A, nd.skip, p);
         if (++steps == CSG_FOLD_EVERY) {
             steps = 0;
+            ++folds;
             kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
             bx = by = bz = 0.f;
         }
-        const int nxt = D ? ii + 1 : nd.skip;
-        ii = (nxt > ii) ? nxt : ii + 1;  // strictly increasing: the loop always ends
+        const bool down = D || nd.skip <= ii;
+        ii = down ? ii + 1 : nd.skip;  // strictly increasing: the loop always ends
     }
+    if (wo.cost && lane == 0) wo.cost[wv] = folds * CSG_FOLD_EVERY + steps;
     kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
     if (act) {
         ax[me] = G * ((double)sx - (double)kx);

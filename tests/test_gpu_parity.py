@@ -245,12 +245,15 @@ def test_fp32_traversal_forms_agree(monkeypatch):
     b = ca.generate(ca.IC_MERGER, 50000, 7)
     b["mass"][::101] = 0.0
     res = {}
-    for gs in ["0", "64", "32", "8"]:
-        monkeypatch.setenv("COSMOSIM_WALK_GROUP32", gs)
+    for gs in ["0", "64", "32", "8", "0-unordered"]:
+        monkeypatch.setenv("COSMOSIM_WALK_GROUP32", gs.split("-")[0])
+        monkeypatch.setenv("COSMOSIM_WAVE_ORDER", "0" if "unordered" in gs else "1")
         sim = sim_for(b, 0.005, 0.6, precision="fp32")
+        sim.advance(2)          # later walks dispatch longest-first from recorded costs
         res[gs] = forces_on(sim, 0.6, "fp32")
         sim.close()
     assert same_bits(res["64"][0], res["0"][0])
+    assert same_bits(res["0-unordered"][0], res["0"][0])
     for gs in ["64", "32", "8"]:
         assert np.array_equal(res[gs][1], res["0"][1])
         assert rel_err(res[gs][0], res["0"][0]).max() <= 1e-5
