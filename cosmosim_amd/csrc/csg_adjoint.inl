@@ -486,6 +486,9 @@ static int adj_reserve_records(csg_ctx *c, size_t R) {
 
 // Walk that records the tape; grows the per-wave tape when a wave overflows it.
 static int walk_taped(csg_ctx *c, int precision, double G, double softening, double theta) {
+    // the replay of the tape reads the FP64 node records
+    int s0 = ensure_nodes64(c, COSMOSIM_PREC_FP64);
+    if (s0) return s0;
     int cap = c->tape_cap > 0 ? c->tape_cap : 2048;
     for (int attempt = 0; attempt < 8; ++attempt) {
         int s = adj_reserve(c, cap);

@@ -77,6 +77,8 @@ struct csg_ctx {
     int walk_group = 8;     // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
     int walk_group32 = 0;   // FP32 walk: 0 = wave-mask traversal, else lanes per group
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
+    int nodes64_valid = 0;  // the FP64 node records belong to the current tree
+    double4 *xyzm = nullptr;  // build scratch: (x, y, z, m) interleaved, body order
     int *wcost = nullptr;   // FP32 wave-mask walk: node steps of each wave in the last walk
     int *worder = nullptr;  // dispatch order built from wcost (longest first)
     int wcost_n = 0;        // number of waves wcost describes (0: none)
@@ -129,7 +131,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->depth_part);
     dfree(c->runs); dfree(c->lcp); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
-    dfree(c->wcost); dfree(c->worder);
+    dfree(c->wcost); dfree(c->worder); dfree(c->xyzm);
     c->wcost_n = 0;
     c->cap = 0;
     c->node_cap = 0;
@@ -153,7 +155,8 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->lcp, C1) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
               dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
-              dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, C / 64 + 1);
+              dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, C / 64 + 1) &&
+              dalloc(c->xyzm, C);
     if (!ok) {
         (void)hipGetLastError();
         free_buffers(c);
@@ -477,6 +480,15 @@ int csg_set_state(csg_ctx *c, const double *pos, const double *vel, const double
     return COSMOSIM_OK;
 }
 
+static int launch_build(csg_ctx *c, bool with64 = true);
+
+// The FP64 node records of the current tree, re-emitted by a full build when the
+// last build wrote only the FP32 records (same positions, so the same tree).
+static int ensure_nodes64(csg_ctx *c, int precision) {
+    if (precision == COSMOSIM_PREC_FP32 || c->nodes64_valid) return COSMOSIM_OK;
+    return launch_build(c, true);
+}
+
 static int launch_kick_drift(csg_ctx *c, double dt, int kicks) {
     if (c->n == 0) return COSMOSIM_OK;
     k_kick_drift<<<blocks_for(c->n, CSG_TPB), CSG_TPB, 0, c->stream>>>(
@@ -494,7 +506,9 @@ static int launch_kick(csg_ctx *c, double dt) {
 }
 
 // Whole build on the stream; no host synchronisation.
-static int launch_build(csg_ctx *c) {
+// with64: also write the FP64 node records (the FP64 walks, the adjoint and the
+// tree export read them; the FP32 walk reads only the 40-byte records).
+static int launch_build(csg_ctx *c, bool with64) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const int nbb = n > 0 ? (blocks_for(n, CSG_TPB) < CSG_BBOX_BLOCKS ? blocks_for(n, CSG_TPB)
@@ -505,7 +519,7 @@ static int launch_build(csg_ctx *c) {
     CSG_LAUNCHED();
     if (n > 0) {
         k_keys<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bp, c->key,
-                                                          c->idx);
+                                                          c->idx, c->xyzm);
         const int nb = blocks_for(n, CSG_TILE);
         uint64_t *kin = c->key, *kout = c->key2;
         int *vin = c->idx, *vout = c->idx2;
@@ -527,9 +541,8 @@ static int launch_build(csg_ctx *c) {
     }
     const int n1 = n + 1;
     if (n > 0)
-        k_gather_sorted<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, c->x, c->y, c->z, c->m,
-                                                                   c->bp, c->xs, c->ys, c->zs,
-                                                                   c->ms);
+        k_gather_sorted4<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, c->xyzm, c->bp, c->xs,
+                                                                    c->ys, c->zs, c->ms);
     const int nlb = blocks_for(n1, CSG_TPB);
     k_lcp<<<nlb, CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt, c->depth_part);
     const int nt = blocks_for(n1, CSG_TILE);
@@ -542,9 +555,11 @@ static int launch_build(csg_ctx *c) {
                                       c->cap + 1);
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
         c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->xs, c->ys, c->zs, c->ms,
-        c->bp, c->nodes, c->nodes32);
+        c->bp, with64 ? c->nodes : nullptr, c->nodes32);
     CSG_LAUNCHED();
     c->tree_valid = 1;
+    c->nodes64_valid = with64 ? 1 : 0;
+    c->tape_valid = 0;   // a tape describes the tree it was recorded on
     return COSMOSIM_OK;
 }
 
@@ -676,6 +691,8 @@ int csg_walk(csg_ctx *c, int precision, double G, double softening, double theta
     CSG_USE(c);
     if (!c->tree_valid) return COSMOSIM_E_STATE;
     if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
+    int s0 = ensure_nodes64(c, precision);
+    if (s0) return s0;
     timer_start(c);
     int s = launch_walk(c, precision, G, softening, theta);
     if (s) return s;
@@ -695,7 +712,7 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
     timer_stop(c, PH_KDK);
     for (int sub = 0; sub < k; ++sub) {
         timer_start(c);
-        if ((s = launch_build(c))) return s;
+        if ((s = launch_build(c, precision != COSMOSIM_PREC_FP32))) return s;
         timer_stop(c, PH_BUILD);
         timer_start(c);
         if ((s = launch_walk(c, precision, G, softening, theta, TapeOut{}, false))) return s;
@@ -713,8 +730,9 @@ int csg_export_pool(csg_ctx *c, OctreeNode *pool, int capacity, int *pool_size) 
     CSG_USE(c);
     if (!pool || !pool_size) return COSMOSIM_E_ARG;
     if (!c->tree_valid) return COSMOSIM_E_STATE;
-    int s = read_params(c);
+    int s = ensure_nodes64(c, COSMOSIM_PREC_FP64);
     if (s) return s;
+    if ((s = read_params(c))) return s;
     const int nn = c->hbp->n_nodes;
     if (c->hbp->overflow || nn > capacity) return COSMOSIM_E_CAPACITY;
     if ((size_t)nn > c->pool_dev_cap) {

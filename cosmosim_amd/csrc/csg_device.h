@@ -119,7 +119,7 @@ __device__ __forceinline__ void tape_close(const TapeOut &t, int wave, int lane,
 __device__ __forceinline__ void store_nodes(WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
                                             int id, const WNode &w, const double *root) {
     (void)root;
-    nodes[id] = w;
+    if (nodes) nodes[id] = w;   // FP64 records: skipped when only the FP32 walk reads the tree
     WNode32 f;
     f.cx = w.cx;
     f.cy = w.cy;
@@ -380,22 +380,27 @@ __global__ __launch_bounds__(CSG_TPB) void k_bbox_final(const double *__restrict
 // =================================================================== keys
 // Dead bodies get the all-ones key and sort behind every live body; their
 // acceleration is zeroed here (src/octree.c:227-231).
+// Also writes (x, y, z, m) interleaved, one 32-byte line per body, so the sorted
+// gather after the sort reads one sector per body instead of four.
 __global__ __launch_bounds__(CSG_TPB) void k_keys(const double *__restrict__ x,
                                                   const double *__restrict__ y,
                                                   const double *__restrict__ z,
                                                   const double *__restrict__ m, int n,
                                                   const BuildParams *__restrict__ bp,
                                                   uint64_t *__restrict__ key,
-                                                  int *__restrict__ idx) {
+                                                  int *__restrict__ idx,
+                                                  double4 *__restrict__ xyzm) {
     const int i = blockIdx.x * CSG_TPB + threadIdx.x;
     if (i >= n) return;
     uint64_t k = ~0ull;
-    if (live_mass(m[i])) {
+    const double px = x[i], py = y[i], pz = z[i], pm = m[i];
+    if (live_mass(pm)) {
         double cx = bp->root[0], cy = bp->root[1], cz = bp->root[2], h = bp->root[3];
-        k = path_word(x[i], y[i], z[i], cx, cy, cz, h, CSG_KEY_LEVELS);
+        k = path_word(px, py, pz, cx, cy, cz, h, CSG_KEY_LEVELS);
     }
     key[i] = k;
     idx[i] = i;
+    xyzm[i] = make_double4(px, py, pz, pm);
 }
 
 // ============================================================ radix sort
@@ -628,6 +633,17 @@ __global__ __launch_bounds__(CSG_TPB) void k_gather_sorted(
     if (b >= bp->n_live) return;
     const int i = idx[b];
     xs[b] = x[i]; ys[b] = y[i]; zs[b] = z[i]; ms[b] = m[i];
+}
+
+// The same from the interleaved copy written by k_keys.
+__global__ __launch_bounds__(CSG_TPB) void k_gather_sorted4(
+    const int *__restrict__ idx, const double4 *__restrict__ xyzm,
+    const BuildParams *__restrict__ bp, double *__restrict__ xs, double *__restrict__ ys,
+    double *__restrict__ zs, double *__restrict__ ms) {
+    const int b = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (b >= bp->n_live) return;
+    const double4 v = xyzm[idx[b]];
+    xs[b] = v.x; ys[b] = v.y; zs[b] = v.z; ms[b] = v.w;
 }
 
 // ============================================================ int scan

@@ -15,8 +15,8 @@ import pytest
 
 import cosmosim_amd as ca
 from oracle.cpu_ref import CpuRef
-from tests._util import (acc, from_hot, golden, hot, pool_mass_com_err, pools_same_structure,
-                         rel_err, same_bits)
+from tests._util import (acc, from_hot, golden, hot, pool_mass_com_err, pools_identical,
+                         pools_same_structure, rel_err, same_bits)
 
 pytestmark = pytest.mark.gpu
 ref = CpuRef()
@@ -272,6 +272,31 @@ def test_counts_come_from_explicit_walks():
     assert sim.stats().interactions > 0
     sim.compute_forces()
     assert sim.interaction_counts().sum() == sim.stats().interactions
+
+
+def test_fp64_consumers_after_fp32_substeps():
+    """FP32 substep loops write only the FP32 node records; the tree export, the
+    FP64 walk and the adjoint then re-emit the FP64 records of the same tree.  A
+    fresh handle at the same positions gives the same bits."""
+    b = ca.generate(ca.IC_MERGER, 20000, 9)
+    sim = sim_for(b, 0.005, 0.5, precision="fp32", dt=5e-4)
+    sim.advance(3)
+    pos, vel, _, _ = sim.state()
+    b2 = b.copy()
+    b2["x"], b2["y"], b2["z"] = pos[:, 0], pos[:, 1], pos[:, 2]
+    fresh = sim_for(b2, 0.005, 0.5, precision="fp64")
+    assert pools_identical(sim.export_tree(), fresh.export_tree())
+    sim.set_params(precision="fp64")
+    sim.compute_forces()
+    assert same_bits(sim.accelerations(), fresh.accelerations())
+    sim.set_params(precision="fp32")
+    sim.advance(1)
+    w = np.random.default_rng(4).normal(size=(20000, 3))
+    pos1 = sim.state()[0]
+    b3 = b.copy()
+    b3["x"], b3["y"], b3["z"] = pos1[:, 0], pos1[:, 1], pos1[:, 2]
+    fresh3 = sim_for(b3, 0.005, 0.5, precision="fp32")
+    assert same_bits(sim.force_vjp(w), fresh3.force_vjp(w))
 
 
 # ------------------------------------------------------------ handle API
