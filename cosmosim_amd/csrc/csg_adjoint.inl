@@ -10,10 +10,10 @@
 //   xbar_k += (m_k / M_J) cbar_J for every node J containing k (c_J = sum m x / M)
 // with T = G M r^-3 (I - 3 d d^T / r^2), r^2 = |d|^2 + eps^2.
 //   1. k_adj_walk replays each wave's tape: per-lane -T w, and per tape record the
-//      wave's lane sum of T w (a fixed butterfly order).
+//      wave's lane sum of T w (a fixed DPP reduction tree).
 //   2. The records are stable-radix-sorted by node id: the transposed
-//      interaction lists.  k_adj_node_reduce sums each node's records in that
-//      order: cbar without atomics, reproducibly.
+//      interaction lists.  k_seg_short / k_seg_long sum each node's records in a
+//      fixed order: cbar without atomics on the sums, reproducibly.
 //   3. The push-down is a difference array over each node's sorted member range
 //      [first, last], prefix-summed in double-double (k_adj_diff + k_dd_*).
 // Masses are not differentiated.
@@ -23,7 +23,7 @@ __device__ __forceinline__ void tidal_apply(double dx, double dy, double dz, dou
                                             double wz, double gm, double eps2, double &gx,
                                             double &gy, double &gz) {
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz)) + eps2;
-    const double ri = 1.0 / sqrt(r2);
+    const double ri = rsq64(r2);
     const double ri2 = ri * ri;
     const double s = gm * ri2 * ri;
     const double k = 3.0 * fma(dx, wx, fma(dy, wy, dz * wz)) * ri2;
@@ -61,13 +61,33 @@ __global__ __launch_bounds__(CSG_TPB) void k_tape_scan(const int *__restrict__ c
     if (threadIdx.x == 0) *total = carry;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
-    return v;
+// One DPP move of an FP64 value (as two 32-bit halves); lanes the control does
+// not write, or whose source lies outside the row, read 0.
+template <int CTRL, int ROWS, int BANKS>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, BANKS, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, BANKS, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes of a full wavefront, left in lane 63: shifts by 1, 2, 4, 8
+// inside each row of 16 lanes, then row 0 into row 1 and row 2 into row 3
+// (row_bcast:15), then rows 0-1 into rows 2-3 (row_bcast:31).  A fixed tree, so
+// the rounding is the same on every call; no LDS traffic.
+__device__ __forceinline__ double wave_sum63(double s) {
+    s += dpp_f64<0x111, 0xF, 0xF>(s);  // row_shr:1
+    s += dpp_f64<0x112, 0xF, 0xF>(s);  // row_shr:2
+    s += dpp_f64<0x114, 0xF, 0xE>(s);  // row_shr:4, banks 1-3
+    s += dpp_f64<0x118, 0xF, 0xC>(s);  // row_shr:8, banks 2-3
+    s += dpp_f64<0x142, 0xA, 0xF>(s);  // row_bcast:15 into rows 1 and 3
+    s += dpp_f64<0x143, 0xC, 0xF>(s);  // row_bcast:31 into rows 2 and 3
+    return s;
 }
 
 // Step 1: per-lane -T w and per-record lane sums (one wave per 64 sorted bodies).
+// The wave index is made uniform, so tape records and node records come by
+// scalar loads, four records per round (their latencies overlap).
 __global__ __launch_bounds__(CSG_TPB) void k_adj_walk(
     const WNode *__restrict__ nodes, const TapeRec *__restrict__ tape,
     const int *__restrict__ tcount, const int *__restrict__ toff, int cap,
@@ -76,7 +96,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_adj_walk(
     double G, double eps2, double *__restrict__ xbar_s, uint64_t *__restrict__ rkey,
     int *__restrict__ rval, double *__restrict__ rsum) {
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * CSG_TPB + threadIdx.x) >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * CSG_TPB + threadIdx.x) >> 6));
     const int nl = bp->n_live;
     const int b0 = wave * 64;
     if (b0 >= nl) return;
@@ -88,19 +108,31 @@ __global__ __launch_bounds__(CSG_TPB) void k_adj_walk(
     double ax = 0.0, ay = 0.0, az = 0.0;
     const int cnt = tcount[wave], off = toff[wave];
     const TapeRec *t = tape + (size_t)wave * cap;
-    for (int r = 0; r < cnt; ++r) {
-        const TapeRec rec = t[r];
-        const WNode nd = nodes[rec.node];
-        double gx = 0.0, gy = 0.0, gz = 0.0;
-        if ((rec.imask >> lane) & 1ull)
-            tidal_apply(nd.cx - x, nd.cy - y, nd.cz - z, wx, wy, wz, G * nd.m, eps2, gx, gy, gz);
-        ax -= gx; ay -= gy; az -= gz;
-        gx = wave_sum(gx); gy = wave_sum(gy); gz = wave_sum(gz);
-        if (lane == 0) {
-            const size_t q = (size_t)off + r;
-            rkey[q] = (uint64_t)rec.node;
-            rval[q] = (int)q;
-            rsum[3 * q] = gx; rsum[3 * q + 1] = gy; rsum[3 * q + 2] = gz;
+    for (int r0 = 0; r0 < cnt; r0 += 4) {
+        TapeRec rr[4];
+        double cm[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) rr[k] = t[r0 + k < cnt ? r0 + k : cnt - 1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const WNode *np = nodes + rr[k].node;
+            cm[k][0] = np->cx; cm[k][1] = np->cy; cm[k][2] = np->cz; cm[k][3] = np->m;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (r0 + k >= cnt) break;
+            double gx = 0.0, gy = 0.0, gz = 0.0;
+            if ((rr[k].imask >> lane) & 1ull)
+                tidal_apply(cm[k][0] - x, cm[k][1] - y, cm[k][2] - z, wx, wy, wz, G * cm[k][3],
+                            eps2, gx, gy, gz);
+            ax -= gx; ay -= gy; az -= gz;
+            gx = wave_sum63(gx); gy = wave_sum63(gy); gz = wave_sum63(gz);
+            if (lane == 63) {
+                const size_t q = (size_t)off + r0 + k;
+                rkey[q] = (uint64_t)rr[k].node;
+                rval[q] = (int)q;
+                rsum[3 * q] = gx; rsum[3 * q + 1] = gy; rsum[3 * q + 2] = gz;
+            }
         }
     }
     if (act) {
@@ -130,27 +162,92 @@ static void fill_i32(int *p, size_t n, int v, hipStream_t st) {
     k_fill_i32<<<(unsigned)(b < 4096 ? b : 4096), CSG_TPB, 0, st>>>(p, n, v);
 }
 
-// Step 2: sum each node's records in sorted (node, wave, visit) order.
-__global__ __launch_bounds__(CSG_TPB) void k_adj_node_reduce(const uint64_t *__restrict__ key,
-                                                             const int *__restrict__ val,
-                                                             const double *__restrict__ rsum,
-                                                             const int *__restrict__ total,
-                                                             double *__restrict__ cbar) {
+// Step 2: sum each node's records, sorted by (node, wave, visit), without atomics
+// on the sums.  k_seg_bounds marks each node's record range; k_seg_short sums
+// ranges of at most CSG_SEG_SHORT records in one thread (in order) and queues the
+// longer ones; k_seg_long gives each queued range a block, whose threads take
+// records t, t + 256, ... and combine in a fixed tree.  Either way the summation
+// order depends only on the records, so the result is reproducible.
+#define CSG_SEG_SHORT 32
+__global__ __launch_bounds__(CSG_TPB) void k_seg_bounds(const uint64_t *__restrict__ key,
+                                                        const int *__restrict__ total,
+                                                        int *__restrict__ first,
+                                                        int *__restrict__ end) {
     const int R = *total;
     const int r = blockIdx.x * CSG_TPB + threadIdx.x;
     if (r >= R) return;
     const uint64_t k = key[r];
-    if (r > 0 && key[r - 1] == k) return;
+    if (r == 0 || key[r - 1] != k) first[k] = r;
+    if (r == R - 1 || key[r + 1] != k) end[k] = r + 1;
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_seg_short(const int *__restrict__ first,
+                                                       const int *__restrict__ end,
+                                                       const int *__restrict__ val,
+                                                       const double *__restrict__ rsum,
+                                                       const BuildParams *__restrict__ bp,
+                                                       double *__restrict__ cbar,
+                                                       int *__restrict__ longq,
+                                                       int *__restrict__ nlong) {
+    const int J = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (J >= bp->n_nodes) return;
+    const int f = first[J];
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int q = r; q < R && key[q] == k; ++q) {
-        const size_t v = (size_t)val[q];
-        sx += rsum[3 * v];
-        sy += rsum[3 * v + 1];
-        sz += rsum[3 * v + 2];
+    if (f >= 0) {
+        const int e = end[J];
+        if (e - f > CSG_SEG_SHORT) {
+            longq[atomicAdd(nlong, 1)] = J;   // queue order does not affect the sums
+            return;
+        }
+        for (int q = f; q < e; ++q) {
+            const size_t v = (size_t)val[q];
+            sx += rsum[3 * v];
+            sy += rsum[3 * v + 1];
+            sz += rsum[3 * v + 2];
+        }
     }
-    cbar[3 * k] = sx;
-    cbar[3 * k + 1] = sy;
-    cbar[3 * k + 2] = sz;
+    cbar[3 * (size_t)J] = sx;
+    cbar[3 * (size_t)J + 1] = sy;
+    cbar[3 * (size_t)J + 2] = sz;
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_seg_long(const int *__restrict__ first,
+                                                      const int *__restrict__ end,
+                                                      const int *__restrict__ val,
+                                                      const double *__restrict__ rsum,
+                                                      const int *__restrict__ longq,
+                                                      const int *__restrict__ nlong,
+                                                      double *__restrict__ cbar) {
+    __shared__ double red[3][CSG_TPB];
+    const int nq = *nlong;
+    const int t = threadIdx.x;
+    for (int e0 = blockIdx.x; e0 < nq; e0 += gridDim.x) {
+        const int J = longq[e0];
+        const int f = first[J], e = end[J];
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int q = f + t; q < e; q += CSG_TPB) {
+            const size_t v = (size_t)val[q];
+            sx += rsum[3 * v];
+            sy += rsum[3 * v + 1];
+            sz += rsum[3 * v + 2];
+        }
+        red[0][t] = sx; red[1][t] = sy; red[2][t] = sz;
+        __syncthreads();
+        for (int h = CSG_TPB / 2; h > 0; h >>= 1) {
+            if (t < h) {
+                red[0][t] += red[0][t + h];
+                red[1][t] += red[1][t + h];
+                red[2][t] += red[2][t + h];
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            cbar[3 * (size_t)J] = red[0][0];
+            cbar[3 * (size_t)J + 1] = red[1][0];
+            cbar[3 * (size_t)J + 2] = red[2][0];
+        }
+        __syncthreads();
+    }
 }
 
 // Step 3a: difference array over sorted bodies.  D[b] = sum of cbar/M over the
@@ -429,6 +526,7 @@ static void adj_free(csg_ctx *c) {
     dfree(c->rkey); dfree(c->rkey2); dfree(c->rval); dfree(c->rval2); dfree(c->rsum);
     dfree(c->rhist); dfree(c->rtot); dfree(c->ws); dfree(c->xbar_s); dfree(c->Dv); dfree(c->Uv);
     dfree(c->cbar); dfree(c->vpart); dfree(c->nodes_alt);
+    dfree(c->seg_first); dfree(c->seg_end); dfree(c->seg_long); dfree(c->seg_nlong);
     dfree(c->xb); dfree(c->vb); dfree(c->ab); dfree(c->gtmp); dfree(c->ckpt); dfree(c->seg);
     dfree(c->u64tmp);
     c->tape_cap = 0;
@@ -454,7 +552,9 @@ static int adj_reserve(csg_ctx *c, int cap_per_wave) {
              dalloc(c->vpart, 4 * (size_t)at_least1(blocks_for((long long)C, CSG_TILE))) &&
              dalloc(c->nodes_alt, (size_t)c->node_cap) && dalloc(c->xb, 3 * C) &&
              dalloc(c->vb, 3 * C) && dalloc(c->ab, 3 * C) && dalloc(c->gtmp, 3 * C) &&
-             dalloc(c->u64tmp, 1);
+             dalloc(c->u64tmp, 1) && dalloc(c->seg_first, (size_t)c->node_cap) &&
+             dalloc(c->seg_end, (size_t)c->node_cap) && dalloc(c->seg_long, (size_t)c->node_cap) &&
+             dalloc(c->seg_nlong, 1);
     if (!ok) {
         (void)hipGetLastError();
         adj_free(c);
@@ -556,10 +656,16 @@ static int force_vjp_dev(csg_ctx *c, const double *w_dev, double *xbar_dev, int 
             int *tv = vin; vin = vout; vout = tv;
         }
     }
-    k_fill_f64<<<256, CSG_TPB, 0, st>>>(c->cbar, 3 * (size_t)nn, 0.0);
+    fill_i32(c->seg_first, (size_t)nn, -1, st);
+    k_set_i32<<<1, 1, 0, st>>>(c->seg_nlong, 0);
     if (R > 0)
-        k_adj_node_reduce<<<blocks_for(R, CSG_TPB), CSG_TPB, 0, st>>>(kin, vin, c->rsum,
-                                                                      c->tape_total, c->cbar);
+        k_seg_bounds<<<blocks_for(R, CSG_TPB), CSG_TPB, 0, st>>>(kin, c->tape_total, c->seg_first,
+                                                                 c->seg_end);
+    k_seg_short<<<blocks_for(nn, CSG_TPB), CSG_TPB, 0, st>>>(c->seg_first, c->seg_end, vin, c->rsum,
+                                                             c->bp, c->cbar, c->seg_long,
+                                                             c->seg_nlong);
+    k_seg_long<<<1024, CSG_TPB, 0, st>>>(c->seg_first, c->seg_end, vin, c->rsum, c->seg_long,
+                                         c->seg_nlong, c->cbar);
     k_adj_diff<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(c->key, c->ext, c->lcp, c->nbase,
                                                             c->nodes, c->cbar, c->bp, c->Dv);
     const int nt = blocks_for(nl, CSG_TILE);

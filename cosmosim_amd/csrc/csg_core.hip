@@ -67,6 +67,7 @@ struct csg_ctx {
     double *cbar = nullptr, *xb = nullptr, *vb = nullptr, *ab = nullptr, *gtmp = nullptr;
     DD *vpart = nullptr;
     WNode *nodes_alt = nullptr;
+    int *seg_first = nullptr, *seg_end = nullptr, *seg_long = nullptr, *seg_nlong = nullptr;
     unsigned long long *u64tmp = nullptr;
     double *ckpt = nullptr, *seg = nullptr;
     int ckpt_slots = 0, seg_slots = 0;

@@ -135,6 +135,16 @@ __device__ __forceinline__ void store_nodes(WNode *__restrict__ nodes, WNode32 *
 
 __device__ __forceinline__ bool live_mass(double m) { return !(m <= 0.0); }
 
+// 1/sqrt(r2) in FP64: hardware estimate refined by two Newton steps on the
+// residual e = 1 - r2 y^2 (each step squares the relative error).
+__device__ __forceinline__ double rsq64(double r2) {
+    double y = __builtin_amdgcn_rsq(r2);
+    double e = fma(-r2 * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-r2 * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
 // Octant code: bit0 x >= c, bit1 y < c, bit2 z < c.
 __device__ __forceinline__ int cell_octant(double px, double py, double pz, double cx, double cy,
                                            double cz) {
@@ -989,11 +999,7 @@ __device__ __forceinline__ bool walk_visit(WalkLane &L, const WNode &nd, int i, 
         L.inter = inter;
         if (inter) {
             const double r2 = d2 + eps2;
-            double yv = __builtin_amdgcn_rsq(r2);
-            double e = fma(-r2 * yv, yv, 1.0);
-            yv = fma(0.5 * yv, e, yv);
-            e = fma(-r2 * yv, yv, 1.0);
-            yv = fma(0.5 * yv, e, yv);
+            const double yv = rsq64(r2);
             const double f = nd.m * (yv * yv * yv);
             L.sx = fma(dx, f, L.sx);
             L.sy = fma(dy, f, L.sy);
@@ -1181,11 +1187,21 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
 // which changes only the timing.  One block of 1024 threads.
 __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cost, int nw,
                                                      int *__restrict__ order) {
+    constexpr int PER = 16;   // loads in flight per thread and pass
     __shared__ int hist[4096];
     __shared__ int part[16];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     int mx = 0;
-    for (int i = t; i < nw; i += 1024) mx = max(mx, cost[i]);
+    for (int base = 0; base < nw; base += 1024 * PER) {
+        int v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = base + k * 1024 + t;
+            v[k] = i < nw ? cost[i] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) mx = max(mx, v[k]);
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) part[wid] = mx;
@@ -1196,13 +1212,25 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
     for (int k = 0; k < 16; ++k) mx = max(mx, part[k]);
     int shift = 0;
     while ((mx >> shift) >= 4096) ++shift;
-    for (int i = t; i < nw; i += 1024)
-        atomicAdd(&hist[4095 - (max(cost[i], 0) >> shift)], 1);
+    // any count (stale ones included) lands in a bin: the order is always a
+    // permutation of [0, nw)
+    auto bin = [&](int c) { return 4095 - (max(c, 0) >> shift); };
+    for (int base = 0; base < nw; base += 1024 * PER) {
+        int v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = base + k * 1024 + t;
+            v[k] = i < nw ? cost[i] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (base + k * 1024 + t < nw) atomicAdd(&hist[bin(v[k])], 1);
+    }
     __syncthreads();
     // exclusive scan of the 4096 bins: 4 per thread, then across threads
-    int v[4], sum = 0;
+    int h[4], sum = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) { v[k] = hist[4 * t + k]; sum += v[k]; }
+    for (int k = 0; k < 4; ++k) { h[k] = hist[4 * t + k]; sum += h[k]; }
     int inc = sum;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -1212,15 +1240,23 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
     __syncthreads();
     if (lane == 63) part[wid] = inc;
     __syncthreads();
-    int base = 0;
-    for (int k = 0; k < wid; ++k) base += part[k];
-    int run = base + inc - sum;
+    int base0 = 0;
+    for (int k = 0; k < wid; ++k) base0 += part[k];
+    int run = base0 + inc - sum;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) { hist[4 * t + k] = run; run += v[k]; }
+    for (int k = 0; k < 4; ++k) { hist[4 * t + k] = run; run += h[k]; }
     __syncthreads();
-    for (int i = t; i < nw; i += 1024) {
-        const int pos = atomicAdd(&hist[4095 - (max(cost[i], 0) >> shift)], 1);
-        order[pos] = i;
+    for (int base = 0; base < nw; base += 1024 * PER) {
+        int v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = base + k * 1024 + t;
+            v[k] = i < nw ? cost[i] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (base + k * 1024 + t < nw)
+                order[atomicAdd(&hist[bin(v[k])], 1)] = base + k * 1024 + t;
     }
 }
 
@@ -1274,6 +1310,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         const int slot = (int)blockIdx.x * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
         if (slot >= wo.n) return;
         wv = wo.order[slot];
+        if (wv < 0 || wv >= wo.n) return;
     } else {
         wv = xcd_tile(nb8, chunk) * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
     }
