@@ -565,16 +565,19 @@ static int launch_build(csg_ctx *c, bool with64) {
 }
 
 extern "C++" {
+// FP64 walks: dispatched through `wo` (grid = blocks for the ordered slots) or
+// the XCD-chunked mapping (grid = nb8); the grouped FP32 forms ignore the order.
 template <int GS>
-static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, double th2, int nb8,
-                           TapeOut tape) {
+static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, double th2, int grid,
+                           int nb8, TapeOut tape, WaveOrder wo) {
     hipStream_t st = c->stream;
     if (precision == COSMOSIM_PREC_FP64_STRICT)
-        k_walk<WALK_FP64_STRICT, GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y,
-            c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
+        k_walk<WALK_FP64_STRICT, GS><<<grid, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y,
+            c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape,
+            wo);
     else if (precision == COSMOSIM_PREC_FP64)
-        k_walk<WALK_FP64, GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
-            c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
+        k_walk<WALK_FP64, GS><<<grid, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
+            c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape, wo);
     else
         k_walk32<GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes32, c->bp, c->idx, c->x, c->y, c->z, c->ax, c->ay,
                                           c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
@@ -605,19 +608,23 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     nb8 = (nb8 + gran - 1) / gran * gran;
     if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
     c->counts_valid = 0;
-    if (precision == COSMOSIM_PREC_FP32 && (c->walk_group32 == 0 || tape.rec)) {
+    const bool wave_mask32 = precision == COSMOSIM_PREC_FP32 && (c->walk_group32 == 0 || tape.rec);
+    const bool grouped32 = precision == COSMOSIM_PREC_FP32 && !wave_mask32;
+    // longest-first dispatch from the previous walk's per-wave step counts (the
+    // wave-mask FP32 walk and the FP64 walks record them)
+    const int nw = blocks_for(n, 64);
+    WaveOrder wo{nullptr, nw, grouped32 ? nullptr : c->wcost};
+    int grid = nb8;
+    if (!grouped32 && c->wave_order && c->wcost_n == nw) {
+        k_wave_order<<<1, 1024, 0, st>>>(c->wcost, nw, c->worder);
+        wo.order = c->worder;
+        grid = blocks_for(nw, CSG_TPB / 64);
+    }
+    if (!grouped32) c->wcost_n = nw;
+    if (wave_mask32) {
         // wave-mask traversal (also the taped FP32 walk: the tape is per wavefront)
         const bool off32 = (size_t)c->node_cap * sizeof(WNode32) < ((size_t)1 << 32);
         const bool taping = tape.rec != nullptr;
-        // longest-first dispatch from the previous walk's per-wave step counts
-        const int nw = blocks_for(n, 64);
-        WaveOrder wo{nullptr, nw, c->wcost};
-        int grid = nb8;
-        if (c->wave_order && c->wcost_n == nw) {
-            k_wave_order<<<1, 1024, 0, st>>>(c->wcost, nw, c->worder);
-            wo.order = c->worder;
-            grid = blocks_for(nw, CSG_TPB / 64);
-        }
         const int v = (lane_counts ? 4 : 0) | (taping ? 2 : 0) | (off32 ? 1 : 0);
         switch (v) {
         case 0: launch_walk32w<false, false, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
@@ -629,7 +636,6 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         case 6: launch_walk32w<true, true, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
         default: launch_walk32w<true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
         }
-        c->wcost_n = nw;
         CSG_LAUNCHED();
         c->counts_valid = lane_counts ? 1 : 0;
         return COSMOSIM_OK;
@@ -638,10 +644,10 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     const int gs = tape.rec ? 64 : (precision == COSMOSIM_PREC_FP32 ? c->walk_group32 : c->walk_group);
     c->counts_valid = 1;
     switch (gs) {
-    case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, nb8, tape); break;
-    case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, nb8, tape); break;
-    case 32: launch_walk_gs<32>(c, precision, G, eps2, th2, nb8, tape); break;
-    default: launch_walk_gs<64>(c, precision, G, eps2, th2, nb8, tape); break;
+    case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, grid, nb8, tape, wo); break;
+    case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, grid, nb8, tape, wo); break;
+    case 32: launch_walk_gs<32>(c, precision, G, eps2, th2, grid, nb8, tape, wo); break;
+    default: launch_walk_gs<64>(c, precision, G, eps2, th2, grid, nb8, tape, wo); break;
     }
     CSG_LAUNCHED();
     return COSMOSIM_OK;

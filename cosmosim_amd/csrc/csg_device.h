@@ -953,6 +953,19 @@ __device__ __forceinline__ int xcd_tile(int grid, int chunk) {
     return (q / chunk) * (8 * chunk) + x * chunk + (q % chunk);
 }
 
+// Wave index of the calling wavefront in the walks: through the dispatch order
+// when one is given (slot s of the grid -> order[s]), else the XCD-chunked tile
+// mapping.  -1: a slot past the order's end (no wave).
+__device__ __forceinline__ int walk_wave(int nb8, int chunk, const WaveOrder &wo) {
+    if (wo.order) {
+        const int slot = (int)blockIdx.x * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
+        if (slot >= wo.n) return -1;
+        const int w = wo.order[slot];
+        return (w < 0 || w >= wo.n) ? -1 : w;
+    }
+    return xcd_tile(nb8, chunk) * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
+}
+
 // Per-lane state of the FP64 walks.  A lane holds one body; p is the next
 // pre-order node that this body must look at (its own DFS position).
 struct WalkLane {
@@ -1025,18 +1038,22 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk(
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
     const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
     double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
-    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape) {
+    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape, WaveOrder wo) {
     const int n_live = bp->n_live;
     const int nnodes = bp->n_nodes;
     if (bp->overflow) return;
     const int lane = threadIdx.x & 63;
-    const int tile = xcd_tile(nb8, chunk);
-    const int b0 = (tile * CSG_TPB + (int)(threadIdx.x & ~63));
-    if (b0 >= n_live) return;
+    const int wv = walk_wave(nb8, chunk, wo);   // wave w walks sorted bodies [64 w, 64 w + 64)
+    if (wv < 0) return;
+    const int b0 = wv * 64;
+    if (b0 >= n_live) {
+        if (wo.cost && lane == 0 && wv < wo.n) wo.cost[wv] = 0;
+        return;
+    }
     const int b = b0 + lane;
     const bool act = b < n_live;
     const bool taping = (GS == 64) && tape.rec != nullptr;
-    int nrec = 0;
+    int nrec = 0, nit = 0;
     WalkLane L;
     L.me = act ? sidx[b] : -1;
     L.x = act ? px[L.me] : 0.0;
@@ -1067,7 +1084,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk(
         const bool gdesc = (__ballot(desc) & gmask) != 0ull;
         const int nxt = gdesc ? ii + 1 : skip;
         i = nxt > ii ? nxt : ii + 1;  // strictly increasing: the loop always ends
+        ++nit;
     }
+    if (wo.cost && lane == 0) wo.cost[wv] = nit;
     if (act) {
         if constexpr (PREC == WALK_FP64_STRICT) {
             ax[L.me] = L.sx; ay[L.me] = L.sy; az[L.me] = L.sz;
@@ -1305,15 +1324,8 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
     const int nnodes = bp->n_nodes;
     if (bp->overflow) return;
     const int lane = threadIdx.x & 63;
-    int wv;
-    if (wo.order) {
-        const int slot = (int)blockIdx.x * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
-        if (slot >= wo.n) return;
-        wv = wo.order[slot];
-        if (wv < 0 || wv >= wo.n) return;
-    } else {
-        wv = xcd_tile(nb8, chunk) * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
-    }
+    const int wv = walk_wave(nb8, chunk, wo);
+    if (wv < 0) return;
     const int b0 = wv * 64;
     if (b0 >= n_live) {
         if (wo.cost && lane == 0 && wv < wo.n) wo.cost[wv] = 0;
