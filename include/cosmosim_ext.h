@@ -101,7 +101,10 @@ int cosmosim_ext_set_state(SimHandle h, const double *pos, const double *vel,
 int cosmosim_ext_direct_forces(SimHandle h, const int *idx, int k, double *acc);
 /* Softened total energy and momentum of the device state (O(N^2) FP64). */
 int cosmosim_ext_energy(SimHandle h, double *kinetic, double *potential, double momentum[3]);
-/* Per-body interaction counts of the last walk (original body order). */
+/* Per-body interaction counts of the last explicit force evaluation (creation,
+ * cosmosim_ext_compute_forces, the legacy walks), in original body order.  The
+ * substep loops keep only the total (CosmosimExtStats.interactions); after one,
+ * this returns COSMOSIM_E_STATE instead of stale counts. */
 int cosmosim_ext_interaction_counts(SimHandle h, int *counts, int n);
 int cosmosim_ext_get_stats(SimHandle h, CosmosimExtStats *out);
 /* Enable and reset per-phase GPU event timing (events on the core's stream,
