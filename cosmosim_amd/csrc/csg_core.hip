@@ -2,6 +2,7 @@
 // HIP gravity core.  Every entry point makes the context's device current on the
 // calling thread, launches on the context's stream and returns 0 or a negative
 // COSMOSIM_E_* code.  Kernels live in csg_device.h.
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -75,6 +76,8 @@ struct csg_ctx {
     struct FrozenStep *frozen = nullptr;
     int n_frozen = 0;
     double rec_dt = 0, rec_G = 1, rec_eps = 0, rec_theta = 0;
+    double theta_hint = 0.5;  // opening angle of the next walks (set by every walk entry)
+    double th32 = -1.0;       // opening angle folded into the FP32 records (-1: none)
     int walk_group = 8;     // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
     int walk_group32 = 0;   // FP32 walk: 0 = wave-mask traversal, else lanes per group
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
@@ -556,10 +559,12 @@ static int launch_build(csg_ctx *c, bool with64) {
                                       c->cap + 1);
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
         c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->xs, c->ys, c->zs, c->ms,
-        c->bp, with64 ? c->nodes : nullptr, c->nodes32);
+        c->bp, with64 ? c->nodes : nullptr, c->nodes32,
+        c->theta_hint > 0.0 ? 1.0 / (c->theta_hint * c->theta_hint) : HUGE_VAL);
     CSG_LAUNCHED();
     c->tree_valid = 1;
     c->nodes64_valid = with64 ? 1 : 0;
+    c->th32 = c->theta_hint;
     c->tape_valid = 0;   // a tape describes the tree it was recorded on
     return COSMOSIM_OK;
 }
@@ -583,23 +588,30 @@ static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, dou
                                           c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
 }
 
-template <bool LC, bool TP, bool O32>
+template <bool LC, bool TP, bool O32, bool TOT>
 static void launch_walk32w(csg_ctx *c, double G, double eps2, double th2, int grid, int nb8,
                            TapeOut tape, WaveOrder wo) {
-    k_walk32w<LC, TP, O32><<<grid, CSG_TPB, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x,
+    k_walk32w<LC, TP, O32, TOT><<<grid, CSG_TPB, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x,
         c->y, c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk,
         tape, wo);
 }
 }  // extern "C++"
 
 // lane_counts: record per-body interaction counts (c->icount); walks inside the
-// substep loops skip them (the total is always kept).
+// substep loops skip them.  count_total: keep the interaction total (the FP32
+// substep loop keeps it only in its last walk; the statistics report that one).
 static int launch_walk(csg_ctx *c, int precision, double G, double softening, double theta,
                        TapeOut tape = TapeOut{nullptr, nullptr, 0, nullptr},
-                       bool lane_counts = true) {
+                       bool lane_counts = true, bool count_total = true) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const double eps2 = softening * softening, th2 = theta * theta;
+    c->theta_hint = theta;
+    if (precision == COSMOSIM_PREC_FP32 && c->tree_valid && c->th32 != theta) {
+        // the FP32 records carry the opening angle: emit them again for this one
+        const int s = launch_build(c, c->nodes64_valid != 0);
+        if (s) return s;
+    }
     k_set_u64<<<1, 1, 0, st>>>(c->itotal, 0ull);
     if (n == 0) return COSMOSIM_OK;
     k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, c->icount);
@@ -626,15 +638,20 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         const bool off32 = (size_t)c->node_cap * sizeof(WNode32) < ((size_t)1 << 32);
         const bool taping = tape.rec != nullptr;
         const int v = (lane_counts ? 4 : 0) | (taping ? 2 : 0) | (off32 ? 1 : 0);
-        switch (v) {
-        case 0: launch_walk32w<false, false, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-        case 1: launch_walk32w<false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-        case 2: launch_walk32w<false, true, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-        case 3: launch_walk32w<false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-        case 4: launch_walk32w<true, false, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-        case 5: launch_walk32w<true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-        case 6: launch_walk32w<true, true, false>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-        default: launch_walk32w<true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+        if (v < 2 && !count_total) {
+            if (off32) launch_walk32w<false, false, true, false>(c, G, eps2, th2, grid, nb8, tape, wo);
+            else launch_walk32w<false, false, false, false>(c, G, eps2, th2, grid, nb8, tape, wo);
+        } else {
+            switch (v) {
+            case 0: launch_walk32w<false, false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 1: launch_walk32w<false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 2: launch_walk32w<false, true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 3: launch_walk32w<false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 4: launch_walk32w<true, false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 5: launch_walk32w<true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 6: launch_walk32w<true, true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 7: launch_walk32w<true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            }
         }
         CSG_LAUNCHED();
         c->counts_valid = lane_counts ? 1 : 0;
@@ -711,6 +728,7 @@ int csg_walk(csg_ctx *c, int precision, double G, double softening, double theta
 int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, double theta,
                  int precision) {
     CSG_USE(c);
+    c->theta_hint = theta;
     if (precision < 0 || precision > 2 || k < 0) return COSMOSIM_E_ARG;
     if (k == 0) return COSMOSIM_OK;
     int s;
@@ -722,7 +740,8 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
         if ((s = launch_build(c, precision != COSMOSIM_PREC_FP32))) return s;
         timer_stop(c, PH_BUILD);
         timer_start(c);
-        if ((s = launch_walk(c, precision, G, softening, theta, TapeOut{}, false))) return s;
+        if ((s = launch_walk(c, precision, G, softening, theta, TapeOut{}, false, sub + 1 == k)))
+            return s;
         timer_stop(c, PH_WALK);
         timer_start(c);
         // end-of-substep half kick fused with the next substep's kick + drift

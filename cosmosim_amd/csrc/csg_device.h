@@ -61,11 +61,13 @@ static_assert(sizeof(WNode) == 64, "WNode is one 64-byte record");
 // The same node for the FP32 walk (40 bytes).  The centre of mass stays FP64:
 // a lane forms its body-node offset as fp32(C - x) with the subtraction in FP64
 // (full rate on CDNA4), so the offset carries a single FP32 rounding however far
-// the pair sits from the origin.  Mass and (2h)^2 are FP32.
+// the pair sits from the origin.  Mass is FP32; s2 is (2h)^2 / theta^2 for the
+// opening angle the records were emitted for (divided in FP64, +inf at theta 0),
+// so the walk's opening test is s2 < d^2.
 struct __align__(8) WNode32 {
     double cx, cy, cz;  // centre of mass
     float m;            // total mass
-    float s2;           // (2h)^2; -1 for a leaf
+    float s2;           // (2h)^2 / theta^2; -1 for a leaf
     int skip;
     int body;
 };
@@ -117,15 +119,14 @@ __device__ __forceinline__ void tape_close(const TapeOut &t, int wave, int lane,
 }
 
 __device__ __forceinline__ void store_nodes(WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
-                                            int id, const WNode &w, const double *root) {
-    (void)root;
+                                            int id, const WNode &w, double inv_th2) {
     if (nodes) nodes[id] = w;   // FP64 records: skipped when only the FP32 walk reads the tree
     WNode32 f;
     f.cx = w.cx;
     f.cy = w.cy;
     f.cz = w.cz;
     f.m = (float)w.m;
-    f.s2 = w.body >= 0 ? -1.0f : (float)w.s2;  // a leaf passes every opening test
+    f.s2 = w.body >= 0 ? -1.0f : (float)(w.s2 * inv_th2);  // a leaf passes every test
     f.skip = w.skip;
     f.body = w.body;
     n32[id] = f;
@@ -861,12 +862,12 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const int *__restrict__ idx, const signed char *__restrict__ L, const int *__restrict__ nbase,
     const DD *__restrict__ pref, int cap1, const double *__restrict__ xs,
     const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ms,
-    const BuildParams *__restrict__ bp, WNode *__restrict__ nodes, WNode32 *__restrict__ n32) {
+    const BuildParams *__restrict__ bp, WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
+    double inv_th2) {
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
     if (bp->overflow) return;
     const double H = bp->root[3];
-    const double root[3] = {bp->root[0], bp->root[1], bp->root[2]};
     if (nl == 0) {
         if (b == 0) {
             WNode r;
@@ -875,7 +876,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
             const double s = 2.0 * H;
             r.s2 = s * s;
             r.skip = 1; r.body = -1; r.depth = 0; r.first = 0; r.last = -1; r.pad = 0;
-            store_nodes(nodes, n32, 0, r, root);
+            store_nodes(nodes, n32, 0, r, inv_th2);
         }
         return;
     }
@@ -896,7 +897,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
         w.s2 = s * s;
         w.skip = nbase[e + 1];
         w.body = -1; w.depth = d; w.first = b; w.last = e; w.pad = 0;
-        store_nodes(nodes, n32, id, w, root);
+        store_nodes(nodes, n32, id, w, inv_th2);
     }
     if (lp < 64) {
         WNode w;
@@ -920,7 +921,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
         w.s2 = s * s;
         w.skip = id + 1;
         w.body = i; w.depth = depth; w.first = b; w.last = e; w.pad = 0;
-        store_nodes(nodes, n32, id, w, root);
+        store_nodes(nodes, n32, id, w, inv_th2);
     }
 }
 
@@ -1110,7 +1111,7 @@ __device__ __forceinline__ void kahan_add(float &s, float &k, float t) {
     s = u;
 }
 
-#define CSG_FOLD_EVERY 16
+#define CSG_FOLD_EVERY 32
 
 // FP32 walk over WNode32 records, same traversal as k_walk (groups of GS lanes).
 // Offsets: d = fp32(C - x) with the subtraction in FP64 (one FP32 rounding of
@@ -1139,7 +1140,8 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     int nrec = 0;
     const int me = act ? sidx[b] : -1;
     const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
-    const float eps2f = (float)eps2, th2f = (float)theta2;
+    const float eps2f = (float)eps2;  // theta is folded into the records' s2
+    (void)theta2;
     const unsigned long long gmask =
         (GS == 64) ? ~0ull : (((1ull << (GS & 63)) - 1ull) << (lane & ~(GS - 1)));
     int p = act ? 0 : 0x7fffffff;
@@ -1164,7 +1166,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
         const float dz = (float)(nd.cz - z);
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         // leaves carry s2 = -1, so crit holds for them and only self is skipped
-        const bool crit = nd.s2 < th2f * d2;
+        const bool crit = nd.s2 < d2;
         const bool inter = here && crit && nd.body != me;
         const bool desc = here && !crit;
         const float yv = __builtin_amdgcn_rsqf(d2 + eps2f);
@@ -1307,13 +1309,15 @@ __device__ __forceinline__ int lane_sel_i32(unsigned long long m, int t, int f) 
 //                kept, on the scalar unit);
 //   TAPING:      record the walk tape;
 //   OFF32:       the node array spans < 4 GiB, so records are addressed by a
-//                32-bit byte offset (scalar loads with an SGPR offset).
+//                32-bit byte offset (scalar loads with an SGPR offset);
+//   TOTAL:       add the wave's interaction count to *itotal (the substep loop
+//                counts only in its last walk, the one the statistics report).
 // Wave w walks sorted bodies [64 w, 64 w + 64).  Without an order, w follows the
 // XCD-chunked tile mapping; with one (WaveOrder), wave slot s of the grid takes
 // w = order[s] (longest first, see k_wave_order).  Either way the results do not
 // depend on the dispatch order.  With WaveOrder::cost set, every wave stores its
 // node-step count there.
-template <bool LANE_COUNTS, bool TAPING, bool OFF32>
+template <bool LANE_COUNTS, bool TAPING, bool OFF32, bool TOTAL>
 __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
     const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
@@ -1336,11 +1340,13 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
     int nrec = 0;
     const int me = act ? sidx[b] : -1;
     const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
-    const float eps2f = (float)eps2, th2f = (float)theta2;
+    const float eps2f = (float)eps2;  // theta is folded into the records' s2
+    (void)theta2;
     const char *const base = (const char *)nodes;
     int p = act ? 0 : 0x7fffffff;
     int cnt = 0;
     unsigned long long wcnt = 0;
+    unsigned wc32 = 0;
     float bx = 0.f, by = 0.f, bz = 0.f;               // block sums
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     int ii = 0, steps = 0, folds = 0;
@@ -1355,7 +1361,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         // compare predicates: 41 = signed <=, 4 = ordered <, 33 = !=
         const unsigned long long H = __builtin_amdgcn_sicmp(p, ii, 41);
         // leaves carry s2 = -1, so the criterion holds for them
-        const unsigned long long C = __builtin_amdgcn_fcmpf(nd.s2, th2f * d2, 4);
+        const unsigned long long C = __builtin_amdgcn_fcmpf(nd.s2, d2, 4);
         const unsigned long long S = __builtin_amdgcn_sicmp(nd.body, me, 33);
         const unsigned long long A = H & C;          // accepted (own leaf included)
         const unsigned long long I = A & S;          // interacting
@@ -1365,19 +1371,25 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         bx = fmaf(dx, f, bx);
         by = fmaf(dy, f, by);
         bz = fmaf(dz, f, bz);
-        wcnt += (unsigned long long)__builtin_popcountll(I);
+        if constexpr (TOTAL) wc32 += (unsigned)__builtin_popcountll(I);   // <= 64 per step
         if constexpr (LANE_COUNTS) cnt += (int)((I >> lane) & 1ull);
         if constexpr (TAPING) tape_push(tape, b0 >> 6, lane, nrec, I, ii);
         p = lane_sel_i32(A, nd.skip, p);
         if (++steps == CSG_FOLD_EVERY) {
             steps = 0;
             ++folds;
+            if constexpr (TOTAL) {
+                wcnt += wc32;
+                wc32 = 0;
+            }
             kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
             bx = by = bz = 0.f;
         }
-        const bool down = D || nd.skip <= ii;
-        ii = down ? ii + 1 : nd.skip;  // strictly increasing: the loop always ends
+        // strictly increasing even for a malformed skip: the loop always ends
+        const int jump = max(nd.skip, ii + 1);
+        ii = D ? ii + 1 : jump;
     }
+    wcnt += wc32;
     if (wo.cost && lane == 0) wo.cost[wv] = folds * CSG_FOLD_EVERY + steps;
     kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
     if (act) {
@@ -1387,7 +1399,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         if constexpr (LANE_COUNTS) icount[me] = cnt;
     }
     if constexpr (TAPING) tape_close(tape, b0 >> 6, lane, nrec);
-    if (lane == 0) atomicAdd(itotal, wcnt);
+    if constexpr (TOTAL) {
+        if (lane == 0) atomicAdd(itotal, wcnt);
+    }
 }
 
 // Dead bodies get a = 0 (src/octree.c:227-231).
