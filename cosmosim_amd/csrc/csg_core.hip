@@ -588,10 +588,10 @@ static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, dou
                                           c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape);
 }
 
-template <bool LC, bool TP, bool O32, bool TOT>
+template <bool LC, bool TP, bool O32, bool TOT, bool SELF>
 static void launch_walk32w(csg_ctx *c, double G, double eps2, double th2, int grid, int nb8,
                            TapeOut tape, WaveOrder wo) {
-    k_walk32w<LC, TP, O32, TOT><<<grid, CSG_TPB, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x,
+    k_walk32w<LC, TP, O32, TOT, SELF><<<grid, CSG_TPB, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x,
         c->y, c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk,
         tape, wo);
 }
@@ -639,18 +639,23 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         const bool taping = tape.rec != nullptr;
         const int v = (lane_counts ? 4 : 0) | (taping ? 2 : 0) | (off32 ? 1 : 0);
         if (v < 2 && !count_total) {
-            if (off32) launch_walk32w<false, false, true, false>(c, G, eps2, th2, grid, nb8, tape, wo);
-            else launch_walk32w<false, false, false, false>(c, G, eps2, th2, grid, nb8, tape, wo);
+            // softened: the own leaf adds exactly 0, so it needs no exclusion
+            if (off32 && eps2 > 0.0)
+                launch_walk32w<false, false, true, false, false>(c, G, eps2, th2, grid, nb8, tape, wo);
+            else if (off32)
+                launch_walk32w<false, false, true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo);
+            else
+                launch_walk32w<false, false, false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo);
         } else {
             switch (v) {
-            case 0: launch_walk32w<false, false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 1: launch_walk32w<false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 2: launch_walk32w<false, true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 3: launch_walk32w<false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 4: launch_walk32w<true, false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 5: launch_walk32w<true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 6: launch_walk32w<true, true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 7: launch_walk32w<true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 0: launch_walk32w<false, false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 1: launch_walk32w<false, false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 2: launch_walk32w<false, true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 3: launch_walk32w<false, true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 4: launch_walk32w<true, false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 5: launch_walk32w<true, false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 6: launch_walk32w<true, true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 7: launch_walk32w<true, true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
             }
         }
         CSG_LAUNCHED();

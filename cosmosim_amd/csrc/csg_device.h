@@ -1311,13 +1311,17 @@ __device__ __forceinline__ int lane_sel_i32(unsigned long long m, int t, int f) 
 //   OFF32:       the node array spans < 4 GiB, so records are addressed by a
 //                32-bit byte offset (scalar loads with an SGPR offset);
 //   TOTAL:       add the wave's interaction count to *itotal (the substep loop
-//                counts only in its last walk, the one the statistics report).
+//                counts only in its last walk, the one the statistics report);
+//   SELF:        exclude the body's own leaf by index.  Without it (only for
+//                softening > 0, no counts, no tape) the own leaf is taken as an
+//                interaction: its offset is fp32(x - x) = 0 exactly and the
+//                softened factor is finite, so it adds exactly 0.
 // Wave w walks sorted bodies [64 w, 64 w + 64).  Without an order, w follows the
 // XCD-chunked tile mapping; with one (WaveOrder), wave slot s of the grid takes
 // w = order[s] (longest first, see k_wave_order).  Either way the results do not
 // depend on the dispatch order.  With WaveOrder::cost set, every wave stores its
 // node-step count there.
-template <bool LANE_COUNTS, bool TAPING, bool OFF32, bool TOTAL>
+template <bool LANE_COUNTS, bool TAPING, bool OFF32, bool TOTAL, bool SELF>
 __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
     const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
@@ -1349,8 +1353,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
     unsigned wc32 = 0;
     float bx = 0.f, by = 0.f, bz = 0.f;               // block sums
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
-    int ii = 0, steps = 0, folds = 0;
-    while (ii < nnodes) {
+    int ii = 0, quads = 0, folds = 0;
+    // one node step of the wave (ii -> next node in the wave's depth-first order)
+    auto step = [&]() {
         const WNode32 nd = OFF32
             ? *(const WNode32 *)(base + (unsigned)ii * (unsigned)sizeof(WNode32))
             : nodes[ii];
@@ -1362,9 +1367,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         const unsigned long long H = __builtin_amdgcn_sicmp(p, ii, 41);
         // leaves carry s2 = -1, so the criterion holds for them
         const unsigned long long C = __builtin_amdgcn_fcmpf(nd.s2, d2, 4);
-        const unsigned long long S = __builtin_amdgcn_sicmp(nd.body, me, 33);
         const unsigned long long A = H & C;          // accepted (own leaf included)
-        const unsigned long long I = A & S;          // interacting
+        unsigned long long I = A;                    // interacting
+        if constexpr (SELF) I &= __builtin_amdgcn_sicmp(nd.body, me, 33);
         const unsigned long long D = H & ~C;         // descending
         const float yv = __builtin_amdgcn_rsqf(d2 + eps2f);
         const float f = lane_sel_f32(I, nd.m * (yv * yv * yv), 0.0f);
@@ -1375,8 +1380,23 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         if constexpr (LANE_COUNTS) cnt += (int)((I >> lane) & 1ull);
         if constexpr (TAPING) tape_push(tape, b0 >> 6, lane, nrec, I, ii);
         p = lane_sel_i32(A, nd.skip, p);
-        if (++steps == CSG_FOLD_EVERY) {
-            steps = 0;
+        // strictly increasing even for a malformed skip: the loop always ends
+        const int jump = max(nd.skip, ii + 1);
+        ii = D ? ii + 1 : jump;
+    };
+    // four steps per round (the exit test between them, the fold count once per
+    // round); the compensated fold still comes after every CSG_FOLD_EVERY steps
+    for (;;) {
+        if (ii >= nnodes) break;
+        step();
+        if (ii >= nnodes) break;
+        step();
+        if (ii >= nnodes) break;
+        step();
+        if (ii >= nnodes) break;
+        step();
+        if (++quads == CSG_FOLD_EVERY / 4) {
+            quads = 0;
             ++folds;
             if constexpr (TOTAL) {
                 wcnt += wc32;
@@ -1385,12 +1405,10 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
             kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
             bx = by = bz = 0.f;
         }
-        // strictly increasing even for a malformed skip: the loop always ends
-        const int jump = max(nd.skip, ii + 1);
-        ii = D ? ii + 1 : jump;
     }
     wcnt += wc32;
-    if (wo.cost && lane == 0) wo.cost[wv] = folds * CSG_FOLD_EVERY + steps;
+    // node steps of this wave, to within the last round (the dispatch order's key)
+    if (wo.cost && lane == 0) wo.cost[wv] = folds * CSG_FOLD_EVERY + quads * 4;
     kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
     if (act) {
         ax[me] = G * ((double)sx - (double)kx);
