@@ -83,6 +83,7 @@ struct csg_ctx {
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
     int nodes64_valid = 0;  // the FP64 node records belong to the current tree
     double4 *xyzm = nullptr;  // build scratch: (x, y, z, m) interleaved, body order
+    int *owner = nullptr;     // build scratch: body that opens each internal node id (-1: leaf)
     int *wcost = nullptr;   // FP32 wave-mask walk: node steps of each wave in the last walk
     int *worder = nullptr;  // dispatch order built from wcost (longest first)
     int wcost_n = 0;        // number of waves wcost describes (0: none)
@@ -135,7 +136,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->depth_part);
     dfree(c->runs); dfree(c->lcp); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
-    dfree(c->wcost); dfree(c->worder); dfree(c->xyzm);
+    dfree(c->wcost); dfree(c->worder); dfree(c->xyzm); dfree(c->owner);
     c->wcost_n = 0;
     c->cap = 0;
     c->node_cap = 0;
@@ -160,7 +161,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
               dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
               dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, C / 64 + 1) &&
-              dalloc(c->xyzm, C);
+              dalloc(c->xyzm, C) && dalloc(c->owner, 8 * C + 4096);
     if (!ok) {
         (void)hipGetLastError();
         free_buffers(c);
@@ -557,10 +558,13 @@ static int launch_build(csg_ctx *c, bool with64) {
     k_dd_top<<<1, CSG_TPB, 0, st>>>(c->dd_part, nt);
     k_dd_down<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part, c->pref,
                                       c->cap + 1);
+    const double inv_th2 = c->theta_hint > 0.0 ? 1.0 / (c->theta_hint * c->theta_hint) : HUGE_VAL;
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
         c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->xs, c->ys, c->zs, c->ms,
-        c->bp, with64 ? c->nodes : nullptr, c->nodes32,
-        c->theta_hint > 0.0 ? 1.0 / (c->theta_hint * c->theta_hint) : HUGE_VAL);
+        c->bp, with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->owner);
+    k_emit_internal<<<at_least1(blocks_for(2 * (long long)n + 1, CSG_TPB)), CSG_TPB, 0, st>>>(
+        c->key, c->ext, c->lcp, c->nbase, c->pref, c->cap + 1, c->bp,
+        with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->owner);
     CSG_LAUNCHED();
     c->tree_valid = 1;
     c->nodes64_valid = with64 ? 1 : 0;

@@ -857,13 +857,39 @@ __device__ __forceinline__ double dd_range(const DD *__restrict__ pref, int cap1
 }
 
 // ================================================================== emit
+// Node records.  The nodes a sorted body b opens are depths (min(L[b-1],63),
+// min(L[b],63)] (internal cells whose first member is b) plus its leaf, with
+// consecutive pre-order ids from nbase[b].  k_emit writes the leaves and, for
+// every internal id, the body that opens it (owner); k_emit_internal then gives
+// each internal node its own thread, whose one galloping search finds the cell's
+// last member (a body that opens many nested cells, like body 0 with all the
+// ancestors of the first leaf, no longer searches for them one after another).
+__device__ __forceinline__ void emit_internal(
+    const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext, const int *__restrict__ nbase,
+    const DD *__restrict__ pref, int cap1, const BuildParams *__restrict__ bp, int nl, int b,
+    int d, int id, WNode *__restrict__ nodes, WNode32 *__restrict__ n32, double inv_th2) {
+    const double H = bp->root[3];
+    const int e = range_end(key, ext, b, d, nl);
+    WNode w;
+    const double M = dd_range(pref, cap1, 0, b, e);
+    w.cx = dd_range(pref, cap1, 1, b, e) / M;
+    w.cy = dd_range(pref, cap1, 2, b, e) / M;
+    w.cz = dd_range(pref, cap1, 3, b, e) / M;
+    w.m = M;
+    const double s = 2.0 * ldexp(H, -d);
+    w.s2 = s * s;
+    w.skip = nbase[e + 1];
+    w.body = -1; w.depth = d; w.first = b; w.last = e; w.pad = 0;
+    store_nodes(nodes, n32, id, w, inv_th2);
+}
+
 __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext,
     const int *__restrict__ idx, const signed char *__restrict__ L, const int *__restrict__ nbase,
     const DD *__restrict__ pref, int cap1, const double *__restrict__ xs,
     const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ms,
     const BuildParams *__restrict__ bp, WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
-    double inv_th2) {
+    double inv_th2, int *__restrict__ owner) {
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
     if (bp->overflow) return;
@@ -885,21 +911,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const int lc = L[b];
     const int lo = min(lp, 63) + 1, hi = min(lc, 63);
     int id = nbase[b];
-    for (int d = lo; d <= hi; ++d, ++id) {
-        const int e = range_end(key, ext, b, d, nl);
-        WNode w;
-        const double M = dd_range(pref, cap1, 0, b, e);
-        w.cx = dd_range(pref, cap1, 1, b, e) / M;
-        w.cy = dd_range(pref, cap1, 2, b, e) / M;
-        w.cz = dd_range(pref, cap1, 3, b, e) / M;
-        w.m = M;
-        const double s = 2.0 * ldexp(H, -d);
-        w.s2 = s * s;
-        w.skip = nbase[e + 1];
-        w.body = -1; w.depth = d; w.first = b; w.last = e; w.pad = 0;
-        store_nodes(nodes, n32, id, w, inv_th2);
-    }
+    for (int d = lo; d <= hi; ++d, ++id) owner[id] = b;   // internal: k_emit_internal
     if (lp < 64) {
+        owner[id] = -1;
         WNode w;
         const int i = idx[b];
         int e = b;
@@ -922,6 +936,23 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
         w.skip = id + 1;
         w.body = i; w.depth = depth; w.first = b; w.last = e; w.pad = 0;
         store_nodes(nodes, n32, id, w, inv_th2);
+    }
+}
+
+// One thread per node id (grid-stride): internal nodes found through owner[].
+__global__ __launch_bounds__(CSG_TPB) void k_emit_internal(
+    const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext,
+    const signed char *__restrict__ L, const int *__restrict__ nbase, const DD *__restrict__ pref,
+    int cap1, const BuildParams *__restrict__ bp, WNode *__restrict__ nodes,
+    WNode32 *__restrict__ n32, double inv_th2, const int *__restrict__ owner) {
+    const int nl = bp->n_live, nn = bp->n_nodes;
+    if (bp->overflow || nl == 0) return;
+    for (int id = blockIdx.x * CSG_TPB + threadIdx.x; id < nn; id += gridDim.x * CSG_TPB) {
+        const int b = owner[id];
+        if (b < 0) continue;
+        const int lp = (b > 0) ? L[b - 1] : -1;
+        const int d = min(lp, 63) + 1 + (id - nbase[b]);
+        emit_internal(key, ext, nbase, pref, cap1, bp, nl, b, d, id, nodes, n32, inv_th2);
     }
 }
 
