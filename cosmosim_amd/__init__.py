@@ -63,6 +63,7 @@ _SIGS = {
     "cosmosim_ext_set_state": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "cosmosim_ext_direct_forces": (_I, [_VP, _VP, _I, _VP]),
     "cosmosim_ext_energy": (_I, [_VP, _VP, _VP, _VP]),
+    "cosmosim_ext_find_neighbors": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),
     "cosmosim_ext_interaction_counts": (_I, [_VP, _VP, _I]),
     "cosmosim_ext_get_stats": (_I, [_VP, ctypes.POINTER(ExtStats)]),
     "cosmosim_ext_set_timing": (_I, [_VP, _I]),
@@ -101,6 +102,7 @@ _SIGS = {
     "csg_walk_pool": (_I, [_VP, _VP, _I, _I, _D, _D, _D]),
     "csg_direct_forces": (_I, [_VP, _VP, _I, _D, _D, _VP]),
     "csg_energy": (_I, [_VP, _D, _D, _VP, _VP, _VP]),
+    "csg_find_neighbors": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),
     "csg_interaction_counts": (_I, [_VP, _VP, _I]),
     "csg_get_stats": (_I, [_VP, _VP]),
     "csg_set_timing": (_I, [_VP, _I]),
@@ -346,6 +348,19 @@ class Simulation(Handle):
 
     def set_timing(self, on=True):
         _check(lib().cosmosim_ext_set_timing(self.h, int(on)), "set_timing")
+
+    def find_neighbors(self, centres, radius, cap=128):
+        """Sphere queries over the device tree: a list of index arrays, one per
+        centre, equal to octree_find_neighbors' output for each (radius: scalar or
+        one per centre)."""
+        centres = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
+        k = len(centres)
+        rad = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, dtype=np.float64), (k,)))
+        out = np.zeros((k, max(cap, 1)), dtype=np.int32)
+        counts = np.zeros(k, dtype=np.int32)
+        _check(lib().cosmosim_ext_find_neighbors(self.h, ptr(centres), ptr(rad), k, cap, ptr(out),
+                                                 ptr(counts)), "find_neighbors")
+        return [out[q, :counts[q]].copy() for q in range(k)]
 
     def export_tree(self):
         cap = 8 * self.n + 4096

@@ -518,6 +518,23 @@ COSMOSIM_API int cosmosim_ext_export_tree(SimHandle h, OctreeNode *pool, int cap
     return csg_export_pool(((SimState *)h)->ctx, pool, capacity, pool_size);
 }
 
+COSMOSIM_API int cosmosim_ext_find_neighbors(SimHandle h, const double *centres,
+                                             const double *radius, int k, int cap, int *out,
+                                             int *counts)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    int st = csg_find_neighbors(s->ctx, centres, radius, k, cap, out, counts);
+    if (st == COSMOSIM_E_STATE) {
+        /* no tree for the current positions yet: build it, then query */
+        st = csg_build(s->ctx);
+        if (!st)
+            st = csg_find_neighbors(s->ctx, centres, radius, k, cap, out, counts);
+    }
+    return st;
+}
+
 COSMOSIM_API int cosmosim_ext_handle_error(SimHandle h)
 {
     return h ? ((SimState *)h)->error : COSMOSIM_E_ARG;

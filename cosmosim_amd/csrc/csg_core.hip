@@ -84,6 +84,11 @@ struct csg_ctx {
     int nodes64_valid = 0;  // the FP64 node records belong to the current tree
     double4 *xyzm = nullptr;  // build scratch: (x, y, z, m) interleaved, body order
     int *owner = nullptr;     // build scratch: body that opens each internal node id (-1: leaf)
+    double4 *geom = nullptr;  // neighbour queries: cell centre and half size per node
+    size_t geom_cap = 0;
+    int geom_valid = 0;       // geom describes the current tree
+    void *nbuf = nullptr;     // neighbour queries: scratch
+    size_t nbuf_cap = 0;
     int *wcost = nullptr;   // FP32 wave-mask walk: node steps of each wave in the last walk
     int *worder = nullptr;  // dispatch order built from wcost (longest first)
     int wcost_n = 0;        // number of waves wcost describes (0: none)
@@ -111,6 +116,7 @@ struct csg_ctx {
 
 static void adj_free(csg_ctx *c);  // csg_adjoint.inl
 static void frozen_free(csg_ctx *c);
+static void nb_free(csg_ctx *c);   // csg_neighbors.inl
 
 static inline int blocks_for(long long n, int per) { return (int)((n + per - 1) / per); }
 static inline int at_least1(int v) { return v < 1 ? 1 : v; }
@@ -318,6 +324,7 @@ void csg_destroy(csg_ctx *c) {
     free_buffers(c);
     adj_free(c);
     frozen_free(c);
+    nb_free(c);
     dfree(c->bp);
     dfree(c->itotal);
     dfree(c->err_dev);
@@ -567,6 +574,7 @@ static int launch_build(csg_ctx *c, bool with64) {
         with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->owner);
     CSG_LAUNCHED();
     c->tree_valid = 1;
+    c->geom_valid = 0;
     c->nodes64_valid = with64 ? 1 : 0;
     c->th32 = c->theta_hint;
     c->tape_valid = 0;   // a tape describes the tree it was recorded on
@@ -969,3 +977,4 @@ void csg_host_free(void *p) {
 }  // extern "C"
 
 #include "csg_adjoint.inl"
+#include "csg_neighbors.inl"

@@ -112,6 +112,15 @@ int cosmosim_ext_get_stats(SimHandle h, CosmosimExtStats *out);
 int cosmosim_ext_set_timing(SimHandle h, int enabled);
 /* Write the last tree in reference layout (root 0); capacity in nodes. */
 int cosmosim_ext_export_tree(SimHandle h, OctreeNode *pool, int capacity, int *pool_size);
+/* Batched sphere queries over the device tree: the GPU form of
+ * octree_find_neighbors (src/octree.c:236-318) for many centres at once, as the
+ * SPH passes issue them (src/sph.c:57,120).  centres [k][3], radius [k];
+ * out [k][cap] receives the first counts[q] <= cap bodies (original indices) of
+ * query q, in the reference's visiting order (children in slot order), so the
+ * lists equal the reference's, cap included.  Builds the tree if the handle's
+ * positions have none yet. */
+int cosmosim_ext_find_neighbors(SimHandle h, const double *centres, const double *radius, int k,
+                                int cap, int *out, int *counts);
 /* Last error recorded on the handle by a void legacy call (cosmosim_step). */
 int cosmosim_ext_handle_error(SimHandle h);
 /* Last error recorded on the calling thread by octree_* / integrator_*. */

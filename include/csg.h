@@ -66,6 +66,11 @@ int  csg_substeps(csg_ctx *ctx, int k, double dt, double G, double softening,
 /* Last tree in reference layout (root 0); fails with COSMOSIM_E_CAPACITY when
  * the tree has more than capacity nodes. */
 int  csg_export_pool(csg_ctx *ctx, OctreeNode *pool, int capacity, int *pool_size);
+/* Sphere queries over the current tree (batched GPU form of src/octree.c:236-318):
+ * centres [k][3], radii [k]; out [k][cap] receives the first counts[q] <= cap
+ * reported bodies of query q in the reference's visiting order. */
+int  csg_find_neighbors(csg_ctx *ctx, const double *centres, const double *radius, int k,
+                        int cap, int *out, int *counts);
 /* Walk a caller-supplied reference-layout pool from `root` (exact reference
  * traversal, FP64 strict arithmetic). */
 int  csg_walk_pool(csg_ctx *ctx, const OctreeNode *pool, int pool_size, int root,
