@@ -105,6 +105,7 @@ def cpu_baseline(ca, bodies, theta):
         "value": 1.0 / med, "unit": "substeps/s", "cores": threads, "kind": impl.kind,
         "sample": (f"3 KDK substeps (median) of the same 1M-body merger at theta {theta}, "
                    f"eps {EPS}: serial build, OpenMP walk ({threads} threads), serial KDK"),
+        "host_cpu": host_cpu(), "nproc": os.cpu_count(),
         "seconds_per_substep": med,
     }
 
@@ -150,6 +151,38 @@ def adjoint_runs(ca, big_n, big_steps):
     return out
 
 
+def neighbor_runs(ca, bodies, radius=0.03, cap=64):
+    """Batched sphere queries (the GPU octree_find_neighbors) around every body of
+    the 1M merger on its FP32-path tree: kernel rate from the core's events and
+    the call's rate including the PCIe copies of centres and lists."""
+    sim = ca.Simulation(bodies, softening=EPS, theta=0.5, dt=DT, precision="fp32")
+    pos = ca.positions(bodies)
+    sim.find_neighbors(pos[:1000], radius, cap)          # geometry + buffers
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    lists = sim.find_neighbors(pos, radius, cap)
+    t1 = time.perf_counter()
+    st = sim.stats()
+    counts = np.fromiter((len(x) for x in lists), dtype=np.int64, count=len(lists))
+    sim.close()
+    return {"queries": len(pos), "radius": radius, "cap": cap,
+            "queries_per_s_kernel": st.neighbor_queries / (st.ms_neighbors * 1e-3),
+            "kernel_ms": st.ms_neighbors,
+            "queries_per_s_with_transfers": len(pos) / (t1 - t0),
+            "mean_neighbors": float(counts.mean()), "capped_fraction": float(np.mean(counts == cap))}
+
+
+def host_cpu():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def copy_calibration(nbytes=1 << 30, launches=5):
     """A plain device copy of known size, to scale rocprofv3's FETCH_SIZE /
     WRITE_SIZE counters: torch's elementwise negation of a float32 array
@@ -184,6 +217,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-bodies", type=int, default=4_000_000)
     ap.add_argument("--adjoint-steps", type=int, default=20)
+    ap.add_argument("--no-neighbors", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to scale the traffic counters")
     args = ap.parse_args()
@@ -258,6 +292,19 @@ def main():
         },
         "device": {"name": dev_name, "compute_units": cus},
     }
+    # HBM fractions of the HBM-bound phases (algorithmic bytes: build reads x, y, z, m,
+    # writes and reads key + index, writes 40 B per node; inside a batch the KDK is one
+    # fused kick + drift per substep, reading x, v, a, m and writing x, v: 128 B per body)
+    build_bytes = N_BODIES * (32 + 24) + head["nodes"] * 40
+    line["phases"] = {
+        "walk_ms": round(head["walk_ms"], 4), "build_ms": round(head["build_ms"], 4),
+        "kdk_ms": round(head["kdk_ms"], 4),
+        "build_hbm_frac": round(build_bytes / (head["build_ms"] * 1e-3) / 8e12, 4),
+        "kdk_hbm_frac": round(N_BODIES * 128 / (head["kdk_ms"] * 1e-3) / 8e12, 4),
+        "interactions_per_s": float(f"{head['interactions'] / (head['walk_ms'] * 1e-3):.4g}"),
+    }
+    free_b, total_b = ca.device_memory(0)
+    line["hbm_total_bytes"] = int(total_b)
     fp64 = results.get("theta0.5_fp64")
     strict = results.get("theta0.5_fp64_strict")
     if fp64:
@@ -273,6 +320,9 @@ def main():
                                for kk, vv in v.items()} for k, v in adj.items()}
         big = adj[f"config4_{args.adjoint_bodies}_fp32_theta0.7"]
         line["adjoint_substeps_per_s"] = round(big["adjoint_substeps_per_s_checkpoint"], 3)
+    if not args.no_neighbors:
+        line["neighbors"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
+                             for k, v in neighbor_runs(ca, bodies).items()}
     if not args.no_cpu:
         cb = cpu_baseline(ca, bodies, 0.5)
         line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v)

@@ -89,6 +89,8 @@ class ExtStats(ctypes.Structure):
         ("ms_build", ctypes.c_double),
         ("ms_walk", ctypes.c_double),
         ("ms_kdk", ctypes.c_double),
+        ("ms_neighbors", ctypes.c_double),
+        ("neighbor_queries", ctypes.c_longlong),
     ]
 
 

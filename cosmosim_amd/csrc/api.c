@@ -500,6 +500,8 @@ COSMOSIM_API int cosmosim_ext_get_stats(SimHandle h, CosmosimExtStats *out)
     out->ms_walk = cs.ms_walk;
     out->ms_kdk = cs.ms_kdk;
     out->timed_walks = cs.timed_walks;
+    out->ms_neighbors = cs.ms_neighbors;
+    out->neighbor_queries = cs.neighbor_queries;
     return COSMOSIM_OK;
 }
 

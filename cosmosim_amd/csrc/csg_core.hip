@@ -12,7 +12,7 @@
 #include "csg.h"
 
 #define CSG_EV_MAX 4096
-enum { PH_BUILD = 0, PH_WALK = 1, PH_KDK = 2 };
+enum { PH_BUILD = 0, PH_WALK = 1, PH_KDK = 2, PH_NB = 3 };
 
 struct csg_ctx {
     int device = 0;
@@ -54,7 +54,8 @@ struct csg_ctx {
     hipEvent_t *ev = nullptr;  // 2 * CSG_EV_MAX events, created on first use
     int ev_phase[CSG_EV_MAX];
     int ev_count = 0, ev_open = -1;
-    double ms_build = 0, ms_walk = 0, ms_kdk = 0;
+    double ms_build = 0, ms_walk = 0, ms_kdk = 0, ms_nb = 0;
+    long long nb_queries = 0;
     int timed_walks = 0;
     // adjoint: walk tape, transposed records, per-body adjoint buffers, checkpoints
     int tape_on = 0, tape_cap = 0, tape_waves = 0, tape_valid = 0;
@@ -224,6 +225,7 @@ static void collect_timers(csg_ctx *c) {
         (void)hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]);
         if (c->ev_phase[k] == PH_BUILD) c->ms_build += ms;
         else if (c->ev_phase[k] == PH_WALK) { c->ms_walk += ms; c->timed_walks += 1; }
+        else if (c->ev_phase[k] == PH_NB) c->ms_nb += ms;
         else c->ms_kdk += ms;
     }
     c->ev_count = 0;
@@ -934,6 +936,8 @@ int csg_get_stats(csg_ctx *c, csg_stats *o) {
     o->ms_walk = c->ms_walk;
     o->ms_kdk = c->ms_kdk;
     o->timed_walks = c->timed_walks;
+    o->ms_neighbors = c->ms_nb;
+    o->neighbor_queries = c->nb_queries;
     return COSMOSIM_OK;
 }
 
@@ -948,8 +952,9 @@ int csg_set_timing(csg_ctx *c, int enabled) {
     c->timing = enabled ? 1 : 0;
     c->ev_count = 0;
     c->ev_open = -1;
-    c->ms_build = c->ms_walk = c->ms_kdk = 0.0;
+    c->ms_build = c->ms_walk = c->ms_kdk = c->ms_nb = 0.0;
     c->timed_walks = 0;
+    c->nb_queries = 0;
     return COSMOSIM_OK;
 }
 

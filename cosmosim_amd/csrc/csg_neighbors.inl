@@ -194,13 +194,17 @@ int csg_find_neighbors(csg_ctx *c, const double *centres, const double *radius, 
         uint64_t *tk = kin; kin = kout; kout = tk;
         int *tv = vin; vin = vout; vout = tv;
     }
+    timer_start(c);
     k_neighbors<<<blocks_for(k, CSG_TPB), CSG_TPB, 0, st>>>(
         c->nodes32, c->geom, c->bp, c->x, c->y, c->z, c->m, dq, dr, vin, k, cap, dout, dcnt);
+    timer_stop(c, PH_NB);
     CSG_LAUNCHED();
+    if (c->timing) c->nb_queries += k;
     if (cap > 0)
         CSG_CHECK(hipMemcpyAsync(out, dout, sizeof(int) * (size_t)k * cap, hipMemcpyDeviceToHost, st));
     CSG_CHECK(hipMemcpyAsync(counts, dcnt, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost, st));
     CSG_CHECK(hipStreamSynchronize(st));
+    collect_timers(c);
     return COSMOSIM_OK;
 }
 

@@ -60,6 +60,8 @@ typedef struct {
     double ms_build;       /* GPU time in build / walk / KDK since timing was enabled (0 if off) */
     double ms_walk;
     double ms_kdk;
+    double ms_neighbors;   /* GPU time of neighbour-query traversals since timing was enabled */
+    long long neighbor_queries; /* queries covered by ms_neighbors */
 } CosmosimExtStats;
 
 /* Seeded initial conditions (own generator; the reference's generators are

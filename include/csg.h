@@ -30,6 +30,8 @@ typedef struct {
     double ms_build, ms_walk, ms_kdk; /* GPU time per phase since timing was (re)enabled */
     int timed_walks;                /* walk launches covered by ms_walk */
     int pad;
+    double ms_neighbors;            /* GPU time of neighbour-query traversals (timing on) */
+    long long neighbor_queries;     /* queries covered by ms_neighbors */
 } csg_stats;
 
 int  csg_device_count(int *count);
