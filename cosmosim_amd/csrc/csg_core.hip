@@ -31,6 +31,9 @@ struct csg_ctx {
     uint64_t *key = nullptr, *key2 = nullptr, *ext = nullptr, *ext_tmp = nullptr;
     int *idx = nullptr, *idx2 = nullptr;
     int *rs_hist = nullptr, *rs_tot = nullptr;
+    RsCtl *rs_ctl = nullptr;           // body sort: tickets and digit counts
+    uint64_t *rs_status = nullptr;     // body sort: per-tile digit counts (look-back)
+    unsigned rs_epoch = 0;             // epoch of the last sort pass
     double *xs = nullptr, *ys = nullptr, *zs = nullptr, *ms = nullptr;  // sorted copies
     int *depth_part = nullptr;
     int *runs = nullptr;
@@ -139,7 +142,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->vx); dfree(c->vy); dfree(c->vz);
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->m); dfree(c->aos);
     dfree(c->bbox_part); dfree(c->key); dfree(c->key2); dfree(c->ext); dfree(c->ext_tmp);
-    dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
+    dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
     dfree(c->depth_part);
     dfree(c->runs); dfree(c->lcp); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
@@ -147,6 +150,19 @@ static void free_buffers(csg_ctx *c) {
     c->wcost_n = 0;
     c->cap = 0;
     c->node_cap = 0;
+}
+
+template <class T>
+__global__ __launch_bounds__(CSG_TPB) void k_clear(T *__restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * CSG_TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * CSG_TPB)
+        p[i] = T(0);
+}
+
+template <class T>
+static void clear_words(T *p, size_t n, hipStream_t st) {
+    if (n == 0) return;
+    const size_t b = (n + CSG_TPB - 1) / CSG_TPB;
+    k_clear<T><<<(unsigned)(b < 4096 ? b : 4096), CSG_TPB, 0, st>>>(p, n);
 }
 
 // Device buffers for `cap` bodies; about 900 bytes per body (the node records,
@@ -168,11 +184,21 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
               dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
               dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, C / 64 + 1) &&
-              dalloc(c->xyzm, C) && dalloc(c->owner, 8 * C + 4096);
+              dalloc(c->xyzm, C) && dalloc(c->owner, 8 * C + 4096) && dalloc(c->rs_ctl, 1) &&
+              dalloc(c->rs_status, (size_t)256 * blocks_for((long long)C, RS_TILE));
     if (!ok) {
         (void)hipGetLastError();
         free_buffers(c);
         return COSMOSIM_E_ALLOC;
+    }
+    // look-back words start unpublished; the index buffers start as valid indices
+    clear_words(c->rs_status, (size_t)256 * blocks_for((long long)C, RS_TILE), c->stream);
+    clear_words(c->idx, C, c->stream);
+    clear_words(c->idx2, C, c->stream);
+    c->rs_epoch = 0;
+    if (hipGetLastError() != hipSuccess) {
+        free_buffers(c);
+        return COSMOSIM_E_HIP;
     }
     c->cap = (int)C;
     c->node_cap = (int)(8 * C + 4096);
@@ -529,25 +555,33 @@ static int launch_build(csg_ctx *c, bool with64) {
                                                                        : CSG_BBOX_BLOCKS)
                           : 1;
     k_bbox_partial<<<nbb, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bbox_part);
-    k_bbox_final<<<1, CSG_TPB, 0, st>>>(c->bbox_part, nbb, c->bp, c->node_cap);
+    k_bbox_final<<<1, CSG_TPB, 0, st>>>(c->bbox_part, nbb, c->bp, c->node_cap, c->rs_ctl);
     CSG_LAUNCHED();
     if (n > 0) {
+        const int nt = blocks_for(n, RS_TILE);
         k_keys<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bp, c->key,
-                                                          c->idx, c->xyzm);
-        const int nb = blocks_for(n, CSG_TILE);
+                                                          c->xyzm);
+        k_rs_count<<<nt, CSG_TPB, 0, st>>>(c->key, n, c->rs_ctl);
+        if (c->rs_epoch > 0x7fffff00u) {   // epochs are 31 bits: start the words over
+            clear_words(c->rs_status, (size_t)256 * blocks_for(c->cap, RS_TILE), st);
+            c->rs_epoch = 0;
+        }
         uint64_t *kin = c->key, *kout = c->key2;
         int *vin = c->idx, *vout = c->idx2;
-        for (int pass = 0; pass < 8; ++pass) {
-            const int shift = 8 * pass;
-            k_rs_hist<<<nb, CSG_TPB, 0, st>>>(kin, n, shift, c->rs_hist, nb);
-            k_rs_rowscan<<<256, CSG_TPB, 0, st>>>(c->rs_hist, nb, c->rs_tot);
-            k_rs_scatter<<<nb, CSG_TPB, 0, st>>>(kin, vin, kout, vout, n, shift, c->rs_hist,
-                                                 c->rs_tot, nb);
+        static_assert(CSG_SORT_PASSES % 2 == 0, "the sorted pairs end in key / idx");
+        for (int pass = 0; pass < CSG_SORT_PASSES; ++pass) {
+            const unsigned ep = ++c->rs_epoch;
+            if (pass == 0)
+                k_rs_sweep<true><<<nt, RS_TPB, 0, st>>>(kin, nullptr, kout, vout, n, pass, c->rs_ctl,
+                                                         c->rs_status, ep, c->bp);
+            else
+                k_rs_sweep<false><<<nt, RS_TPB, 0, st>>>(kin, vin, kout, vout, n, pass, c->rs_ctl,
+                                                          c->rs_status, ep, c->bp);
             uint64_t *tk = kin; kin = kout; kout = tk;
             int *tv = vin; vin = vout; vout = tv;
         }
         CSG_LAUNCHED();
-        // 8 passes: the sorted pairs are back in key / idx
+        // an even number of passes: the sorted pairs are back in key / idx
         k_runs_mark<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->key, c->bp, c->runs);
         k_runs_fix<<<256, CSG_TPB, 0, st>>>(c->key, c->idx, c->x, c->y, c->z, c->bp, c->runs,
                                             c->ext, c->idx2, c->ext_tmp);
@@ -716,6 +750,10 @@ static int check_overflow(csg_ctx *c) {
     if (c->hbp->overflow) {
         c->tree_valid = 0;
         c->counts_valid = 0;
+        if (c->hbp->sort_stall) {
+            fprintf(stderr, "cosmosim: body sort stalled waiting for an earlier tile\n");
+            return COSMOSIM_E_HIP;
+        }
         return COSMOSIM_E_CAPACITY;
     }
     return COSMOSIM_OK;

@@ -3,10 +3,10 @@
 //
 // Pipeline per substep (all on one stream, no host synchronisation):
 //   bbox  : min/max over live bodies -> root cube            (src/octree.c:119-159)
-//   keys  : 21-level octant path per body from the FP64 cell recurrence
+//   keys  : 16-level octant path per body from the FP64 cell recurrence (48 bits)
 //                                                            (src/octree.c:30-50, 87-108)
-//   sort  : LSD radix sort of (key, body) pairs, 8 passes of 8 bits
-//   runs  : bodies sharing all 21 levels get their levels 21..63 and are
+//   sort  : LSD radix sort of (key, body) pairs, 6 passes of 8 bits
+//   runs  : bodies sharing all 16 levels get their levels 16..63 and are
 //           re-ranked inside the run                         (src/octree.c:71-79 depth limit)
 //   lcp   : common-prefix depth of sorted neighbours -> nodes per body
 //   scan  : node ids in depth-first pre-order
@@ -26,7 +26,17 @@
 #define CSG_TPB 256
 #define CSG_ITEMS 8
 #define CSG_TILE (CSG_TPB * CSG_ITEMS)
-#define CSG_KEY_LEVELS 21
+// Levels in the sorted key.  Bodies sharing all of them (rare: leaves sit at
+// depths 10-12 at a million bodies) form collision runs resolved with their
+// deeper levels.  3 * CSG_KEY_LEVELS bits, sorted in CSG_SORT_PASSES 8-bit passes.
+// A live key is never all ones in those bits (that cell lies in the 1% margin
+// of the root cube), so the all-ones key of dead bodies sorts last.
+#define CSG_KEY_LEVELS 16
+#define CSG_KEY_BITS (3 * CSG_KEY_LEVELS)
+#define CSG_SORT_PASSES ((CSG_KEY_BITS + 7) / 8)
+// collision words: ext[0] levels L..L+20, ext[1] L+21..L+41, ext[2] L+42..63
+#define CSG_EXT2_LEVELS (64 - CSG_KEY_LEVELS - 42)
+static_assert(CSG_EXT2_LEVELS >= 1 && CSG_EXT2_LEVELS <= 21, "three collision words cover levels L..63");
 #define CSG_DEPTH_LIMIT 64
 #define CSG_BBOX_BLOCKS 1024
 
@@ -35,11 +45,12 @@ struct BuildParams {
     double root[4];  // centre x, y, z and half size H of the root cube
     int n_live;      // bodies with !(mass <= 0)
     int n_nodes;     // nodes of the tree (reference count: internal + leaves)
-    int n_runs;      // groups of sorted bodies with equal 21-level keys
+    int n_runs;      // groups of sorted bodies with equal keys (all CSG_KEY_LEVELS levels)
     int max_depth;   // deepest leaf
     int overflow;    // 1 when n_nodes exceeds the node capacity
     int node_cap;
-    int pad[2];
+    int sort_stall;  // 1 when a radix pass gave up waiting for an earlier tile
+    int pad;
     unsigned long long interactions;  // body-node interactions of the last walk
 };
 
@@ -175,36 +186,41 @@ __device__ __forceinline__ uint64_t path_word(double px, double py, double pz, d
     return k;
 }
 
-// Levels 21..63 of a body path: ext[0] levels 21..41, ext[1] 42..62, ext[2] level 63.
+// Levels CSG_KEY_LEVELS..63 of a body path in three words (21, 21 and
+// CSG_EXT2_LEVELS levels).
 __device__ __forceinline__ void path_ext(double px, double py, double pz, const double *root,
                                          uint64_t *e) {
     double cx = root[0], cy = root[1], cz = root[2], h = root[3];
-    (void)path_word(px, py, pz, cx, cy, cz, h, 21);
+    (void)path_word(px, py, pz, cx, cy, cz, h, CSG_KEY_LEVELS);
     e[0] = path_word(px, py, pz, cx, cy, cz, h, 21);
     e[1] = path_word(px, py, pz, cx, cy, cz, h, 21);
-    e[2] = path_word(px, py, pz, cx, cy, cz, h, 1);
+    e[2] = path_word(px, py, pz, cx, cy, cz, h, CSG_EXT2_LEVELS);
 }
 
 // Number of leading levels two sorted bodies share (0..64).  Bodies with equal
-// 21-level keys belong to one collision run and have their ext words.
+// keys belong to one collision run and have their ext words.  (A word of w
+// levels sits in the low 3w bits, so its first level is bit 3w-1.)
 __device__ __forceinline__ int lcp_sorted(const uint64_t *__restrict__ key,
                                           const uint64_t *__restrict__ ext, int a, int b) {
+    constexpr int L = CSG_KEY_LEVELS;
     const uint64_t ka = key[a], kb = key[b];
-    if (ka != kb) return (__clzll((long long)(ka ^ kb)) - 1) / 3;
+    if (ka != kb) return (__clzll((long long)(ka ^ kb)) - (64 - 3 * L)) / 3;
     const uint64_t *ea = ext + 3 * (size_t)a, *eb = ext + 3 * (size_t)b;
-    if (ea[0] != eb[0]) return 21 + (__clzll((long long)(ea[0] ^ eb[0])) - 1) / 3;
-    if (ea[1] != eb[1]) return 42 + (__clzll((long long)(ea[1] ^ eb[1])) - 1) / 3;
-    if (ea[2] != eb[2]) return 63;
+    if (ea[0] != eb[0]) return L + (__clzll((long long)(ea[0] ^ eb[0])) - 1) / 3;
+    if (ea[1] != eb[1]) return L + 21 + (__clzll((long long)(ea[1] ^ eb[1])) - 1) / 3;
+    if (ea[2] != eb[2])
+        return L + 42 + (__clzll((long long)(ea[2] ^ eb[2])) - (64 - 3 * CSG_EXT2_LEVELS)) / 3;
     return 64;
 }
 
 // Octant digit of sorted body b at level l.
 __device__ __forceinline__ int digit_at(const uint64_t *key, const uint64_t *ext, int b, int l) {
-    if (l < 21) return (int)((key[b] >> (3 * (20 - l))) & 7);
+    constexpr int L = CSG_KEY_LEVELS;
+    if (l < L) return (int)((key[b] >> (3 * (L - 1 - l))) & 7);
     const uint64_t *e = ext + 3 * (size_t)b;
-    if (l < 42) return (int)((e[0] >> (3 * (41 - l))) & 7);
-    if (l < 63) return (int)((e[1] >> (3 * (62 - l))) & 7);
-    return (int)(e[2] & 7);
+    if (l < L + 21) return (int)((e[0] >> (3 * (L + 20 - l))) & 7);
+    if (l < L + 42) return (int)((e[1] >> (3 * (L + 41 - l))) & 7);
+    return (int)((e[2] >> (3 * (63 - l))) & 7);
 }
 
 // Last sorted body j >= b sharing at least d levels with b (galloping search).
@@ -299,6 +315,18 @@ __device__ __forceinline__ int block_excl_scan(int v, int *lds4, int &total) {
     return off + inc - v;
 }
 
+// Control block of the body sort: the tile tickets of the passes and the count
+// of every 8-bit digit value in every pass (pass p sorts bits 8p..8p+7).
+#define RS_TPB 1024                  // threads per block of the sort passes
+#define RS_ITEMS 4                   // keys per thread in the sort passes
+#define RS_WAVES (RS_TPB / 64)
+#define RS_TILE (RS_TPB * RS_ITEMS)  // keys per tile
+#define RS_COUNT_ITEMS (RS_TILE / CSG_TPB)
+struct RsCtl {
+    int ticket[CSG_SORT_PASSES];
+    int hist[CSG_SORT_PASSES][256];
+};
+
 // =================================================================== bbox
 __global__ __launch_bounds__(CSG_TPB) void k_bbox_partial(const double *__restrict__ x,
                                                           const double *__restrict__ y,
@@ -337,7 +365,11 @@ __global__ __launch_bounds__(CSG_TPB) void k_bbox_partial(const double *__restri
 
 // Root cube exactly as src/octree.c:141-159; also resets the per-build counters.
 __global__ __launch_bounds__(CSG_TPB) void k_bbox_final(const double *__restrict__ part, int nblk,
-                                                        BuildParams *bp, int node_cap) {
+                                                        BuildParams *bp, int node_cap,
+                                                        RsCtl *__restrict__ ctl) {
+    // the sort's tickets and digit counts start from zero in every build
+    for (int k = threadIdx.x; k < (int)(sizeof(RsCtl) / sizeof(int)); k += CSG_TPB)
+        ((int *)ctl)[k] = 0;
     double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     double cnt = 0.0;
     for (int b = threadIdx.x; b < nblk; b += CSG_TPB) {
@@ -369,6 +401,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_bbox_final(const double *__restrict
         bp->n_runs = 0;
         bp->max_depth = 0;
         bp->overflow = 0;
+        bp->sort_stall = 0;
         bp->node_cap = node_cap;
         bp->interactions = 0ull;
         bp->n_nodes = 1;
@@ -389,8 +422,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_bbox_final(const double *__restrict
 }
 
 // =================================================================== keys
-// Dead bodies get the all-ones key and sort behind every live body; their
-// acceleration is zeroed here (src/octree.c:227-231).
+// Dead bodies get the all-ones key and sort behind every live body.
 // Also writes (x, y, z, m) interleaved, one 32-byte line per body, so the sorted
 // gather after the sort reads one sector per body instead of four.
 __global__ __launch_bounds__(CSG_TPB) void k_keys(const double *__restrict__ x,
@@ -399,7 +431,6 @@ __global__ __launch_bounds__(CSG_TPB) void k_keys(const double *__restrict__ x,
                                                   const double *__restrict__ m, int n,
                                                   const BuildParams *__restrict__ bp,
                                                   uint64_t *__restrict__ key,
-                                                  int *__restrict__ idx,
                                                   double4 *__restrict__ xyzm) {
     const int i = blockIdx.x * CSG_TPB + threadIdx.x;
     if (i >= n) return;
@@ -410,8 +441,35 @@ __global__ __launch_bounds__(CSG_TPB) void k_keys(const double *__restrict__ x,
         k = path_word(px, py, pz, cx, cy, cz, h, CSG_KEY_LEVELS);
     }
     key[i] = k;
-    idx[i] = i;
     xyzm[i] = make_double4(px, py, pz, pm);
+}
+
+// Count of every digit value of every sort pass (the sort's global digit
+// offsets) from one read of the keys: RS_TILE keys per block, per-block counts
+// in LDS, then one global add per nonzero count.
+__global__ __launch_bounds__(CSG_TPB) void k_rs_count(const uint64_t *__restrict__ key, int n,
+                                                      RsCtl *__restrict__ ctl) {
+    __shared__ int h[CSG_SORT_PASSES][256];
+    for (int k = threadIdx.x; k < CSG_SORT_PASSES * 256; k += CSG_TPB) (&h[0][0])[k] = 0;
+    const int base = blockIdx.x * RS_TILE + threadIdx.x;
+    uint64_t kk[RS_COUNT_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_COUNT_ITEMS; ++r) {
+        const int i = base + r * CSG_TPB;
+        kk[r] = i < n ? key[i] : 0ull;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_COUNT_ITEMS; ++r) {
+        if (base + r * CSG_TPB >= n) break;
+#pragma unroll
+        for (int p = 0; p < CSG_SORT_PASSES; ++p) atomicAdd(&h[p][(int)((kk[r] >> (8 * p)) & 255)], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < CSG_SORT_PASSES * 256; k += CSG_TPB) {
+        const int v = (&h[0][0])[k];
+        if (v) atomicAdd(&(&ctl->hist[0][0])[k], v);
+    }
 }
 
 // ============================================================ radix sort
@@ -526,6 +584,215 @@ __global__ __launch_bounds__(CSG_TPB) void k_rs_scatter(
     }
 }
 
+// ============================================== single-sweep radix sort
+// The body sort: CSG_SORT_PASSES LSD passes of 8-bit digits over the key
+// bits, one kernel per pass.  The counts of every digit value come from k_keys, so a pass
+// needs no histogram of its own; what a tile needs besides is how many keys of
+// each digit value the tiles before it hold.  Tiles are numbered by ticket in
+// the order their blocks start.  Each tile publishes its own count per digit
+// value at once, and the running total (its own plus everything before) as
+// soon as it knows it; a tile learns its prefix by adding published counts
+// backwards from the previous tile until it meets a running total (decoupled
+// look-back).  A tile only waits on tiles with smaller tickets, whose blocks
+// are resident already, so every wait ends; the waits are also bounded
+// (RS_SPIN_LIMIT rounds that find nothing new) after which the build flags a stalled sort (and an
+// overflow, so the kernels after it do nothing) instead of waiting further.
+//
+// status[tile][digit] = epoch << 33 | running-total flag << 32 | count.  Every
+// pass uses a new epoch, so words left by earlier passes read as not yet
+// published and the array is never cleared between passes.
+#define RS_SPIN_LIMIT (1 << 20)
+#define RS_WINDOW 8
+
+__device__ __forceinline__ uint64_t rs_word(unsigned epoch, bool total, int count) {
+    return ((uint64_t)epoch << 33) | ((uint64_t)(total ? 1 : 0) << 32) | (uint64_t)(unsigned)count;
+}
+
+// Load the tile's keys and values: wave w holds elements w*64*RS_ITEMS + r*64 +
+// lane (round r), the order the ranking keeps.
+template <bool FIRST>
+__device__ __forceinline__ void rs_load(const uint64_t *__restrict__ kin,
+                                        const int *__restrict__ vin, int n, int tile, int w,
+                                        int lane, uint64_t (&kk)[RS_ITEMS], int (&vv)[RS_ITEMS]) {
+    const int base = tile * RS_TILE + w * (64 * RS_ITEMS) + lane;
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int i = base + r * 64;
+        kk[r] = i < n ? kin[i] : ~0ull;
+        vv[r] = FIRST ? i : (i < n ? vin[i] : 0);
+    }
+}
+
+__device__ __forceinline__ uint64_t rs_status_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pass `pass` (bits 8*pass .. 8*pass+7).  FIRST: the values are the positions
+// of the keys (the body indices), so vin is not read.  Stable: within a tile
+// the element order is (wave, round, lane), as the loads.
+//
+// Order of work, so the waits overlap: the tile is loaded speculatively as
+// tile blockIdx.x before the ticket is taken (blocks almost always start in
+// order; a different ticket reloads); the tile's digit counts (LDS adds) are
+// published at once and the first look-back window is requested; the stable
+// ranking (ballots, 16 waves per block so their latencies overlap) runs while
+// those loads are in flight.  Threads 0..255 own one digit value each.
+template <bool FIRST>
+__global__ __launch_bounds__(RS_TPB) void k_rs_sweep(
+    const uint64_t *__restrict__ kin, const int *__restrict__ vin, uint64_t *__restrict__ kout,
+    int *__restrict__ vout, int n, int pass, RsCtl *__restrict__ ctl,
+    uint64_t *__restrict__ status, unsigned epoch, BuildParams *__restrict__ bp) {
+    __shared__ int s_tile;
+    __shared__ int tcnt[256];
+    __shared__ unsigned short wcnt[RS_WAVES][256];   // per-wave counts, then offsets
+    __shared__ int tstart[256];
+    __shared__ int binpos[256];
+    __shared__ int lback[256];
+    __shared__ uint64_t skey[RS_TILE];
+    __shared__ int sval[RS_TILE];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool owner = tid < 256;   // owns digit value d = tid
+    const int d = tid & 255;
+    uint64_t kk[RS_ITEMS];
+    int vv[RS_ITEMS];
+    rs_load<FIRST>(kin, vin, n, (int)blockIdx.x, w, lane, kk, vv);
+    if (tid == 0) s_tile = atomicAdd(&ctl->ticket[pass], 1);
+    if (owner) tcnt[d] = 0;
+    for (int k = tid; k < RS_WAVES * 256; k += RS_TPB) (&wcnt[0][0])[k] = 0;
+    __syncthreads();
+    const int tile = s_tile;
+    if (tile != (int)blockIdx.x) rs_load<FIRST>(kin, vin, n, tile, w, lane, kk, vv);
+    const int shift = 8 * pass;
+    const int base = tile * RS_TILE + w * (64 * RS_ITEMS);
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r)
+        if (base + r * 64 + lane < n) atomicAdd(&tcnt[(int)((kk[r] >> shift) & 255)], 1);
+    __syncthreads();
+    int cnt_d = 0, t = tile - 1;
+    uint64_t v[RS_WINDOW];
+    uint64_t *const mine = status + (size_t)tile * 256 + d;
+    if (owner) {
+        cnt_d = tcnt[d];
+        __hip_atomic_store(mine, rs_word(epoch, tile == 0, cnt_d), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        // first look-back window, consumed after the ranking
+#pragma unroll
+        for (int k = 0; k < RS_WINDOW; ++k)
+            v[k] = t - k >= 0 ? rs_status_load(status + (size_t)(t - k) * 256 + d) : 0ull;
+    }
+    // rank inside the wave's sub-tile: lanes with the same digit value are
+    // found with eight ballots (lane j matches bit b when ballot_b ^ flip_b has
+    // bit j set, flip_b = all ones where this lane's bit b is clear); the lowest
+    // such lane bumps the wave's counter
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int rk[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const bool valid = base + r * 64 + lane < n;
+        const int dg = (int)((kk[r] >> shift) & 255);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const uint64_t bal = __ballot((dg >> bit) & 1);
+            const uint64_t flip = (uint64_t)(int64_t)(((dg >> bit) & 1) - 1);
+            peers &= bal ^ flip;
+        }
+        const int below = __popcll(peers & lt);
+        const int old = valid ? (int)wcnt[w][dg] : 0;
+        if (valid && below == 0) wcnt[w][dg] = (unsigned short)(old + __popcll(peers));
+        rk[r] = old + below;
+    }
+    // look-back for digit value d, RS_WINDOW tiles per round trip: counts are
+    // added from tile-1 downwards up to the first running total; an unpublished
+    // word ends the round and is requested again
+    if (owner) {
+        int before = 0;
+        if (tile > 0) {
+            int polls = 0;
+            for (;;) {
+                bool done = false;
+                int used = 0;
+#pragma unroll
+                for (int k = 0; k < RS_WINDOW; ++k) {
+                    if (done || used < k || t - k < 0 || (unsigned)(v[k] >> 33) != epoch) continue;
+                    before += (int)(unsigned)v[k];
+                    used = k + 1;
+                    done = ((v[k] >> 32) & 1ull) != 0;
+                }
+                if (done) break;
+                t -= used;
+                if (t < 0) break;   // not reached: tile 0 publishes a running total
+                if (used == 0) {
+                    if (++polls >= RS_SPIN_LIMIT) {
+                        atomicOr(&bp->sort_stall, 1);
+                        atomicOr(&bp->overflow, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+#pragma unroll
+                for (int k = 0; k < RS_WINDOW; ++k)
+                    v[k] = t - k >= 0 ? rs_status_load(status + (size_t)(t - k) * 256 + d) : 0ull;
+            }
+            __hip_atomic_store(mine, rs_word(epoch, true, before + cnt_d), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        lback[d] = before;
+    }
+    __syncthreads();
+    if (owner) {   // per-wave counts -> each wave's offset inside the digit's run
+        int o = 0;
+#pragma unroll
+        for (int k = 0; k < RS_WAVES; ++k) {
+            const int c = wcnt[k][d];
+            wcnt[k][d] = (unsigned short)o;
+            o += c;
+        }
+    }
+    if (w == 0) {
+        // digit runs inside the tile and in the output: exclusive scans over the
+        // 256 digit values, four per lane
+        int c[4], g[4], sc = 0, sg = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] = tcnt[4 * lane + j];
+            g[j] = ctl->hist[pass][4 * lane + j];
+            sc += c[j];
+            sg += g[j];
+        }
+        int oc = wave_incl_scan(sc) - sc, og = wave_incl_scan(sg) - sg;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            tstart[4 * lane + j] = oc;
+            binpos[4 * lane + j] = og + lback[4 * lane + j];
+            oc += c[j];
+            og += g[j];
+        }
+    }
+    __syncthreads();
+    // stage the tile in digit order, then each digit run leaves with
+    // consecutive lanes
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        if (base + r * 64 + lane < n) {
+            const int dg = (int)((kk[r] >> shift) & 255);
+            const int lp = tstart[dg] + (int)wcnt[w][dg] + rk[r];
+            skey[lp] = kk[r];
+            sval[lp] = vv[r];
+        }
+    }
+    __syncthreads();
+    const int cnt = min(RS_TILE, n - tile * RS_TILE);
+    for (int j = tid; j < cnt; j += RS_TPB) {
+        const uint64_t k = skey[j];
+        const int dg = (int)((k >> shift) & 255);
+        const int pos = binpos[dg] + (j - tstart[dg]);
+        kout[pos] = k;
+        vout[pos] = sval[j];
+    }
+}
+
 // ======================================================= collision runs
 __global__ __launch_bounds__(CSG_TPB) void k_runs_mark(const uint64_t *__restrict__ key,
                                                        BuildParams *bp, int *__restrict__ runs) {
@@ -539,8 +806,8 @@ __global__ __launch_bounds__(CSG_TPB) void k_runs_mark(const uint64_t *__restric
     }
 }
 
-// One block per run (grid-stride): levels 21..63 for each member, then members
-// are re-ranked by (levels 21..63, body index); ties keep index order.
+// One block per run (grid-stride): levels CSG_KEY_LEVELS..63 for each member,
+// then members are re-ranked by (those levels, body index); ties keep index order.
 __global__ __launch_bounds__(CSG_TPB) void k_runs_fix(
     const uint64_t *__restrict__ key, int *__restrict__ idx, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, const BuildParams *__restrict__ bp,
