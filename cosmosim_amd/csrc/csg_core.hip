@@ -40,6 +40,7 @@ struct csg_ctx {
     signed char *lcp = nullptr;
     int *cnt = nullptr, *nbase = nullptr, *scan_part = nullptr;
     DD *pref = nullptr, *dd_part = nullptr;
+    DD *tpref = nullptr;   // build: each thread's exclusive prefix inside its tile
     WNode *nodes = nullptr;
     WNode32 *nodes32 = nullptr;
     int *icount = nullptr;
@@ -145,7 +146,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
     dfree(c->depth_part);
     dfree(c->runs); dfree(c->lcp); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
-    dfree(c->pref); dfree(c->dd_part); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
+    dfree(c->pref); dfree(c->dd_part); dfree(c->tpref); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
     dfree(c->wcost); dfree(c->worder); dfree(c->xyzm); dfree(c->owner);
     c->wcost_n = 0;
     c->cap = 0;
@@ -182,7 +183,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->ms, C) && dalloc(c->depth_part, (size_t)blocks_for((long long)C1, CSG_TPB)) && dalloc(c->runs, C) &&
               dalloc(c->lcp, C1) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
-              dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
+              dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->tpref, (size_t)4 * CSG_TPB * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
               dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, C / 64 + 1) &&
               dalloc(c->xyzm, C) && dalloc(c->owner, 8 * C + 4096) && dalloc(c->rs_ctl, 1) &&
               dalloc(c->rs_status, (size_t)256 * blocks_for((long long)C, RS_TILE));
@@ -588,19 +589,16 @@ static int launch_build(csg_ctx *c, bool with64) {
         CSG_LAUNCHED();
     }
     const int n1 = n + 1;
-    if (n > 0)
-        k_gather_sorted4<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, c->xyzm, c->bp, c->xs,
-                                                                    c->ys, c->zs, c->ms);
     const int nlb = blocks_for(n1, CSG_TPB);
     k_lcp<<<nlb, CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt, c->depth_part);
     const int nt = blocks_for(n1, CSG_TILE);
     k_iscan_reduce<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->bp, c->scan_part);
     k_iscan_top<<<1, CSG_TPB, 0, st>>>(c->scan_part, nt, c->depth_part, nlb, c->bp);
     k_iscan_down<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->scan_part, c->bp, c->nbase);
-    k_dd_reduce<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part);
-    k_dd_top<<<1, CSG_TPB, 0, st>>>(c->dd_part, nt);
-    k_dd_down<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part, c->pref,
-                                      c->cap + 1);
+    k_gather_dd<<<nt, CSG_TPB, 0, st>>>(c->idx, c->xyzm, c->bp, c->xs, c->ys, c->zs, c->ms,
+                                        c->dd_part, c->tpref);
+    k_dd_down2<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part, c->tpref,
+                                       c->pref);
     const double inv_th2 = c->theta_hint > 0.0 ? 1.0 / (c->theta_hint * c->theta_hint) : HUGE_VAL;
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
         c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->xs, c->ys, c->zs, c->ms,

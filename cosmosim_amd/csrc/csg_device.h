@@ -1116,6 +1116,148 @@ __global__ __launch_bounds__(CSG_TPB) void k_dd_down(const double *__restrict__ 
     }
 }
 
+// ------------------------------------------- fused gather + tile sums, down
+// The build's form of the three kernels above.  k_gather_dd writes the sorted
+// copies of (x, y, z, m) from the interleaved body-order copy and forms, from
+// the same registers, the double-double sums of each thread's 8 consecutive
+// sorted bodies, each thread's exclusive prefix inside its tile (tpref) and the
+// tile total (part).  k_dd_down2 then needs no scan of its own: each block adds
+// the totals of the tiles before it (a fixed-order reduction, so the result is
+// reproducible) and runs the per-body prefixes from tpref.  Global accesses go
+// through LDS so that consecutive lanes touch consecutive bodies; the LDS index
+// is padded by one slot per 8 elements, so the per-thread accesses (stride 8)
+// fall in distinct banks.
+#define DD_PAD(j) ((j) + ((j) >> 3))
+#define DD_TILE_LDS (CSG_TILE + CSG_TILE / 8)
+
+__device__ __forceinline__ void dd_elem4(double4 q, bool live, DD v[4]) {
+    if (live) {
+        v[0] = DD{q.w, 0.0};
+        v[1] = dd_prod(q.w, q.x);
+        v[2] = dd_prod(q.w, q.y);
+        v[3] = dd_prod(q.w, q.z);
+    } else {
+        for (int c = 0; c < 4; ++c) v[c] = DD{0.0, 0.0};
+    }
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_gather_dd(
+    const int *__restrict__ idx, const double4 *__restrict__ xyzm,
+    const BuildParams *__restrict__ bp, double *__restrict__ xs, double *__restrict__ ys,
+    double *__restrict__ zs, double *__restrict__ ms, DD *__restrict__ part,
+    DD *__restrict__ tpref) {
+    __shared__ double sbuf[DD_TILE_LDS];
+    __shared__ int sidx[DD_TILE_LDS];
+    __shared__ DD wtot[4][4];
+    __shared__ DD prev[CSG_TPB][4];
+    const int nl = bp->n_live;
+    const int t = threadIdx.x;
+    const int tile0 = blockIdx.x * CSG_TILE;
+#pragma unroll
+    for (int k = 0; k < CSG_ITEMS; ++k) {
+        const int j = t + k * CSG_TPB;
+        sidx[DD_PAD(j)] = tile0 + j < nl ? idx[tile0 + j] : 0;
+    }
+    __syncthreads();
+    const int e0 = tile0 + t * CSG_ITEMS;
+    double4 q[CSG_ITEMS];
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r)
+        q[r] = e0 + r < nl ? xyzm[sidx[DD_PAD(t * CSG_ITEMS + r)]]
+                           : make_double4(0.0, 0.0, 0.0, 0.0);
+    DD s[4];
+    for (int c = 0; c < 4; ++c) s[c] = DD{0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        DD e[4];
+        dd_elem4(q[r], e0 + r < nl, e);
+        for (int c = 0; c < 4; ++c) s[c] = dd_add(s[c], e[c]);
+    }
+    // sorted copies, one coordinate at a time through LDS
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int r = 0; r < CSG_ITEMS; ++r)
+            sbuf[DD_PAD(t * CSG_ITEMS + r)] = a == 0 ? q[r].x : a == 1 ? q[r].y : a == 2 ? q[r].z : q[r].w;
+        __syncthreads();
+        double *const dst = a == 0 ? xs : a == 1 ? ys : a == 2 ? zs : ms;
+#pragma unroll
+        for (int k = 0; k < CSG_ITEMS; ++k) {
+            const int j = t + k * CSG_TPB;
+            if (tile0 + j < nl) dst[tile0 + j] = sbuf[DD_PAD(j)];
+        }
+        __syncthreads();
+    }
+    block_dd_incl_scan(s, wtot);
+    for (int c = 0; c < 4; ++c) prev[t][c] = s[c];
+    __syncthreads();
+    DD *const tp = tpref + ((size_t)blockIdx.x * CSG_TPB + t) * 4;
+    for (int c = 0; c < 4; ++c) tp[c] = t > 0 ? prev[t - 1][c] : DD{0.0, 0.0};
+    if (t == CSG_TPB - 1)
+        for (int c = 0; c < 4; ++c) part[blockIdx.x * 4 + c] = s[c];
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_dd_down2(
+    const double *__restrict__ xs, const double *__restrict__ ys, const double *__restrict__ zs,
+    const double *__restrict__ ms, const BuildParams *__restrict__ bp,
+    const DD *__restrict__ part, const DD *__restrict__ tpref, DD *__restrict__ pref) {
+    __shared__ double sbuf[DD_TILE_LDS];
+    __shared__ DD red[CSG_TPB][4];
+    const int nl = bp->n_live;
+    const int t = threadIdx.x;
+    const int tile0 = blockIdx.x * CSG_TILE;
+    // every global load first (one round trip): this thread's tile prefix, the
+    // coordinates (coalesced), the totals of the tiles before this one
+    const DD *const tp = tpref + ((size_t)blockIdx.x * CSG_TPB + t) * 4;
+    DD run[4];
+    for (int c = 0; c < 4; ++c) run[c] = tp[c];
+    double g[4][CSG_ITEMS];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const double *const src = a == 0 ? xs : a == 1 ? ys : a == 2 ? zs : ms;
+#pragma unroll
+        for (int k = 0; k < CSG_ITEMS; ++k) {
+            const int j = t + k * CSG_TPB;
+            g[a][k] = tile0 + j < nl ? src[tile0 + j] : 0.0;
+        }
+    }
+    DD acc[4];
+    for (int c = 0; c < 4; ++c) acc[c] = DD{0.0, 0.0};
+    for (int k = t; k < (int)blockIdx.x; k += CSG_TPB)
+        for (int c = 0; c < 4; ++c) acc[c] = dd_add(acc[c], part[k * 4 + c]);
+    for (int c = 0; c < 4; ++c) red[t][c] = acc[c];
+    // this thread's 8 consecutive bodies, one coordinate at a time through LDS
+    double v[4][CSG_ITEMS];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int k = 0; k < CSG_ITEMS; ++k) sbuf[DD_PAD(t + k * CSG_TPB)] = g[a][k];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < CSG_ITEMS; ++r) v[a][r] = sbuf[DD_PAD(t * CSG_ITEMS + r)];
+        __syncthreads();
+    }
+    // tile offset: fixed-order tree over the strided partial sums
+    for (int st = CSG_TPB / 2; st > 0; st >>= 1) {
+        if (t < st)
+            for (int c = 0; c < 4; ++c) red[t][c] = dd_add(red[t][c], red[t + st][c]);
+        __syncthreads();
+    }
+    for (int c = 0; c < 4; ++c) run[c] = dd_add(red[0][c], run[c]);
+    const int e0 = tile0 + t * CSG_ITEMS;
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        const int b = e0 + r;
+        if (b <= nl)
+            for (int c = 0; c < 4; ++c) pref[(size_t)b * 4 + c] = run[c];
+        if (b < nl) {
+            DD e[4];
+            dd_elem4(make_double4(v[0][r], v[1][r], v[2][r], v[3][r]), true, e);
+            for (int c = 0; c < 4; ++c) run[c] = dd_add(run[c], e[c]);
+        }
+    }
+}
+
 __device__ __forceinline__ double dd_range(const DD *__restrict__ pref, int cap1, int c, int b,
                                            int e) {
     (void)cap1;

@@ -198,6 +198,32 @@ def test_collision_runs_and_dead_bodies(seed):
         assert np.all(a[b["mass"] <= 0] == 0.0)
 
 
+def test_pairs_sharing_12_to_45_levels():
+    """Bodies that share more levels than the sorted key holds are ordered and
+    split by their deeper levels; pairs and triples at separations 2^-k of the
+    root cube, k = 12..45, cover every collision word and its boundaries."""
+    rng = np.random.default_rng(7)
+    base = rng.uniform(-1.0, 1.0, size=(400, 3))
+    base[0], base[1] = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]   # root cube of half size ~1
+    pts = [base]
+    for k in range(12, 46):
+        for _ in range(3):
+            p = rng.uniform(-0.9, 0.9, size=3)
+            d = rng.normal(size=3)
+            d *= 2.0 ** -k / np.linalg.norm(d)
+            pts.append(np.stack([p, p + d, p - 0.5 * d]))
+    pos = np.concatenate(pts)
+    b = ca.make_bodies(pos, vel=np.zeros_like(pos), mass=rng.uniform(0.5, 2.0, len(pos)))
+    sim = sim_for(b, 0.01)
+    assert sim.stats().collision_runs >= 1
+    pr = ref.build(b)
+    assert pools_same_structure(ref.canonical(sim.export_tree()), ref.canonical(pr))
+    bb = b.copy()
+    counts = ref.forces(pr, bb, 1.0, 0.01, 0.0)
+    a, cnt = forces_on(sim, 0.0, "fp64_strict")
+    assert np.array_equal(cnt, counts) and same_bits(a, acc(bb))
+
+
 def test_empty_and_all_dead():
     b = ca.make_bodies(np.zeros((3, 3)), mass=[0.0, 0.0, -2.0])
     sim = sim_for(b, 0.01)
