@@ -64,6 +64,10 @@ _SIGS = {
     "cosmosim_ext_direct_forces": (_I, [_VP, _VP, _I, _VP]),
     "cosmosim_ext_energy": (_I, [_VP, _VP, _VP, _VP]),
     "cosmosim_ext_find_neighbors": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),
+    "cosmosim_ext_sph_density": (_I, [_VP]),
+    "cosmosim_ext_sph_forces": (_I, [_VP]),
+    "cosmosim_ext_sph_cooling": (_I, [_VP, _D]),
+    "cosmosim_ext_step_sph": (_I, [_VP, _I]),
     "cosmosim_ext_interaction_counts": (_I, [_VP, _VP, _I]),
     "cosmosim_ext_get_stats": (_I, [_VP, ctypes.POINTER(ExtStats)]),
     "cosmosim_ext_set_timing": (_I, [_VP, _I]),
@@ -103,6 +107,10 @@ _SIGS = {
     "csg_direct_forces": (_I, [_VP, _VP, _I, _D, _D, _VP]),
     "csg_energy": (_I, [_VP, _D, _D, _VP, _VP, _VP]),
     "csg_find_neighbors": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),
+    "csg_sph_density": (_I, [_VP]),
+    "csg_sph_forces": (_I, [_VP]),
+    "csg_sph_cooling": (_I, [_VP, _D]),
+    "csg_substeps_sph": (_I, [_VP, _I, _D, _D, _D, _D, _I]),
     "csg_interaction_counts": (_I, [_VP, _VP, _I]),
     "csg_get_stats": (_I, [_VP, _VP]),
     "csg_set_timing": (_I, [_VP, _I]),
@@ -361,6 +369,23 @@ class Simulation(Handle):
         _check(lib().cosmosim_ext_find_neighbors(self.h, ptr(centres), ptr(rad), k, cap, ptr(out),
                                                  ptr(counts)), "find_neighbors")
         return [out[q, :counts[q]].copy() for q in range(k)]
+
+    # ------------------------------------------------------------ SPH
+    def sph_density(self):
+        """sph_compute_density on the device (src/sph.c:45-95)."""
+        _check(lib().cosmosim_ext_sph_density(self.h), "sph_density")
+
+    def sph_forces(self):
+        """sph_compute_forces on the device (src/sph.c:99-196): adds to a."""
+        _check(lib().cosmosim_ext_sph_forces(self.h), "sph_forces")
+
+    def sph_cooling(self, dt):
+        """sph_apply_cooling on the device (src/sph.c:200-231)."""
+        _check(lib().cosmosim_ext_sph_cooling(self.h, dt), "sph_cooling")
+
+    def advance_sph(self, substeps):
+        """Substeps of integrator_step with SPH enabled, device-resident."""
+        _check(lib().cosmosim_ext_step_sph(self.h, substeps), "cosmosim_ext_step_sph")
 
     def export_tree(self):
         cap = 8 * self.n + 4096

@@ -91,6 +91,7 @@ class ExtStats(ctypes.Structure):
         ("ms_kdk", ctypes.c_double),
         ("ms_neighbors", ctypes.c_double),
         ("neighbor_queries", ctypes.c_longlong),
+        ("ms_sph", ctypes.c_double),
     ]
 
 

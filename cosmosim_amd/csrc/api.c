@@ -502,6 +502,7 @@ COSMOSIM_API int cosmosim_ext_get_stats(SimHandle h, CosmosimExtStats *out)
     out->timed_walks = cs.timed_walks;
     out->ms_neighbors = cs.ms_neighbors;
     out->neighbor_queries = cs.neighbor_queries;
+    out->ms_sph = cs.ms_sph;
     return COSMOSIM_OK;
 }
 
@@ -533,6 +534,57 @@ COSMOSIM_API int cosmosim_ext_find_neighbors(SimHandle h, const double *centres,
         st = csg_build(s->ctx);
         if (!st)
             st = csg_find_neighbors(s->ctx, centres, radius, k, cap, out, counts);
+    }
+    return st;
+}
+
+/* A pass that needs the tree builds it when the positions have none. */
+static int with_tree(SimState *s, int (*pass)(csg_ctx *))
+{
+    int st = pass(s->ctx);
+    if (st == COSMOSIM_E_STATE) {
+        st = csg_build(s->ctx);
+        if (!st)
+            st = pass(s->ctx);
+    }
+    if (!st)
+        s->host_stale = 1;
+    return st;
+}
+
+COSMOSIM_API int cosmosim_ext_sph_density(SimHandle h)
+{
+    return h ? with_tree((SimState *)h, csg_sph_density) : COSMOSIM_E_ARG;
+}
+
+COSMOSIM_API int cosmosim_ext_sph_forces(SimHandle h)
+{
+    return h ? with_tree((SimState *)h, csg_sph_forces) : COSMOSIM_E_ARG;
+}
+
+COSMOSIM_API int cosmosim_ext_sph_cooling(SimHandle h, double dt)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    int st = csg_sph_cooling(s->ctx, dt);
+    if (!st)
+        s->host_stale = 1;
+    return st;
+}
+
+COSMOSIM_API int cosmosim_ext_step_sph(SimHandle h, int substeps)
+{
+    if (!h || substeps < 0)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    if (s->config.quasar)
+        return COSMOSIM_E_STATE;
+    int st = csg_substeps_sph(s->ctx, substeps, s->params.dt, s->params.G, s->params.softening,
+                              s->params.theta, s->params.precision);
+    if (st == COSMOSIM_OK) {
+        s->host_stale = 1;
+        s->sim_time += s->params.dt * substeps;
     }
     return st;
 }

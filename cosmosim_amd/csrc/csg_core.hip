@@ -12,7 +12,7 @@
 #include "csg.h"
 
 #define CSG_EV_MAX 4096
-enum { PH_BUILD = 0, PH_WALK = 1, PH_KDK = 2, PH_NB = 3 };
+enum { PH_BUILD = 0, PH_WALK = 1, PH_KDK = 2, PH_NB = 3, PH_SPH = 4 };
 
 struct csg_ctx {
     int device = 0;
@@ -58,7 +58,7 @@ struct csg_ctx {
     hipEvent_t *ev = nullptr;  // 2 * CSG_EV_MAX events, created on first use
     int ev_phase[CSG_EV_MAX];
     int ev_count = 0, ev_open = -1;
-    double ms_build = 0, ms_walk = 0, ms_kdk = 0, ms_nb = 0;
+    double ms_build = 0, ms_walk = 0, ms_kdk = 0, ms_nb = 0, ms_sph = 0;
     long long nb_queries = 0;
     int timed_walks = 0;
     // adjoint: walk tape, transposed records, per-body adjoint buffers, checkpoints
@@ -92,6 +92,9 @@ struct csg_ctx {
     double4 *geom = nullptr;  // neighbour queries: cell centre and half size per node
     size_t geom_cap = 0;
     int geom_valid = 0;       // geom describes the current tree
+    int *gq = nullptr;        // SPH: live gas bodies in tree order, tile offsets, count
+    size_t gq_cap = 0;
+    int gq_valid = 0;         // gq describes the current tree
     void *nbuf = nullptr;     // neighbour queries: scratch
     size_t nbuf_cap = 0;
     int *wcost = nullptr;   // FP32 wave-mask walk: node steps of each wave in the last walk
@@ -122,6 +125,7 @@ struct csg_ctx {
 static void adj_free(csg_ctx *c);  // csg_adjoint.inl
 static void frozen_free(csg_ctx *c);
 static void nb_free(csg_ctx *c);   // csg_neighbors.inl
+static void sph_free(csg_ctx *c);  // csg_sph.inl
 
 static inline int blocks_for(long long n, int per) { return (int)((n + per - 1) / per); }
 static inline int at_least1(int v) { return v < 1 ? 1 : v; }
@@ -253,6 +257,7 @@ static void collect_timers(csg_ctx *c) {
         if (c->ev_phase[k] == PH_BUILD) c->ms_build += ms;
         else if (c->ev_phase[k] == PH_WALK) { c->ms_walk += ms; c->timed_walks += 1; }
         else if (c->ev_phase[k] == PH_NB) c->ms_nb += ms;
+        else if (c->ev_phase[k] == PH_SPH) c->ms_sph += ms;
         else c->ms_kdk += ms;
     }
     c->ev_count = 0;
@@ -354,6 +359,7 @@ void csg_destroy(csg_ctx *c) {
     adj_free(c);
     frozen_free(c);
     nb_free(c);
+    sph_free(c);
     dfree(c->bp);
     dfree(c->itotal);
     dfree(c->err_dev);
@@ -609,6 +615,7 @@ static int launch_build(csg_ctx *c, bool with64) {
     CSG_LAUNCHED();
     c->tree_valid = 1;
     c->geom_valid = 0;
+    c->gq_valid = 0;
     c->nodes64_valid = with64 ? 1 : 0;
     c->th32 = c->theta_hint;
     c->tape_valid = 0;   // a tape describes the tree it was recorded on
@@ -973,6 +980,7 @@ int csg_get_stats(csg_ctx *c, csg_stats *o) {
     o->ms_kdk = c->ms_kdk;
     o->timed_walks = c->timed_walks;
     o->ms_neighbors = c->ms_nb;
+    o->ms_sph = c->ms_sph;
     o->neighbor_queries = c->nb_queries;
     return COSMOSIM_OK;
 }
@@ -988,7 +996,7 @@ int csg_set_timing(csg_ctx *c, int enabled) {
     c->timing = enabled ? 1 : 0;
     c->ev_count = 0;
     c->ev_open = -1;
-    c->ms_build = c->ms_walk = c->ms_kdk = c->ms_nb = 0.0;
+    c->ms_build = c->ms_walk = c->ms_kdk = c->ms_nb = c->ms_sph = 0.0;
     c->timed_walks = 0;
     c->nb_queries = 0;
     return COSMOSIM_OK;
@@ -1019,3 +1027,4 @@ void csg_host_free(void *p) {
 
 #include "csg_adjoint.inl"
 #include "csg_neighbors.inl"
+#include "csg_sph.inl"

@@ -15,12 +15,6 @@
 #include "integrator.h"
 #include "octree.h"
 
-/* SPH hooks live in the caller's sph.c (src/sph.h:13-20).  Weak references: a
- * program that does not link sph.c can still use every non-SPH path. */
-extern void sph_compute_density(Body *bodies, int n, const OctreeNode *tree) __attribute__((weak));
-extern void sph_compute_forces(Body *bodies, int n, const OctreeNode *tree) __attribute__((weak));
-extern void sph_apply_cooling(Body *bodies, int n, double dt) __attribute__((weak));
-
 /* ------------------------------------------------------------ errors, ctx */
 static _Thread_local int tl_error = 0;
 static _Thread_local csg_ctx *tl_ctx = NULL;
@@ -185,23 +179,23 @@ int cs_legacy_step(csg_ctx *ctx, Body *bodies, int n, double dt, double G, doubl
     if (pool && (st = csg_export_pool(ctx, pool, pool_cap, &used)))
         return st;
     if (sph_enabled) {
-        if (!sph_compute_density || !sph_compute_forces || !sph_apply_cooling)
-            return COSMOSIM_E_UNSUPPORTED;
-        /* host SPH adds to a (src/integrator.c:39-42) between the walk and the
-         * second half kick */
-        if ((st = cs_download_kinematics(ctx, bodies, n, 1, 1, 1)))
+        /* SPH on the device between the walk and the second half kick
+         * (src/integrator.c:39-42), cooling after it (:54-56); the download
+         * returns the gas fields with the kinematics */
+        if ((st = csg_sph_density(ctx)) || (st = csg_sph_forces(ctx)))
             return st;
-        sph_compute_density(bodies, n, pool);
-        sph_compute_forces(bodies, n, pool);
-        if ((st = csg_set_accelerations(ctx, bodies, n)))
+        if ((st = csg_kick(ctx, dt)) || (st = csg_sph_cooling(ctx, dt)))
             return st;
+        if ((st = csg_get_bodies(ctx, bodies, n)))
+            return st;
+        if (pool)
+            cs_note_tree(pool, used, bodies, n);
+        return COSMOSIM_OK;
     }
     if ((st = csg_kick(ctx, dt)))
         return st;
     if ((st = cs_download_kinematics(ctx, bodies, n, 1, 1, 1)))
         return st;
-    if (sph_enabled)
-        sph_apply_cooling(bodies, n, dt);
     if (pool)
         cs_note_tree(pool, used, bodies, n);
     return COSMOSIM_OK;

@@ -52,7 +52,7 @@ typedef struct {
     int n_live;            /* bodies with mass > 0 */
     int n_nodes;           /* nodes of the last tree (reference-layout count) */
     int max_depth;         /* deepest leaf of the last tree */
-    int collision_runs;    /* groups of bodies sharing a 21-level cell path in the last build */
+    int collision_runs;    /* groups of bodies sharing all 16 levels of the sorted key in the last build */
     int timed_walks;       /* walk launches covered by ms_walk */
     long long interactions;/* body-node interactions of the last walk */
     double root_center[3]; /* root cube of the last build */
@@ -62,6 +62,7 @@ typedef struct {
     double ms_kdk;
     double ms_neighbors;   /* GPU time of neighbour-query traversals since timing was enabled */
     long long neighbor_queries; /* queries covered by ms_neighbors */
+    double ms_sph;         /* GPU time of SPH density + force passes since timing was enabled */
 } CosmosimExtStats;
 
 /* Seeded initial conditions (own generator; the reference's generators are
@@ -123,6 +124,19 @@ int cosmosim_ext_export_tree(SimHandle h, OctreeNode *pool, int capacity, int *p
  * positions have none yet. */
 int cosmosim_ext_find_neighbors(SimHandle h, const double *centres, const double *radius, int k,
                                 int cap, int *out, int *counts);
+/* SPH of the handle's gas bodies on the device (replace sph_compute_density,
+ * sph_compute_forces and sph_apply_cooling, src/sph.h:11-20): density, pressure
+ * and the adaptive smoothing length; pressure-gradient plus viscosity
+ * accelerations added to a; implicit cooling of the internal energy.  Results
+ * equal src/sph.c's bit for bit on the same bodies and tree.  The density and
+ * force passes build the tree if the handle's positions have none yet. */
+int cosmosim_ext_sph_density(SimHandle h);
+int cosmosim_ext_sph_forces(SimHandle h);
+int cosmosim_ext_sph_cooling(SimHandle h, double dt);
+/* Device-resident substeps of integrator_step with SPH enabled
+ * (src/integrator.c:12-57): kick + drift, build, gravity, SPH density and
+ * forces, kick, cooling. */
+int cosmosim_ext_step_sph(SimHandle h, int substeps);
 /* Last error recorded on the handle by a void legacy call (cosmosim_step). */
 int cosmosim_ext_handle_error(SimHandle h);
 /* Last error recorded on the calling thread by octree_* / integrator_*. */

@@ -32,6 +32,7 @@ typedef struct {
     int pad;
     double ms_neighbors;            /* GPU time of neighbour-query traversals (timing on) */
     long long neighbor_queries;     /* queries covered by ms_neighbors */
+    double ms_sph;                  /* GPU time of SPH density + force passes (timing on) */
 } csg_stats;
 
 int  csg_device_count(int *count);
@@ -73,6 +74,18 @@ int  csg_export_pool(csg_ctx *ctx, OctreeNode *pool, int capacity, int *pool_siz
  * reported bodies of query q in the reference's visiting order. */
 int  csg_find_neighbors(csg_ctx *ctx, const double *centres, const double *radius, int k,
                         int cap, int *out, int *counts);
+/* SPH of the gas bodies on the device state and the current tree: the GPU forms
+ * of sph_compute_density, sph_compute_forces and sph_apply_cooling
+ * (src/sph.h:11-20), bit-identical to src/sph.c on the same bodies and tree.
+ * Type, smoothing length, density, pressure and internal energy live in the
+ * device body records; downloads (csg_get_bodies) return them. */
+int  csg_sph_density(csg_ctx *ctx);
+int  csg_sph_forces(csg_ctx *ctx);
+int  csg_sph_cooling(csg_ctx *ctx, double dt);
+/* k device-resident substeps of integrator_step with SPH enabled
+ * (src/integrator.c:12-57). */
+int  csg_substeps_sph(csg_ctx *ctx, int k, double dt, double G, double softening, double theta,
+                      int precision);
 /* Walk a caller-supplied reference-layout pool from `root` (exact reference
  * traversal, FP64 strict arithmetic). */
 int  csg_walk_pool(csg_ctx *ctx, const OctreeNode *pool, int pool_size, int root,

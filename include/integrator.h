@@ -5,10 +5,10 @@
  *   live bodies (mass > 0):  v += (0.5*a)*dt;  x += v*dt;
  *   rebuild tree; recompute a;  [SPH hooks when sph_enabled];
  *   live bodies:             v += (0.5*a)*dt;  [SPH cooling when sph_enabled]
- * The kicks, drift, build and walk run on the GPU.  The SPH hooks
- * (sph_compute_density, sph_compute_forces, sph_apply_cooling) are the
- * caller's host code and are called exactly where the reference calls them,
- * with the caller's pool holding the freshly built tree.
+ * The kicks, drift, build, walk and, with sph_enabled, the SPH passes
+ * (sph_compute_density, sph_compute_forces, sph_apply_cooling, src/sph.c) run
+ * on the GPU where the reference calls them; the caller's pool receives the
+ * freshly built tree and the body array every field, gas fields included.
  */
 #ifndef INTEGRATOR_H
 #define INTEGRATOR_H

@@ -8,6 +8,7 @@
  *   src/octree.c:5-234     (tree build by sequential insertion, monopole walk)
  *   src/integrator.c:4-57  (KDK leapfrog, without the SPH hooks)
  *   src/octree.c:236-318   (sphere neighbour query)
+ *   src/sph.c:10-231       (SPH density, forces and cooling over those queries)
  *   tests/test_physics.c:34-71 (direct-sum and energy judges)
  * Floating-point expressions are kept in the reference's evaluation order so
  * that, compiled with -O2 -ffp-contract=off on x86-64, results are bit-identical
@@ -385,6 +386,149 @@ int cref_find_neighbors(const OctreeNode *pool, int root, const Body *bodies, in
     int count = 0;
     cref_nb_visit(pool, root, bodies, qx, qy, qz, radius * radius, buf, &count, cap);
     return count;
+}
+
+/* ------------------------------------------------------------------ SPH
+ * Smoothed-particle hydrodynamics of the gas bodies (src/sph.c:10-231): the M4
+ * cubic-spline kernel (:12-41), density with the adaptive smoothing length
+ * (:45-95), pressure gradient plus Monaghan viscosity (:99-196) and implicit
+ * cooling (:200-231).  Every sum runs over the body's neighbour list (the
+ * sphere query above, radius h, at most 128 bodies, in visiting order) with
+ * the reference's expression order, so results are bit-identical.  The loops
+ * are sequential: no body's result depends on another's within one pass. */
+#define CREF_SPH_GAMMA (5.0 / 3.0)     /* src/sph.h:7 */
+#define CREF_SPH_CAP 128               /* src/sph.h:9 */
+#define CREF_PI 3.14159265358979323846
+
+static double cref_spline(double r, double h)          /* src/sph.c:12-25 */
+{
+    if (h < 1e-10)
+        return 0.0;
+    double q = r / h;
+    if (q > 1.0)
+        return 0.0;
+    double norm = 8.0 / (CREF_PI * h * h * h);
+    if (q > 0.5) {
+        double t = 1.0 - q;
+        return norm * 2.0 * t * t * t;
+    }
+    return norm * (1.0 - 6.0 * q * q + 6.0 * q * q * q);
+}
+
+static double cref_spline_dr(double r, double h)       /* src/sph.c:27-41 */
+{
+    if (h < 1e-10)
+        return 0.0;
+    double q = r / h;
+    if (q > 1.0 || r < 1e-15)
+        return 0.0;
+    double norm = 8.0 / (CREF_PI * h * h * h);
+    if (q > 0.5) {
+        double t = 1.0 - q;
+        return norm * (-6.0 * t * t) / h;
+    }
+    return norm * (-12.0 * q + 18.0 * q * q) / h;
+}
+
+static inline int cref_gas(const Body *b) { return b->type == BODY_GAS && !(b->mass <= 0.0); }
+
+void cref_sph_density(Body *bodies, int n, const OctreeNode *pool)
+{
+    int list[CREF_SPH_CAP];
+    for (int i = 0; i < n; ++i) {
+        Body *bi = &bodies[i];
+        if (!cref_gas(bi))
+            continue;
+        const double h = bi->smoothing_h;
+        const int k = cref_find_neighbors(pool, 0, bodies, n, bi->x, bi->y, bi->z, h, list,
+                                          CREF_SPH_CAP);
+        if (k < 20 && h < 50.0)                        /* :71-75 */
+            bi->smoothing_h = h * 1.2;
+        else if (k > 48 && h > 0.1)
+            bi->smoothing_h = h * 0.8;
+        double rho = 0.0;
+        for (int a = 0; a < k; ++a) {
+            const Body *bj = &bodies[list[a]];
+            if (bj->mass <= 0.0)
+                continue;
+            double dx = bi->x - bj->x, dy = bi->y - bj->y, dz = bi->z - bj->z;
+            rho += bj->mass * cref_spline(sqrt(dx * dx + dy * dy + dz * dz), h);
+        }
+        if (rho < 1e-15 || isnan(rho))
+            rho = 1e-15;
+        bi->density = rho;
+        double u = bi->internal_energy;
+        if (u < 1e-15)
+            u = 1e-15;
+        bi->pressure = (CREF_SPH_GAMMA - 1.0) * rho * u;
+    }
+}
+
+void cref_sph_forces(Body *bodies, int n, const OctreeNode *pool)
+{
+    int list[CREF_SPH_CAP];
+    for (int i = 0; i < n; ++i) {
+        Body *bi = &bodies[i];
+        if (!cref_gas(bi))
+            continue;
+        const double hi = bi->smoothing_h, rhoi = bi->density, Pi = bi->pressure;
+        const double ci = sqrt(CREF_SPH_GAMMA * Pi / (rhoi + 1e-15));
+        const int k = cref_find_neighbors(pool, 0, bodies, n, bi->x, bi->y, bi->z, hi, list,
+                                          CREF_SPH_CAP);
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int a = 0; a < k; ++a) {
+            const int j = list[a];
+            const Body *bj = &bodies[j];
+            if (j == i || bj->mass <= 0.0 || bj->type != BODY_GAS)
+                continue;
+            double dx = bi->x - bj->x, dy = bi->y - bj->y, dz = bi->z - bj->z;
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (r < 1e-15)
+                continue;
+            double hbar = 0.5 * (hi + bj->smoothing_h);
+            double rhoj = bj->density, Pj = bj->pressure;
+            double vr = (bi->vx - bj->vx) * dx + (bi->vy - bj->vy) * dy + (bi->vz - bj->vz) * dz;
+            double visc = 0.0;
+            if (vr < 0.0) {                            /* approaching: :152-161 */
+                double mu = hbar * vr / (r * r + 0.01 * hbar * hbar);
+                double cj = sqrt(CREF_SPH_GAMMA * Pj / (rhoj + 1e-15));
+                double cbar = 0.5 * (ci + cj), rhobar = 0.5 * (rhoi + rhoj);
+                visc = (-1.0 * cbar * mu + 2.0 * mu * mu) / (rhobar + 1e-15);
+            }
+            double g = cref_spline_dr(r, hbar);
+            double gx = g * dx / r, gy = g * dy / r, gz = g * dz / r;
+            double f = bj->mass * (Pi / (rhoi * rhoi + 1e-15) + Pj / (rhoj * rhoj + 1e-15) + visc);
+            fx -= f * gx;
+            fy -= f * gy;
+            fz -= f * gz;
+        }
+        if (isnan(fx))
+            fx = 0.0;
+        if (isnan(fy))
+            fy = 0.0;
+        if (isnan(fz))
+            fz = 0.0;
+        bi->ax += fx * 0.1;                            /* :186-190 */
+        bi->ay += fy * 0.1;
+        bi->az += fz * 0.1;
+    }
+}
+
+void cref_sph_cooling(Body *bodies, int n, double dt)
+{
+    for (int i = 0; i < n; ++i) {
+        Body *b = &bodies[i];
+        if (!cref_gas(b))
+            continue;
+        double u = b->internal_energy, rho = b->density;
+        if (u < 1e-15 || rho < 1e-15)
+            continue;
+        double unew = u / (1.0 + dt * 0.05 * rho * 0.5 / sqrt(u));   /* :219-220 */
+        if (unew < 0.01)
+            unew = 0.01;
+        b->internal_energy = unew;
+        b->pressure = (CREF_SPH_GAMMA - 1.0) * rho * unew;
+    }
 }
 
 /* Reference-layout pools differ between builders only in node numbering when

@@ -70,6 +70,16 @@ class _Base:
                      _D(radius), _p(buf), cap)
         return buf[:k].copy()
 
+    # SPH passes over a body array (in place) with the tree `pool` (root 0)
+    def sph_density(self, bodies, pool):
+        self._sph_density(_p(bodies), len(bodies), _p(pool))
+
+    def sph_forces(self, bodies, pool):
+        self._sph_forces(_p(bodies), len(bodies), _p(pool))
+
+    def sph_cooling(self, bodies, dt):
+        self._sph_cooling(_p(bodies), len(bodies), _D(dt))
+
 
 class CpuRef(_Base):
     kind = "port"
@@ -91,9 +101,15 @@ class CpuRef(_Base):
         L.cref_find_neighbors.restype = _I
         L.cref_canonical.argtypes = [_VP, _I, _VP, _VP, _I]
         L.cref_canonical.restype = _I
+        for f in ("cref_sph_density", "cref_sph_forces"):
+            getattr(L, f).argtypes = [_VP, _I, _VP]
+        L.cref_sph_cooling.argtypes = [_VP, _I, _D]
         self._build = L.cref_build
         self._init = L.cref_init_acc
         self._nb = L.cref_find_neighbors
+        self._sph_density = L.cref_sph_density
+        self._sph_forces = L.cref_sph_forces
+        self._sph_cooling = L.cref_sph_cooling
 
     def forces(self, pool, bodies, G, softening, theta, root=0):
         """Zero a, walk every live body; returns per-body interaction counts."""
@@ -150,9 +166,15 @@ class RefBuild(_Base):
         L.integrator_step.argtypes = [_VP, _I, _D, _D, _D, _D, _VP, _I]
         L.octree_find_neighbors.argtypes = [_VP, _I, _VP, _I, _D, _D, _D, _D, _VP, _I]
         L.octree_find_neighbors.restype = _I
+        for f in ("sph_compute_density", "sph_compute_forces"):
+            getattr(L, f).argtypes = [_VP, _I, _VP]
+        L.sph_apply_cooling.argtypes = [_VP, _I, _D]
         self._build = L.octree_build
         self._init = L.integrator_init_accelerations
         self._nb = L.octree_find_neighbors
+        self._sph_density = L.sph_compute_density
+        self._sph_forces = L.sph_compute_forces
+        self._sph_cooling = L.sph_apply_cooling
 
     def forces(self, pool, bodies, G, softening, theta, root=0):
         self.lib.octree_compute_forces(_p(pool), root, _p(bodies), len(bodies), _D(G),

@@ -84,3 +84,23 @@ def energy_numpy(pos, vel, mass, G, eps):
         r = np.sqrt(np.einsum("ij,ij->i", d, d) + eps * eps)
         pe -= G * mass[i] * np.sum(mass[i + 1:] / r)
     return ke, pe
+
+
+BODY_FLOAT_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "mass", "spin_x", "spin_y",
+                     "spin_z", "accretion_rate", "luminosity", "lifetime", "density", "pressure",
+                     "internal_energy", "smoothing_h"]
+
+
+def bodies_identical(a, b):
+    """Every field of two body arrays bit-identical (padding ignored)."""
+    if len(a) != len(b) or not np.array_equal(a["type"], b["type"]):
+        return False
+    return all(same_bits(a[f], b[f]) for f in BODY_FLOAT_FIELDS)
+
+
+def first_body_difference(a, b):
+    for f in BODY_FLOAT_FIELDS:
+        d = np.nonzero(np.asarray(a[f]).view(np.uint64) != np.asarray(b[f]).view(np.uint64))[0]
+        if len(d):
+            return f, int(d[0]), float(a[f][d[0]]), float(b[f][d[0]])
+    return None
