@@ -172,6 +172,52 @@ def neighbor_runs(ca, bodies, radius=0.03, cap=64):
             "mean_neighbors": float(counts.mean()), "capped_fraction": float(np.mean(counts == cap))}
 
 
+def sph_runs(ca, bodies, cpu=True):
+    """SPH on the device tree of the 1M merger with every other star made gas:
+    smoothing lengths first settle over four density passes (each moves h by
+    x1.2 / x0.8 toward 20-48 neighbours), then one density and one force pass
+    are timed with the core's events.  The reference's own sph.c (oracle/_ref,
+    else the oracle's restatement) runs the same two passes on the host cores
+    from the same state, and its densities and accelerations are compared."""
+    b = bodies.copy()
+    gas = (np.arange(len(b)) % 2 == 0) & (b["type"] == 0)
+    b["type"] = np.where(gas, 1, b["type"])
+    b["smoothing_h"] = np.where(gas, 0.02, 0.0)
+    b["internal_energy"] = np.where(gas, 1.0, 0.0)
+    sim = ca.Simulation(b, softening=EPS, theta=0.5, dt=DT, precision="fp64")
+    for _ in range(4):
+        sim.sph_density()
+    start = sim.bodies()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.sph_density()
+    sim.sph_forces()
+    t1 = time.perf_counter()
+    st = sim.stats()
+    got = sim.bodies()
+    sim.close()
+    ngas = int(np.sum((got["type"] == 1) & (got["mass"] > 0)))
+    out = {"gas_bodies": ngas, "kernel_ms_density_forces": st.ms_sph,
+           "gas_bodies_per_s_kernel": ngas / (st.ms_sph * 1e-3),
+           "call_s_density_forces": t1 - t0}
+    if cpu:
+        from oracle.cpu_ref import CpuRef, RefBuild
+        impl = RefBuild() if RefBuild.available() else CpuRef()
+        cb = start.copy()
+        pool = impl.build(cb)
+        c0 = time.perf_counter()
+        impl.sph_density(cb, pool)
+        impl.sph_forces(cb, pool)
+        c1 = time.perf_counter()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        same = all(np.array_equal(cb[f].view(np.uint64), got[f].view(np.uint64))
+                   for f in ("density", "pressure", "smoothing_h", "ax", "ay", "az"))
+        out.update({"cpu_kind": impl.kind, "cpu_threads": threads if impl.kind == "reference" else 1,
+                    "cpu_s_density_forces": c1 - c0, "cpu_gas_bodies_per_s": ngas / (c1 - c0),
+                    "bit_identical_to_cpu": bool(same)})
+    return out
+
+
 def host_cpu():
     try:
         with open("/proc/cpuinfo") as f:
@@ -218,6 +264,7 @@ def main():
     ap.add_argument("--adjoint-bodies", type=int, default=4_000_000)
     ap.add_argument("--adjoint-steps", type=int, default=20)
     ap.add_argument("--no-neighbors", action="store_true")
+    ap.add_argument("--no-sph", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to scale the traffic counters")
     args = ap.parse_args()
@@ -323,6 +370,9 @@ def main():
     if not args.no_neighbors:
         line["neighbors"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
                              for k, v in neighbor_runs(ca, bodies).items()}
+    if not args.no_sph:
+        line["sph"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
+                       for k, v in sph_runs(ca, bodies, cpu=not args.no_cpu).items()}
     if not args.no_cpu:
         cb = cpu_baseline(ca, bodies, 0.5)
         line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v)
