@@ -68,6 +68,8 @@ _SIGS = {
     "cosmosim_ext_sph_forces": (_I, [_VP]),
     "cosmosim_ext_sph_cooling": (_I, [_VP, _D]),
     "cosmosim_ext_step_sph": (_I, [_VP, _I]),
+    "cosmosim_ext_snapshot_ue": (_I, [_VP, _VP, _I, _VP]),
+    "cosmosim_ext_snapshot_render": (_I, [_VP, _VP, _VP]),
     "cosmosim_ext_interaction_counts": (_I, [_VP, _VP, _I]),
     "cosmosim_ext_get_stats": (_I, [_VP, ctypes.POINTER(ExtStats)]),
     "cosmosim_ext_set_timing": (_I, [_VP, _I]),
@@ -111,6 +113,7 @@ _SIGS = {
     "csg_sph_forces": (_I, [_VP]),
     "csg_sph_cooling": (_I, [_VP, _D]),
     "csg_substeps_sph": (_I, [_VP, _I, _D, _D, _D, _D, _I]),
+    "csg_snapshot": (_I, [_VP, _I, _VP, _I, _VP]),
     "csg_interaction_counts": (_I, [_VP, _VP, _I]),
     "csg_get_stats": (_I, [_VP, _VP]),
     "csg_set_timing": (_I, [_VP, _I]),
@@ -386,6 +389,23 @@ class Simulation(Handle):
     def advance_sph(self, substeps):
         """Substeps of integrator_step with SPH enabled, device-resident."""
         _check(lib().cosmosim_ext_step_sph(self.h, substeps), "cosmosim_ext_step_sph")
+
+    # ------------------------------------------------------------ snapshots
+    def snapshot_ue(self, max_bodies=None):
+        """FCosmosimBodyGPU records of the live bodies: float32 [count, 16]."""
+        m = self.n if max_bodies is None else int(max_bodies)
+        out = np.zeros((max(m, 1), 16), dtype=np.float32)
+        k = ctypes.c_int(0)
+        _check(lib().cosmosim_ext_snapshot_ue(self.h, ptr(out), m, ctypes.byref(k)), "snapshot_ue")
+        return out[: k.value].copy()
+
+    def snapshot_render(self):
+        """The renderer's upload records of the live bodies: float32 [count, 10]."""
+        out = np.zeros((max(self.n, 1), 10), dtype=np.float32)
+        k = ctypes.c_int(0)
+        _check(lib().cosmosim_ext_snapshot_render(self.h, ptr(out), ctypes.byref(k)),
+               "snapshot_render")
+        return out[: k.value].copy()
 
     def export_tree(self):
         cap = 8 * self.n + 4096

@@ -589,6 +589,30 @@ COSMOSIM_API int cosmosim_ext_step_sph(SimHandle h, int substeps)
     return st;
 }
 
+static int snapshot(SimHandle h, int fmt, float *out, int max_bodies, int *count)
+{
+    if (!h || !count)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    if (s->config.quasar && !s->host_stale) {
+        /* quasar mode: the host bodies are the current state */
+        int st = csg_set_bodies(s->ctx, s->bodies, s->current_n);
+        if (st)
+            return st;
+    }
+    return csg_snapshot(s->ctx, fmt, out, max_bodies, count);
+}
+
+COSMOSIM_API int cosmosim_ext_snapshot_ue(SimHandle h, float *out, int max_bodies, int *count)
+{
+    return snapshot(h, 0, out, max_bodies, count);
+}
+
+COSMOSIM_API int cosmosim_ext_snapshot_render(SimHandle h, float *out, int *count)
+{
+    return snapshot(h, 1, out, h ? ((SimState *)h)->current_n : 0, count);
+}
+
 COSMOSIM_API int cosmosim_ext_handle_error(SimHandle h)
 {
     return h ? ((SimState *)h)->error : COSMOSIM_E_ARG;

@@ -95,6 +95,8 @@ struct csg_ctx {
     int *gq = nullptr;        // SPH: live gas bodies in tree order, tile offsets, count
     size_t gq_cap = 0;
     int gq_valid = 0;         // gq describes the current tree
+    void *snap = nullptr;     // snapshot records and scan scratch
+    size_t snap_cap = 0;
     void *nbuf = nullptr;     // neighbour queries: scratch
     size_t nbuf_cap = 0;
     int *wcost = nullptr;   // FP32 wave-mask walk: node steps of each wave in the last walk
@@ -126,6 +128,7 @@ static void adj_free(csg_ctx *c);  // csg_adjoint.inl
 static void frozen_free(csg_ctx *c);
 static void nb_free(csg_ctx *c);   // csg_neighbors.inl
 static void sph_free(csg_ctx *c);  // csg_sph.inl
+static void snap_free(csg_ctx *c); // csg_snapshot.inl
 
 static inline int blocks_for(long long n, int per) { return (int)((n + per - 1) / per); }
 static inline int at_least1(int v) { return v < 1 ? 1 : v; }
@@ -360,6 +363,7 @@ void csg_destroy(csg_ctx *c) {
     frozen_free(c);
     nb_free(c);
     sph_free(c);
+    snap_free(c);
     dfree(c->bp);
     dfree(c->itotal);
     dfree(c->err_dev);
@@ -1028,3 +1032,4 @@ void csg_host_free(void *p) {
 #include "csg_adjoint.inl"
 #include "csg_neighbors.inl"
 #include "csg_sph.inl"
+#include "csg_snapshot.inl"

@@ -137,6 +137,20 @@ int cosmosim_ext_sph_cooling(SimHandle h, double dt);
  * (src/integrator.c:12-57): kick + drift, build, gravity, SPH density and
  * forces, kick, cooling. */
 int cosmosim_ext_step_sph(SimHandle h, int substeps);
+/* Snapshots for the host consumers, built on the device from the device state
+ * and returned with one copy: the live bodies (mass > 0) in index order,
+ * converted to float.
+ *   _ue: the Unreal plugin's FCosmosimBodyGPU records, 16 floats each (pos,
+ *        mass, vel, type, internal energy, density, lifetime, luminosity, spin,
+ *        0), at most max_bodies of them - replaces the body loop of
+ *        FCosmosimPhysicsRunnable::SnapshotToBuffer (CosmosimSubsystem.cpp:12-43);
+ *   _render: the renderer's upload record, 10 floats each (pos, mass, vel,
+ *        type, internal energy, density) - replaces the upload loop of
+ *        src/renderer.c:348-364.
+ * *count receives the number of records.  In quasar mode the host bodies are
+ * current, so they are uploaded first. */
+int cosmosim_ext_snapshot_ue(SimHandle h, float *out, int max_bodies, int *count);
+int cosmosim_ext_snapshot_render(SimHandle h, float *out, int *count);
 /* Last error recorded on the handle by a void legacy call (cosmosim_step). */
 int cosmosim_ext_handle_error(SimHandle h);
 /* Last error recorded on the calling thread by octree_* / integrator_*. */

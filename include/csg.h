@@ -86,6 +86,11 @@ int  csg_sph_cooling(csg_ctx *ctx, double dt);
  * (src/integrator.c:12-57). */
 int  csg_substeps_sph(csg_ctx *ctx, int k, double dt, double G, double softening, double theta,
                       int precision);
+/* Host snapshot formats written on the device from the device state: the live
+ * bodies in index order as floats.  fmt 0: the Unreal plugin's FCosmosimBodyGPU
+ * (16 floats, at most max_bodies records); fmt 1: the renderer's upload record
+ * (10 floats).  *count = records copied to out. */
+int  csg_snapshot(csg_ctx *ctx, int fmt, float *out, int max_bodies, int *count);
 /* Walk a caller-supplied reference-layout pool from `root` (exact reference
  * traversal, FP64 strict arithmetic). */
 int  csg_walk_pool(csg_ctx *ctx, const OctreeNode *pool, int pool_size, int root,

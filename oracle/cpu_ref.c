@@ -9,6 +9,7 @@
  *   src/integrator.c:4-57  (KDK leapfrog, without the SPH hooks)
  *   src/octree.c:236-318   (sphere neighbour query)
  *   src/sph.c:10-231       (SPH density, forces and cooling over those queries)
+ *   CosmosimSubsystem.cpp:12-43, src/renderer.c:348-364 (float snapshots)
  *   tests/test_physics.c:34-71 (direct-sum and energy judges)
  * Floating-point expressions are kept in the reference's evaluation order so
  * that, compiled with -O2 -ffp-contract=off on x86-64, results are bit-identical
@@ -529,6 +530,36 @@ void cref_sph_cooling(Body *bodies, int n, double dt)
         b->internal_energy = unew;
         b->pressure = (CREF_SPH_GAMMA - 1.0) * rho * unew;
     }
+}
+
+/* ------------------------------------------------------------ snapshots
+ * The host consumers' float snapshots of the live bodies in index order:
+ * the Unreal plugin's FCosmosimBodyGPU, 16 floats, at most max_out records
+ * (ue/CosmosimPlugin/.../CosmosimSubsystem.cpp:12-43), and the renderer's
+ * upload record, 10 floats (src/renderer.c:348-364).  Returns the count. */
+int cref_snapshot(const Body *bodies, int n, int fmt, float *out, int max_out)
+{
+    int k = 0;
+    for (int i = 0; i < n && k < max_out; ++i) {
+        const Body *b = &bodies[i];
+        if (b->mass <= 0.0)
+            continue;
+        float *o = out + (size_t)k * (fmt == 0 ? 16 : 10);
+        o[0] = (float)b->x; o[1] = (float)b->y; o[2] = (float)b->z;
+        o[3] = (float)b->mass;
+        o[4] = (float)b->vx; o[5] = (float)b->vy; o[6] = (float)b->vz;
+        o[7] = (float)b->type;
+        o[8] = (float)b->internal_energy;
+        o[9] = (float)b->density;
+        if (fmt == 0) {
+            o[10] = (float)b->lifetime;
+            o[11] = (float)b->luminosity;
+            o[12] = (float)b->spin_x; o[13] = (float)b->spin_y; o[14] = (float)b->spin_z;
+            o[15] = 0.0f;
+        }
+        ++k;
+    }
+    return k;
 }
 
 /* Reference-layout pools differ between builders only in node numbering when

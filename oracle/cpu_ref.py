@@ -104,6 +104,8 @@ class CpuRef(_Base):
         for f in ("cref_sph_density", "cref_sph_forces"):
             getattr(L, f).argtypes = [_VP, _I, _VP]
         L.cref_sph_cooling.argtypes = [_VP, _I, _D]
+        L.cref_snapshot.argtypes = [_VP, _I, _I, _VP, _I]
+        L.cref_snapshot.restype = _I
         self._build = L.cref_build
         self._init = L.cref_init_acc
         self._nb = L.cref_find_neighbors
@@ -139,6 +141,13 @@ class CpuRef(_Base):
         out = np.zeros(2)
         self.lib.cref_energy(_p(bodies), len(bodies), _D(G), _D(softening), _p(out))
         return float(out[0]), float(out[1])
+
+    def snapshot(self, bodies, fmt, max_out=None):
+        """Float snapshot of the live bodies (fmt 0: 16 floats, 1: 10 floats)."""
+        m = len(bodies) if max_out is None else max_out
+        out = np.zeros((max(m, 1), 16 if fmt == 0 else 10), dtype=np.float32)
+        k = self.lib.cref_snapshot(_p(bodies), len(bodies), fmt, _p(out), m)
+        return out[:k].copy()
 
     def canonical(self, pool, root=0):
         """Pool renumbered in depth-first pre-order, children in slot order."""
