@@ -10,6 +10,7 @@
  *   src/octree.c:236-318   (sphere neighbour query)
  *   src/sph.c:10-231       (SPH density, forces and cooling over those queries)
  *   CosmosimSubsystem.cpp:12-43, src/renderer.c:348-364 (float snapshots)
+ *   src/quasar.c:11-529    (quasar step and compaction, explicit random state)
  *   tests/test_physics.c:34-71 (direct-sum and energy judges)
  * Floating-point expressions are kept in the reference's evaluation order so
  * that, compiled with -O2 -ffp-contract=off on x86-64, results are bit-identical
@@ -21,6 +22,7 @@
  * report the number of interactions per body.
  */
 #include <float.h>
+#include <stdint.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -530,6 +532,357 @@ void cref_sph_cooling(Body *bodies, int n, double dt)
         b->internal_energy = unew;
         b->pressure = (CREF_SPH_GAMMA - 1.0) * rho * unew;
     }
+}
+
+/* ------------------------------------------------------------ quasar
+ * One quasar physics step (src/quasar.c:477-510) and the compaction of dead
+ * bodies (:512-523), with the random stream and the jets' knot phase as explicit
+ * state instead of the reference's file-scope statics (quasar.c:11, :282).  The
+ * sub-steps, their loops over bodies in index order and every expression keep
+ * the reference's order, so results are bit-identical for the same state:
+ *   accretion onto each black hole in index order (:113-198), whose mass,
+ *   velocity and spin change inside the loop (:58-97); accretion rate and
+ *   luminosity (:486-499); radiation feedback (:200-230); jet decay with lobes
+ *   (:393-432); recycling of distant jets (:434-475); jet spawning (:232-391). */
+typedef struct {               /* src/quasar.h:6-22 */
+    double accretion_radius, swallow_radius, isco_radius, viscosity_alpha, jet_speed, jet_mass,
+        jet_lifetime, feedback_strength, eta_eff, eddington_k, accretion_smoothing;
+    int jet_cap, jet_count, max_bodies, jet_ring_count;
+} CrefQuasarConfig;
+
+static double crq_uniform(uint64_t *st)                 /* xorshift64, :14-27 */
+{
+    uint64_t x = *st;
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    *st = x;
+    return (x >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static double crq_gaussian(uint64_t *st)                /* Box-Muller, :29-36 */
+{
+    double u1 = crq_uniform(st);
+    double u2 = crq_uniform(st);
+    if (u1 < 1e-15)
+        u1 = 1e-15;
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * CREF_PI * u2);
+}
+
+static void crq_kill(Body *b)
+{
+    b->mass = 0.0;
+    b->vx = b->vy = b->vz = 0.0;
+    b->ax = b->ay = b->az = 0.0;
+}
+
+/* The black hole takes the victim's mass and momentum; its spin leans toward the
+ * victim's orbital angular momentum (computed with the new velocity). */
+static void crq_swallow(Body *bh, Body *v)              /* :58-97 */
+{
+    double M = bh->mass + v->mass;
+    bh->vx = (bh->mass * bh->vx + v->mass * v->vx) / M;
+    bh->vy = (bh->mass * bh->vy + v->mass * v->vy) / M;
+    bh->vz = (bh->mass * bh->vz + v->mass * v->vz) / M;
+    double rx = v->x - bh->x, ry = v->y - bh->y, rz = v->z - bh->z;
+    double ux = v->vx - bh->vx, uy = v->vy - bh->vy, uz = v->vz - bh->vz;
+    double lx = ry * uz - rz * uy, ly = rz * ux - rx * uz, lz = rx * uy - ry * ux;
+    double l = sqrt(lx * lx + ly * ly + lz * lz);
+    if (l > 1e-15) {
+        double w = v->mass / M * 0.01;
+        bh->spin_x += w * lx / l;
+        bh->spin_y += w * ly / l;
+        bh->spin_z += w * lz / l;
+        double sm = sqrt(bh->spin_x * bh->spin_x + bh->spin_y * bh->spin_y +
+                         bh->spin_z * bh->spin_z);
+        if (sm > 1e-15) {
+            bh->spin_x /= sm;
+            bh->spin_y /= sm;
+            bh->spin_z /= sm;
+        }
+    }
+    bh->mass = M;
+    crq_kill(v);
+}
+
+static void crq_drag(const Body *bh, Body *g, double alpha)   /* :99-121 */
+{
+    double rx = g->x - bh->x, ry = g->y - bh->y, rz = g->z - bh->z;
+    double r = sqrt(rx * rx + ry * ry + rz * rz);
+    if (r < 1e-15)
+        return;
+    double ux = g->vx - bh->vx, uy = g->vy - bh->vy, uz = g->vz - bh->vz;
+    double vr = (ux * rx + uy * ry + uz * rz) / r;
+    g->ax -= alpha * (vr * rx / r);
+    g->ay -= alpha * (vr * ry / r);
+    g->az -= alpha * (vr * rz / r);
+}
+
+static double crq_accretion(Body *b, int n, const CrefQuasarConfig *q)   /* :123-198 */
+{
+    double swallowed = 0.0;
+    const double racc2 = q->accretion_radius * q->accretion_radius;
+    const double rsw2 = q->swallow_radius * q->swallow_radius;
+    for (int s = 0; s < n; ++s) {
+        if (b[s].type != BODY_SMBH || b[s].mass <= 0.0)
+            continue;
+        Body *bh = &b[s];
+        for (int i = 0; i < n; ++i) {
+            Body *v = &b[i];
+            if (i == s || v->mass <= 0.0 || v->type == BODY_SMBH)
+                continue;
+            double dx = v->x - bh->x, dy = v->y - bh->y, dz = v->z - bh->z;
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > racc2)
+                continue;
+            if (d2 < rsw2) {
+                swallowed += v->mass;
+                crq_swallow(bh, v);
+                continue;
+            }
+            double r = sqrt(d2);
+            double ux = v->vx - bh->vx, uy = v->vy - bh->vy, uz = v->vz - bh->vz;
+            double lx = dy * uz - dz * uy, ly = dz * ux - dx * uz, lz = dx * uy - dy * ux;
+            double rc = (lx * lx + ly * ly + lz * lz) / (bh->mass * r);
+            if (rc > q->isco_radius && v->type != BODY_GAS) {
+                v->type = BODY_GAS;
+                if (v->smoothing_h < 1e-10)
+                    v->smoothing_h = q->accretion_radius * 0.3;
+                if (v->internal_energy < 1e-10)
+                    v->internal_energy =
+                        0.001 * (v->vx * v->vx + v->vy * v->vy + v->vz * v->vz) + 0.01;
+            }
+            if (v->type == BODY_GAS) {
+                crq_drag(bh, v, q->viscosity_alpha);
+                if (r < q->isco_radius) {
+                    swallowed += v->mass;
+                    crq_swallow(bh, v);
+                }
+            }
+        }
+    }
+    return swallowed;
+}
+
+static void crq_feedback(Body *b, int n, const CrefQuasarConfig *q)     /* :200-230 */
+{
+    const double rfb2 = (q->accretion_radius * 3.0) * (q->accretion_radius * 3.0);
+    for (int s = 0; s < n; ++s) {
+        if (b[s].type != BODY_SMBH || b[s].luminosity <= 0.0)
+            continue;
+        const Body *bh = &b[s];
+        for (int i = 0; i < n; ++i) {
+            if (i == s || b[i].mass <= 0.0 || b[i].type == BODY_SMBH)
+                continue;
+            double dx = b[i].x - bh->x, dy = b[i].y - bh->y, dz = b[i].z - bh->z;
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > rfb2 || d2 < 1e-10)
+                continue;
+            double r = sqrt(d2);
+            double f = q->feedback_strength * bh->luminosity / (4.0 * CREF_PI * d2);
+            b[i].ax += f * dx / r;
+            b[i].ay += f * dy / r;
+            b[i].az += f * dz / r;
+        }
+    }
+}
+
+static void crq_decay(Body *b, int *n, int n_alloc, CrefQuasarConfig *q, double dt,
+                      uint64_t *rng)                                      /* :393-432 */
+{
+    const int n0 = *n;
+    for (int i = 0; i < n0; ++i) {
+        if (b[i].type != BODY_JET || b[i].mass <= 0.0)
+            continue;
+        b[i].lifetime -= dt;
+        if (b[i].lifetime > 0.0)
+            continue;
+        if (*n < n_alloc && b[i].mass > 0.2) {
+            Body *lobe = &b[(*n)++];
+            memset(lobe, 0, sizeof(*lobe));
+            lobe->type = BODY_LOBE;
+            lobe->mass = b[i].mass * 0.5;
+            lobe->x = b[i].x;
+            lobe->y = b[i].y;
+            lobe->z = b[i].z;
+            lobe->vx = b[i].vx * 0.15 + (crq_uniform(rng) - 0.5) * 0.5;
+            lobe->vy = b[i].vy * 0.15 + (crq_uniform(rng) - 0.5) * 0.5;
+            lobe->vz = b[i].vz * 0.15 + (crq_uniform(rng) - 0.5) * 0.5;
+            lobe->lifetime = q->jet_lifetime * 3.0;
+        }
+        crq_kill(&b[i]);
+        q->jet_count--;
+    }
+    for (int i = 0; i < *n; ++i) {
+        if (b[i].type != BODY_LOBE || b[i].mass <= 0.0)
+            continue;
+        b[i].lifetime -= dt;
+        if (b[i].lifetime <= 0.0)
+            crq_kill(&b[i]);
+    }
+}
+
+static void crq_recycle(Body *b, int n, CrefQuasarConfig *q, uint64_t *rng)   /* :434-475 */
+{
+    int s = 0;
+    while (s < n && !(b[s].type == BODY_SMBH && b[s].mass > 0.0))
+        ++s;
+    if (s == n)
+        return;
+    const Body *bh = &b[s];
+    const double rr = q->accretion_radius * 10.0, rr2 = rr * rr;
+    for (int i = 0; i < n; ++i) {
+        Body *j = &b[i];
+        if (j->type != BODY_JET || j->mass <= 0.0)
+            continue;
+        double dx = j->x - bh->x, dy = j->y - bh->y, dz = j->z - bh->z;
+        if (dx * dx + dy * dy + dz * dz < rr2)
+            continue;
+        double angle = 2.0 * CREF_PI * crq_uniform(rng);
+        double cphi = 2.0 * crq_uniform(rng) - 1.0;
+        double sphi = sqrt(1.0 - cphi * cphi);
+        double rp = q->accretion_radius;
+        j->x = bh->x + rp * sphi * cos(angle);
+        j->y = bh->y + rp * sphi * sin(angle);
+        j->z = bh->z + rp * cphi;
+        double vin = 0.5 * sqrt(bh->mass / rp);
+        double nx = (j->x - bh->x) / rp, ny = (j->y - bh->y) / rp, nz = (j->z - bh->z) / rp;
+        j->vx = bh->vx - vin * nx;
+        j->vy = bh->vy - vin * ny;
+        j->vz = bh->vz - vin * nz;
+        j->ax = j->ay = j->az = 0.0;
+        j->type = BODY_GAS;
+        j->lifetime = 0.0;
+        j->smoothing_h = q->accretion_radius * 0.3;
+        j->internal_energy = 0.001 * (j->vx * j->vx + j->vy * j->vy + j->vz * j->vz) + 0.01;
+        j->density = 0.0;
+        j->pressure = 0.0;
+        q->jet_count--;
+    }
+}
+
+static void crq_spawn(Body *b, int *n, int n_alloc, CrefQuasarConfig *q, uint64_t *rng,
+                      int *phase)                                         /* :232-391 */
+{
+    const int k = ++(*phase);
+    double smod = 1.0 + 0.6 * sin(k * 0.10);
+    if (smod < 0.5)
+        smod = 0.5;
+    const double cone = 0.05 + 0.13 * (0.5 + 0.5 * sin(k * 0.057));
+    const double hph = k * 0.04;
+    for (int s = 0; s < *n; ++s) {
+        if (b[s].type != BODY_SMBH || b[s].luminosity <= 0.0)
+            continue;
+        const Body *bh = &b[s];
+        const double energy = 0.5 * q->jet_mass * q->jet_speed * q->jet_speed;
+        if (energy < 1e-15)
+            continue;
+        int pulses = (int)(bh->luminosity / energy);
+        if (pulses < 2)
+            pulses = 2;
+        if (pulses > 8)
+            pulses = 8;
+        /* two unit vectors across the spin axis */
+        double px, py, pz;
+        if (fabs(bh->spin_z) < 0.9) {
+            px = -bh->spin_y; py = bh->spin_x; pz = 0.0;
+        } else {
+            px = 0.0; py = -bh->spin_z; pz = bh->spin_y;
+        }
+        double pm = sqrt(px * px + py * py + pz * pz);
+        if (pm > 1e-15) {
+            px /= pm; py /= pm; pz /= pm;
+        }
+        double qx = bh->spin_y * pz - bh->spin_z * py;
+        double qy = bh->spin_z * px - bh->spin_x * pz;
+        double qz = bh->spin_x * py - bh->spin_y * px;
+        double qm = sqrt(qx * qx + qy * qy + qz * qz);
+        if (qm > 1e-15) {
+            qx /= qm; qy /= qm; qz /= qm;
+        }
+        const double ha = q->swallow_radius * 0.35;
+        const double hx = ha * (cos(hph) * px + sin(hph) * qx);
+        const double hy = ha * (cos(hph) * py + sin(hph) * qy);
+        const double hz = ha * (cos(hph) * pz + sin(hph) * qz);
+        const double speed = q->jet_speed * smod;
+        const int total = pulses + ((smod > 1.35) ? 4 : 0);
+        for (int j = 0; j < total && q->jet_count < q->jet_cap; ++j) {
+            if (*n >= n_alloc)
+                break;
+            const int burst = j >= pulses;
+            const double sg = (j % 2 == 0) ? 1.0 : -1.0;
+            const double cn = burst ? 0.012 : cone;
+            const double sp = burst ? speed * 1.28 : speed;
+            const double ms = burst ? q->jet_mass * 1.9 : q->jet_mass;
+            const double life = burst ? q->jet_lifetime * 1.4 : q->jet_lifetime;
+            const double fwd = burst ? q->swallow_radius * 0.6 : 0.0;
+            const double ang = crq_uniform(rng) * 2.0 * CREF_PI;
+            const double off = crq_gaussian(rng) * cn;
+            const int ring = burst ? 1 : q->jet_ring_count;
+            for (int r = 0; r < ring; ++r) {
+                if (*n >= n_alloc || q->jet_count >= q->jet_cap)
+                    break;
+                Body *e = &b[*n];
+                memset(e, 0, sizeof(*e));
+                e->type = BODY_JET;
+                e->lifetime = life;
+                double ra = (ring > 1) ? 2.0 * CREF_PI * r / ring + ang : ang;
+                double rr = (ring > 1) ? q->swallow_radius * 0.15 : 0.0;
+                double ox = rr * (cos(ra) * px + sin(ra) * qx);
+                double oy = rr * (cos(ra) * py + sin(ra) * qy);
+                double oz = rr * (cos(ra) * pz + sin(ra) * qz);
+                e->mass = (ring > 1) ? ms * 0.6 : ms;
+                e->x = bh->x + sg * bh->spin_x * (q->swallow_radius + fwd) + off * (px * cos(ang)) +
+                       sg * hx + ox;
+                e->y = bh->y + sg * bh->spin_y * (q->swallow_radius + fwd) + off * (py * cos(ang)) +
+                       sg * hy + oy;
+                e->z = bh->z + sg * bh->spin_z * (q->swallow_radius + fwd) + off * (pz * sin(ang)) +
+                       sg * hz + oz;
+                double ev = sp * 0.04 * (1.0 + e->lifetime * 0.1);
+                e->vx = bh->vx + sg * bh->spin_x * sp + ox * ev / (rr + 1e-15);
+                e->vy = bh->vy + sg * bh->spin_y * sp + oy * ev / (rr + 1e-15);
+                e->vz = bh->vz + sg * bh->spin_z * sp + oz * ev / (rr + 1e-15);
+                (*n)++;
+                q->jet_count++;
+            }
+        }
+    }
+}
+
+void cref_quasar_step(Body *b, int *n, int n_alloc, CrefQuasarConfig *q, double dt,
+                      uint64_t *rng, int *phase)
+{
+    const double swallowed = crq_accretion(b, *n, q);
+    for (int i = 0; i < *n; ++i) {                                       /* :486-499 */
+        if (b[i].type != BODY_SMBH)
+            continue;
+        const double mdot = swallowed / dt;
+        b[i].accretion_rate = q->accretion_smoothing * b[i].accretion_rate +
+                              (1.0 - q->accretion_smoothing) * mdot;
+        const double ledd = q->eddington_k * b[i].mass;
+        b[i].luminosity = q->eta_eff * b[i].accretion_rate;
+        if (b[i].luminosity > ledd)
+            b[i].luminosity = ledd;
+    }
+    crq_feedback(b, *n, q);
+    crq_decay(b, n, n_alloc, q, dt, rng);
+    crq_recycle(b, *n, q, rng);
+    crq_spawn(b, n, n_alloc, q, rng, phase);
+}
+
+void cref_decay_jets(Body *b, int *n, int n_alloc, CrefQuasarConfig *q, double dt,
+                     uint64_t *rng)
+{
+    crq_decay(b, n, n_alloc, q, dt, rng);
+}
+
+int cref_quasar_compact(Body *b, int n)                                  /* :512-523 */
+{
+    int w = 0;
+    for (int r = 0; r < n; ++r)
+        if (b[r].mass > 0.0)
+            b[w++] = b[r];
+    return w;
 }
 
 /* ------------------------------------------------------------ snapshots

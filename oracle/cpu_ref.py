@@ -23,6 +23,22 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CPUREF_SO = os.path.join(HERE, "libcpuref.so")
 REF_SO = os.path.join(HERE, "_ref", "libcosmosim_ref.so")
 
+class QuasarConfig(ctypes.Structure):
+    """src/quasar.h:6-22 (the defaults of quasar_default_config, src/quasar.c:38-56)."""
+    _fields_ = [(f, ctypes.c_double) for f in (
+        "accretion_radius", "swallow_radius", "isco_radius", "viscosity_alpha", "jet_speed",
+        "jet_mass", "jet_lifetime", "feedback_strength", "eta_eff", "eddington_k",
+        "accretion_smoothing")] + [(f, ctypes.c_int) for f in (
+            "jet_cap", "jet_count", "max_bodies", "jet_ring_count")]
+
+    @classmethod
+    def default(cls, **kw):
+        c = cls(3.0, 0.3, 0.5, 0.05, 20.0, 0.5, 5.0, 1.0, 0.1, 0.5, 0.95, 2000, 0, 0, 4)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
 _VP = ctypes.c_void_p
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -106,6 +122,10 @@ class CpuRef(_Base):
         L.cref_sph_cooling.argtypes = [_VP, _I, _D]
         L.cref_snapshot.argtypes = [_VP, _I, _I, _VP, _I]
         L.cref_snapshot.restype = _I
+        L.cref_quasar_step.argtypes = [_VP, _VP, _I, _VP, _D, _VP, _VP]
+        L.cref_decay_jets.argtypes = [_VP, _VP, _I, _VP, _D, _VP]
+        L.cref_quasar_compact.argtypes = [_VP, _I]
+        L.cref_quasar_compact.restype = _I
         self._build = L.cref_build
         self._init = L.cref_init_acc
         self._nb = L.cref_find_neighbors
@@ -141,6 +161,29 @@ class CpuRef(_Base):
         out = np.zeros(2)
         self.lib.cref_energy(_p(bodies), len(bodies), _D(G), _D(softening), _p(out))
         return float(out[0]), float(out[1])
+
+    def quasar_step(self, bodies, n, cfg, dt, state):
+        """One quasar step on bodies[:n] (capacity len(bodies)); state = [rng, phase]
+        carried between calls (the reference's statics start at rng 1, phase 0).
+        Returns the new n; cfg (QuasarConfig) is updated in place."""
+        nn = _I(n)
+        rng = ctypes.c_uint64(state[0])
+        ph = _I(state[1])
+        self.lib.cref_quasar_step(_p(bodies), ctypes.byref(nn), len(bodies), ctypes.byref(cfg),
+                                  _D(dt), ctypes.byref(rng), ctypes.byref(ph))
+        state[0], state[1] = rng.value, ph.value
+        return nn.value
+
+    def decay_jets(self, bodies, n, cfg, dt, state):
+        nn = _I(n)
+        rng = ctypes.c_uint64(state[0])
+        self.lib.cref_decay_jets(_p(bodies), ctypes.byref(nn), len(bodies), ctypes.byref(cfg),
+                                 _D(dt), ctypes.byref(rng))
+        state[0] = rng.value
+        return nn.value
+
+    def quasar_compact(self, bodies, n):
+        return self.lib.cref_quasar_compact(_p(bodies), n)
 
     def snapshot(self, bodies, fmt, max_out=None):
         """Float snapshot of the live bodies (fmt 0: 16 floats, 1: 10 floats)."""
@@ -193,6 +236,39 @@ class RefBuild(_Base):
     def _step_impl(self, bodies, dt, G, softening, theta, pool):
         self.lib.integrator_step(_p(bodies), len(bodies), _D(dt), _D(G), _D(softening),
                                  _D(theta), _p(pool), 0)
+
+
+class RefQuasar:
+    """The reference's quasar_step / quasar_compact from a private copy of
+    oracle/_ref's library: its random stream and knot phase are file-scope statics,
+    so each instance starts from the reference's initial state (rng 1, phase 0)."""
+
+    def __init__(self):
+        import shutil
+        import tempfile
+        self._dir = tempfile.mkdtemp(prefix="refq")
+        path = os.path.join(self._dir, "libref_copy.so")
+        shutil.copy(REF_SO, path)
+        L = ctypes.CDLL(path)
+        L.quasar_step.argtypes = [_VP, _VP, _I, _VP, _D]
+        L.decay_jets_wrapper.argtypes = [_VP, _VP, _I, _VP, _D]
+        L.quasar_compact.argtypes = [_VP, _I]
+        L.quasar_compact.restype = _I
+        self.lib = L
+
+    def quasar_step(self, bodies, n, cfg, dt):
+        nn = _I(n)
+        self.lib.quasar_step(_p(bodies), ctypes.byref(nn), len(bodies), ctypes.byref(cfg), _D(dt))
+        return nn.value
+
+    def decay_jets(self, bodies, n, cfg, dt):
+        nn = _I(n)
+        self.lib.decay_jets_wrapper(_p(bodies), ctypes.byref(nn), len(bodies), ctypes.byref(cfg),
+                                    _D(dt))
+        return nn.value
+
+    def quasar_compact(self, bodies, n):
+        return self.lib.quasar_compact(_p(bodies), n)
 
 
 def bodies_copy(b):
