@@ -12,13 +12,13 @@ import os
 
 import numpy as np
 
-from ._abi import (BODY_DTYPE, NODE_DTYPE, CosmosimConfig, ExtParams, ExtStats, PRECISIONS,
-                   PREC_FP64_STRICT, PREC_FP64, PREC_FP32, IC_DISK, IC_MERGER, make_bodies,
-                   positions, velocities, accelerations, ptr)
+from ._abi import (BODY_DTYPE, NODE_DTYPE, CosmosimConfig, ExtParams, ExtStats, QuasarConfig,
+                   PRECISIONS, PREC_FP64_STRICT, PREC_FP64, PREC_FP32, IC_DISK, IC_MERGER,
+                   make_bodies, positions, velocities, accelerations, ptr)
 
 __all__ = [
-    "BODY_DTYPE", "NODE_DTYPE", "CosmosimConfig", "ExtParams", "ExtStats", "PRECISIONS",
-    "PREC_FP64_STRICT", "PREC_FP64", "PREC_FP32", "IC_DISK", "IC_MERGER", "make_bodies",
+    "BODY_DTYPE", "NODE_DTYPE", "CosmosimConfig", "ExtParams", "ExtStats", "QuasarConfig",
+    "PRECISIONS", "PREC_FP64_STRICT", "PREC_FP64", "PREC_FP32", "IC_DISK", "IC_MERGER", "make_bodies",
     "positions", "velocities", "accelerations", "lib", "library_path", "CosmosimError",
     "octree_build", "octree_compute_forces", "octree_find_neighbors", "integrator_step",
     "integrator_init_accelerations", "generate", "default_orbit", "orbit_coefficients",
@@ -70,6 +70,10 @@ _SIGS = {
     "cosmosim_ext_step_sph": (_I, [_VP, _I]),
     "cosmosim_ext_snapshot_ue": (_I, [_VP, _VP, _I, _VP]),
     "cosmosim_ext_snapshot_render": (_I, [_VP, _VP, _VP]),
+    "cosmosim_ext_reserve": (_I, [_VP, _I]),
+    "cosmosim_ext_quasar_step": (_I, [_VP, _VP, _D]),
+    "cosmosim_ext_quasar_compact": (_I, [_VP, _VP]),
+    "cosmosim_ext_quasar_state": (_I, [_VP, _VP, _VP, _I]),
     "cosmosim_ext_interaction_counts": (_I, [_VP, _VP, _I]),
     "cosmosim_ext_get_stats": (_I, [_VP, ctypes.POINTER(ExtStats)]),
     "cosmosim_ext_set_timing": (_I, [_VP, _I]),
@@ -114,6 +118,9 @@ _SIGS = {
     "csg_sph_cooling": (_I, [_VP, _D]),
     "csg_substeps_sph": (_I, [_VP, _I, _D, _D, _D, _D, _I]),
     "csg_snapshot": (_I, [_VP, _I, _VP, _I, _VP]),
+    "csg_quasar_step": (_I, [_VP, _VP, _D]),
+    "csg_quasar_compact": (_I, [_VP, _VP]),
+    "csg_quasar_state": (_I, [_VP, _VP, _VP, _I]),
     "csg_interaction_counts": (_I, [_VP, _VP, _I]),
     "csg_get_stats": (_I, [_VP, _VP]),
     "csg_set_timing": (_I, [_VP, _I]),
@@ -389,6 +396,37 @@ class Simulation(Handle):
     def advance_sph(self, substeps):
         """Substeps of integrator_step with SPH enabled, device-resident."""
         _check(lib().cosmosim_ext_step_sph(self.h, substeps), "cosmosim_ext_step_sph")
+
+    # ------------------------------------------------------------ quasar
+    def reserve(self, capacity):
+        """Grow the body capacity (room for bodies a quasar step appends)."""
+        _check(lib().cosmosim_ext_reserve(self.h, int(capacity)), "reserve")
+
+    def quasar_step(self, cfg, dt):
+        """quasar_step on the device (src/quasar.c:477-510); cfg (QuasarConfig) is
+        updated in place; returns the new body count."""
+        _check(lib().cosmosim_ext_quasar_step(self.h, ctypes.byref(cfg), dt), "quasar_step")
+        self.n = self.count()
+        return self.n
+
+    def quasar_compact(self):
+        """quasar_compact on the device (src/quasar.c:512-523); returns the new count."""
+        k = ctypes.c_int(0)
+        _check(lib().cosmosim_ext_quasar_compact(self.h, ctypes.byref(k)), "quasar_compact")
+        self.n = self.count()
+        return k.value
+
+    def quasar_state(self, rng=None, phase=None):
+        """Get [rng, phase]; or set them when given."""
+        r, p = ctypes.c_uint64(0), ctypes.c_int(0)
+        if rng is None and phase is None:
+            _check(lib().cosmosim_ext_quasar_state(self.h, ctypes.byref(r), ctypes.byref(p), 0),
+                   "quasar_state")
+            return [r.value, p.value]
+        r.value, p.value = int(rng), int(phase)
+        _check(lib().cosmosim_ext_quasar_state(self.h, ctypes.byref(r), ctypes.byref(p), 1),
+               "quasar_state")
+        return [r.value, p.value]
 
     # ------------------------------------------------------------ snapshots
     def snapshot_ue(self, max_bodies=None):

@@ -75,6 +75,24 @@ class ExtParams(ctypes.Structure):
     ]
 
 
+class QuasarConfig(ctypes.Structure):
+    """CosmosimQuasarConfig: the byte layout of the reference's QuasarConfig
+    (src/quasar.h:6-22).  default() holds quasar_default_config's values
+    (src/quasar.c:38-56)."""
+    _fields_ = [(f, ctypes.c_double) for f in (
+        "accretion_radius", "swallow_radius", "isco_radius", "viscosity_alpha", "jet_speed",
+        "jet_mass", "jet_lifetime", "feedback_strength", "eta_eff", "eddington_k",
+        "accretion_smoothing")] + [(f, ctypes.c_int) for f in (
+            "jet_cap", "jet_count", "max_bodies", "jet_ring_count")]
+
+    @classmethod
+    def default(cls, **kw):
+        c = cls(3.0, 0.3, 0.5, 0.05, 20.0, 0.5, 5.0, 1.0, 0.1, 0.5, 0.95, 2000, 0, 0, 4)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
 class ExtStats(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int),

@@ -8,10 +8,12 @@
  *   quasar == 0 : device-resident.  The substep loop (kick+drift, build, walk,
  *                 kick) never leaves the GPU; the host Body[] is refreshed once
  *                 at the end of cosmosim_step (src/cosmosim_api.c:115-144).
- *   quasar == 1 : the host Body[] is the source of truth, because the caller's
- *                 quasar_step (src/quasar.c:477-509) changes n, masses, positions
- *                 and accelerations between substeps and sph.c needs the host
- *                 pool; each substep goes through the legacy path.
+ *   quasar == 1 : device-resident as well: each substep is the SPH-enabled
+ *                 integrator step followed by the quasar step (src/quasar.c:477-510),
+ *                 both on the device, which change n, masses, positions and
+ *                 accelerations; compaction every 60 calls (src/cosmosim_api.c:
+ *                 129-139) also runs there.  The host Body[] is refreshed once
+ *                 at the end of cosmosim_step.
  */
 #include <math.h>
 #include <stdio.h>
@@ -28,16 +30,32 @@
 #define API_SEPARATION 40.0
 #define API_APPROACH_VEL 0.15
 
-/* ABI image of the caller's QuasarConfig (src/quasar.h:6-22); this library only
- * passes it to the caller's quasar_* functions. */
-typedef struct {
-    double accretion_radius, swallow_radius, isco_radius, viscosity_alpha, jet_speed, jet_mass,
-        jet_lifetime, feedback_strength, eta_eff, eddington_k, accretion_smoothing;
-    int jet_cap, jet_count, max_bodies, jet_ring_count;
-} QuasarConfigABI;
+/* The reference's QuasarConfig (src/quasar.h:6-22) has the layout of
+ * csg_quasar_config; quasar_default_config's values (src/quasar.c:38-56). */
+typedef csg_quasar_config QuasarConfigABI;
 
-/* Initial conditions and quasar physics are the caller's code
- * (src/initial_conditions.c, src/quasar.c): weak references. */
+static QuasarConfigABI quasar_defaults(void)
+{
+    QuasarConfigABI q;
+    memset(&q, 0, sizeof(q));
+    q.accretion_radius = 3.0;
+    q.swallow_radius = 0.3;
+    q.isco_radius = 0.5;
+    q.viscosity_alpha = 0.05;
+    q.jet_speed = 20.0;
+    q.jet_mass = 0.5;
+    q.jet_lifetime = 5.0;
+    q.feedback_strength = 1.0;
+    q.eta_eff = 0.1;
+    q.eddington_k = 0.5;
+    q.accretion_smoothing = 0.95;
+    q.jet_cap = 2000;
+    q.jet_ring_count = 4;
+    return q;
+}
+
+/* Initial conditions are the caller's code (src/initial_conditions.c): weak
+ * references. */
 extern void generate_spiral_galaxy(Body *, int, double, double, double, double, double, double)
     __attribute__((weak));
 extern void generate_merger(Body *, int, double, double) __attribute__((weak));
@@ -45,9 +63,6 @@ extern void generate_quasar_galaxy(Body *, int, double, double, double, double, 
                                    double) __attribute__((weak));
 extern void generate_quasar_merger(Body *, int, double, double, double) __attribute__((weak));
 extern void generate_merger_dust(Body *, int, int, double) __attribute__((weak));
-extern QuasarConfigABI quasar_default_config(void) __attribute__((weak));
-extern void quasar_step(Body *, int *, int, QuasarConfigABI *, double) __attribute__((weak));
-extern int quasar_compact(Body *, int) __attribute__((weak));
 
 typedef struct {
     Body *bodies;     /* pinned host array of n_alloc records */
@@ -145,9 +160,9 @@ COSMOSIM_API SimHandle cosmosim_create(CosmosimConfig config)
     default_params(&s->params, &config);
 
     if (config.quasar) {
-        if (!generate_quasar_merger || !generate_quasar_galaxy || !generate_merger_dust ||
-            !quasar_default_config || !quasar_step || !quasar_compact) {
-            cs_set_error(COSMOSIM_E_UNSUPPORTED, "cosmosim_create (quasar mode needs quasar.c)");
+        if (!generate_quasar_merger || !generate_quasar_galaxy || !generate_merger_dust) {
+            cs_set_error(COSMOSIM_E_UNSUPPORTED,
+                         "cosmosim_create (quasar mode needs initial_conditions.c)");
             state_free(s);
             return NULL;
         }
@@ -184,7 +199,7 @@ COSMOSIM_API SimHandle cosmosim_create(CosmosimConfig config)
     if (config.quasar) {
         st = cs_legacy_init(s->ctx, s->bodies, initial_n, API_G, API_SOFTENING, config.theta,
                             s->pool, 8 * n_alloc);
-        QuasarConfigABI q = quasar_default_config();
+        QuasarConfigABI q = quasar_defaults();
         q.accretion_radius = config.accretion_radius;
         q.jet_speed = config.jet_speed;
         q.feedback_strength = config.feedback_strength;
@@ -228,16 +243,23 @@ COSMOSIM_API void cosmosim_step(SimHandle handle)
     SimState *s = (SimState *)handle;
     int st = COSMOSIM_OK;
     if (s->config.quasar) {
+        /* integrator_step with SPH, then quasar_step, per substep; all on the
+         * device (src/cosmosim_api.c:124-139) */
         for (int sub = 0; sub < s->config.substeps && st == COSMOSIM_OK; ++sub) {
-            st = cs_legacy_step(s->ctx, s->bodies, s->current_n, s->config.dt, API_G,
-                                API_SOFTENING, s->config.theta, s->pool, 8 * s->n_alloc, 1);
+            st = csg_substeps_sph(s->ctx, 1, s->config.dt, API_G, API_SOFTENING,
+                                  s->config.theta, COSMOSIM_PREC_FP64_STRICT);
             if (st == COSMOSIM_OK)
-                quasar_step(s->bodies, &s->current_n, s->n_alloc, &s->qcfg, s->config.dt);
+                st = csg_quasar_step(s->ctx, &s->qcfg, s->config.dt);
         }
         s->compact_counter++;
-        if (s->compact_counter >= 60) {
-            s->current_n = quasar_compact(s->bodies, s->current_n);
+        if (st == COSMOSIM_OK && s->compact_counter >= 60) {
+            st = csg_quasar_compact(s->ctx, NULL);
             s->compact_counter = 0;
+        }
+        s->current_n = csg_size(s->ctx);
+        if (st == COSMOSIM_OK) {
+            s->host_stale = 1;
+            st = sync_host(s);
         }
     } else {
         st = csg_substeps(s->ctx, s->config.substeps, s->config.dt, s->params.G,
@@ -593,14 +615,62 @@ static int snapshot(SimHandle h, int fmt, float *out, int max_bodies, int *count
 {
     if (!h || !count)
         return COSMOSIM_E_ARG;
+    return csg_snapshot(((SimState *)h)->ctx, fmt, out, max_bodies, count);
+}
+
+COSMOSIM_API int cosmosim_ext_reserve(SimHandle h, int capacity)
+{
+    if (!h || capacity < 0)
+        return COSMOSIM_E_ARG;
     SimState *s = (SimState *)h;
-    if (s->config.quasar && !s->host_stale) {
-        /* quasar mode: the host bodies are the current state */
-        int st = csg_set_bodies(s->ctx, s->bodies, s->current_n);
-        if (st)
-            return st;
+    if (capacity <= s->n_alloc)
+        return COSMOSIM_OK;
+    int st = sync_host(s);
+    if (st)
+        return st;
+    Body *nb = (Body *)csg_host_alloc(sizeof(Body) * (size_t)capacity);
+    if (!nb)
+        return COSMOSIM_E_ALLOC;
+    if (s->current_n > 0)
+        memcpy(nb, s->bodies, sizeof(Body) * (size_t)s->current_n);
+    if ((st = csg_reserve(s->ctx, capacity))) {
+        csg_host_free(nb);
+        return st;
     }
-    return csg_snapshot(s->ctx, fmt, out, max_bodies, count);
+    csg_host_free(s->bodies);
+    s->bodies = nb;
+    s->n_alloc = capacity;
+    return COSMOSIM_OK;
+}
+
+COSMOSIM_API int cosmosim_ext_quasar_step(SimHandle h, CosmosimQuasarConfig *cfg, double dt)
+{
+    if (!h || !cfg)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    int st = csg_quasar_step(s->ctx, (csg_quasar_config *)cfg, dt);
+    s->current_n = csg_size(s->ctx);
+    if (!st)
+        s->host_stale = 1;
+    return st;
+}
+
+COSMOSIM_API int cosmosim_ext_quasar_compact(SimHandle h, int *n)
+{
+    if (!h)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    int st = csg_quasar_compact(s->ctx, n);
+    s->current_n = csg_size(s->ctx);
+    if (!st)
+        s->host_stale = 1;
+    return st;
+}
+
+COSMOSIM_API int cosmosim_ext_quasar_state(SimHandle h, unsigned long long *rng, int *phase,
+                                           int set)
+{
+    return h ? csg_quasar_state(((SimState *)h)->ctx, rng, phase, set) : COSMOSIM_E_ARG;
 }
 
 COSMOSIM_API int cosmosim_ext_snapshot_ue(SimHandle h, float *out, int max_bodies, int *count)

@@ -97,6 +97,13 @@ struct csg_ctx {
     int gq_valid = 0;         // gq describes the current tree
     void *snap = nullptr;     // snapshot records and scan scratch
     size_t snap_cap = 0;
+    void *qbuf = nullptr;     // quasar: flags, ranks, black-hole list, draws
+    size_t qbuf_cap = 0;
+    void *qtmp = nullptr;     // quasar: compaction scratch
+    size_t qtmp_cap = 0;
+    struct QDev *qdev = nullptr, *qhost = nullptr;   // quasar step state (device, pinned)
+    unsigned long long qrng = 1;   // quasar random stream (xorshift64)
+    int qphase = 0;                // quasar knot phase
     void *nbuf = nullptr;     // neighbour queries: scratch
     size_t nbuf_cap = 0;
     int *wcost = nullptr;   // FP32 wave-mask walk: node steps of each wave in the last walk
@@ -129,6 +136,7 @@ static void frozen_free(csg_ctx *c);
 static void nb_free(csg_ctx *c);   // csg_neighbors.inl
 static void sph_free(csg_ctx *c);  // csg_sph.inl
 static void snap_free(csg_ctx *c); // csg_snapshot.inl
+static void q_free(csg_ctx *c);    // csg_quasar.inl
 
 static inline int blocks_for(long long n, int per) { return (int)((n + per - 1) / per); }
 static inline int at_least1(int v) { return v < 1 ? 1 : v; }
@@ -364,6 +372,7 @@ void csg_destroy(csg_ctx *c) {
     nb_free(c);
     sph_free(c);
     snap_free(c);
+    q_free(c);
     dfree(c->bp);
     dfree(c->itotal);
     dfree(c->err_dev);
@@ -1033,3 +1042,4 @@ void csg_host_free(void *p) {
 #include "csg_neighbors.inl"
 #include "csg_sph.inl"
 #include "csg_snapshot.inl"
+#include "csg_quasar.inl"

@@ -151,6 +151,27 @@ int cosmosim_ext_step_sph(SimHandle h, int substeps);
  * current, so they are uploaded first. */
 int cosmosim_ext_snapshot_ue(SimHandle h, float *out, int max_bodies, int *count);
 int cosmosim_ext_snapshot_render(SimHandle h, float *out, int *count);
+/* Quasar physics on the device (replace quasar_step and quasar_compact,
+ * src/quasar.h:27-35).  CosmosimQuasarConfig has the byte layout of the
+ * reference's QuasarConfig (src/quasar.h:6-22), so a caller's QuasarConfig can
+ * be passed by pointer; jet_count is updated.  The step may add bodies (lobes,
+ * jets) up to the handle's capacity; cosmosim_get_count reports the new count.
+ * Results equal src/quasar.c's bit for bit except the positions and velocities
+ * of spawned and recycled jets, which go through sin/cos/log (last-bit
+ * differences between the device's and the host's libm).  The random stream and
+ * the jets' knot phase (file-scope statics in the reference, starting at 1 and
+ * 0) belong to the handle: _quasar_state gets (set = 0) or sets (set = 1) them. */
+typedef struct {
+    double accretion_radius, swallow_radius, isco_radius, viscosity_alpha, jet_speed, jet_mass,
+        jet_lifetime, feedback_strength, eta_eff, eddington_k, accretion_smoothing;
+    int jet_cap, jet_count, max_bodies, jet_ring_count;
+} CosmosimQuasarConfig;
+/* Grow the handle's body capacity (device and host) to at least capacity
+ * records, keeping its bodies: room for the bodies a quasar step appends. */
+int cosmosim_ext_reserve(SimHandle h, int capacity);
+int cosmosim_ext_quasar_step(SimHandle h, CosmosimQuasarConfig *cfg, double dt);
+int cosmosim_ext_quasar_compact(SimHandle h, int *n);
+int cosmosim_ext_quasar_state(SimHandle h, unsigned long long *rng, int *phase, int set);
 /* Last error recorded on the handle by a void legacy call (cosmosim_step). */
 int cosmosim_ext_handle_error(SimHandle h);
 /* Last error recorded on the calling thread by octree_* / integrator_*. */

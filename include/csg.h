@@ -91,6 +91,23 @@ int  csg_substeps_sph(csg_ctx *ctx, int k, double dt, double G, double softening
  * (16 floats, at most max_bodies records); fmt 1: the renderer's upload record
  * (10 floats).  *count = records copied to out. */
 int  csg_snapshot(csg_ctx *ctx, int fmt, float *out, int max_bodies, int *count);
+/* Quasar physics on the device state.  csg_quasar_config has the byte layout of
+ * the reference's QuasarConfig (src/quasar.h:6-22). */
+typedef struct {
+    double accretion_radius, swallow_radius, isco_radius, viscosity_alpha, jet_speed, jet_mass,
+        jet_lifetime, feedback_strength, eta_eff, eddington_k, accretion_smoothing;
+    int jet_cap, jet_count, max_bodies, jet_ring_count;
+} csg_quasar_config;
+/* quasar_step (src/quasar.c:477-510) on the device bodies: accretion, feedback,
+ * jet decay and recycling, jet spawning.  cfg->jet_count is updated; the body
+ * count (csg_size) may grow up to the context's capacity.  The random stream and
+ * the knot phase are the context's (csg_quasar_state). */
+int  csg_quasar_step(csg_ctx *ctx, csg_quasar_config *cfg, double dt);
+/* quasar_compact (src/quasar.c:512-523): live bodies keep their order. */
+int  csg_quasar_compact(csg_ctx *ctx, int *n_out);
+/* Get (set = 0) or set (set = 1) the quasar random state and knot phase; a new
+ * context starts at 1 and 0, as the reference's statics. */
+int  csg_quasar_state(csg_ctx *ctx, unsigned long long *rng, int *phase, int set);
 /* Walk a caller-supplied reference-layout pool from `root` (exact reference
  * traversal, FP64 strict arithmetic). */
 int  csg_walk_pool(csg_ctx *ctx, const OctreeNode *pool, int pool_size, int root,
