@@ -218,6 +218,77 @@ def sph_runs(ca, bodies, cpu=True):
     return out
 
 
+def quasar_scene(n, seed=7):
+    """Quasar-mode bodies: two spinning black holes, each with a disk of stars
+    and gas (a few inside the swallow radius and the ISCO), jets with scattered
+    lifetimes, some of them beyond the recycling radius.  Capacity n + n / 4."""
+    rng = np.random.default_rng(seed)
+    from cosmosim_amd import BODY_DTYPE
+    b = np.zeros(n + n // 4, dtype=BODY_DTYPE)
+    centres = np.array([[-6.0, 0.0, 0.0], [6.0, 1.0, 0.0]])
+    for k in range(2):
+        b["type"][k], b["mass"][k] = 2, 40.0 + 10.0 * k
+        b["x"][k], b["y"][k], b["z"][k] = centres[k]
+        s = rng.normal(size=3)
+        b["spin_x"][k], b["spin_y"][k], b["spin_z"][k] = s / np.linalg.norm(s)
+        b["accretion_rate"][k] = 30.0
+    jets = 400
+    m = n - 2 - jets
+    host = rng.integers(0, 2, m)
+    r = np.abs(rng.normal(0.0, 3.0, m)) + 0.02
+    phi = rng.uniform(0, 2 * np.pi, m)
+    sl = slice(2, 2 + m)
+    b["x"][sl] = centres[host, 0] + r * np.cos(phi)
+    b["y"][sl] = centres[host, 1] + r * np.sin(phi)
+    b["z"][sl] = rng.normal(0, 0.1, m)
+    vc = np.sqrt(b["mass"][host] / r) * rng.uniform(0.5, 1.1, m)
+    b["vx"][sl], b["vy"][sl] = -vc * np.sin(phi), vc * np.cos(phi)
+    b["mass"][sl] = 1e-4
+    b["type"][sl] = np.where(rng.uniform(size=m) < 0.4, 1, 0)
+    js = slice(2 + m, n)
+    b["type"][js], b["mass"][js] = 3, 0.5
+    b["lifetime"][js] = rng.uniform(0.0, 5.0, jets)
+    jr = rng.uniform(1, 60, jets)
+    b["x"][js], b["y"][js] = -6.0 + jr, rng.normal(size=jets)
+    return b, n
+
+
+def quasar_runs(ca, n=1_000_000, steps=10, cpu=True):
+    """quasar_step on the device (accretion passes, feedback, jets) for a
+    1M-body quasar scene, per step, against the reference's quasar_step
+    (oracle/_ref, else the oracle) on the host for the same first steps."""
+    b, n0 = quasar_scene(n)
+    sim = ca.Simulation(b[:n0], softening=0.01, theta=0.5, precision="fp32")
+    sim.reserve(len(b))
+    cfg = ca.QuasarConfig.default(jet_count=400, jet_cap=n // 10)
+    sim.quasar_step(cfg, 5e-4)                         # first-step buffers
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.quasar_step(cfg, 5e-4)
+    t1 = time.perf_counter()
+    out = {"bodies": n0, "steps": steps, "gpu_ms_per_step": (t1 - t0) / steps * 1e3,
+           "bodies_after": sim.count(), "jet_count": cfg.jet_count}
+    sim.close()
+    if cpu:
+        from oracle.cpu_ref import CpuRef, QuasarConfig, RefBuild
+        c = b.copy()
+        rc = QuasarConfig.default(jet_count=400, jet_cap=n // 10)
+        if RefBuild.available():
+            from oracle.cpu_ref import RefQuasar
+            R, kind = RefQuasar(), "reference"
+            step = lambda k: R.quasar_step(c, k, rc, 5e-4)
+        else:
+            R, kind, st = CpuRef(), "port", [1, 0]
+            step = lambda k: R.quasar_step(c, k, rc, 5e-4, st)
+        k = step(n0)
+        c0 = time.perf_counter()
+        for _ in range(3):
+            k = step(k)
+        c1 = time.perf_counter()
+        out.update({"cpu_kind": kind, "cpu_threads": 1, "cpu_ms_per_step": (c1 - c0) / 3 * 1e3})
+    return out
+
+
 def host_cpu():
     try:
         with open("/proc/cpuinfo") as f:
@@ -265,6 +336,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=20)
     ap.add_argument("--no-neighbors", action="store_true")
     ap.add_argument("--no-sph", action="store_true")
+    ap.add_argument("--no-quasar", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to scale the traffic counters")
     args = ap.parse_args()
@@ -373,6 +445,9 @@ def main():
     if not args.no_sph:
         line["sph"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
                        for k, v in sph_runs(ca, bodies, cpu=not args.no_cpu).items()}
+    if not args.no_quasar:
+        line["quasar"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
+                          for k, v in quasar_runs(ca, cpu=not args.no_cpu).items()}
     if not args.no_cpu:
         cb = cpu_baseline(ca, bodies, 0.5)
         line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v)
