@@ -32,6 +32,9 @@
 // A live key is never all ones in those bits (that cell lies in the 1% margin
 // of the root cube), so the all-ones key of dead bodies sorts last.
 #define CSG_KEY_LEVELS 16
+// FP32 node records past the last node: padding the FP32 walk may step onto
+// (it tests for the end once per four steps); never accepted, no mass
+#define CSG_NODE_PAD 4
 #define CSG_KEY_BITS (3 * CSG_KEY_LEVELS)
 #define CSG_SORT_PASSES ((CSG_KEY_BITS + 7) / 8)
 // collision words: ext[0] levels L..L+20, ext[1] L+21..L+41, ext[2] L+42..63
@@ -983,7 +986,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_iscan_down(const int *__restrict__ 
             if (i == nl) {
                 const int nn = (nl == 0) ? 1 : run;
                 bp->n_nodes = nn;
-                if (nn > bp->node_cap) bp->overflow = 1;
+                if (nn + CSG_NODE_PAD > bp->node_cap) bp->overflow = 1;
             }
         }
         run += v[r];
@@ -1303,6 +1306,21 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const int nl = bp->n_live;
     if (bp->overflow) return;
     const double H = bp->root[3];
+    if (b == 0) {
+        // padding records after the tree (the FP32 walk's unrolled rounds may
+        // step onto up to three of them): opening criterion never met
+        // (s2 = +inf), so no lane interacts, descends past them one by one
+        const int nn = bp->n_nodes;
+        for (int k = 0; k < CSG_NODE_PAD; ++k) {
+            WNode32 f;
+            f.cx = f.cy = f.cz = 0.0;
+            f.m = 0.0f;
+            f.s2 = __builtin_inff();
+            f.skip = nn + k + 1;
+            f.body = -1;
+            n32[nn + k] = f;
+        }
+    }
     if (nl == 0) {
         if (b == 0) {
             WNode r;
@@ -1824,16 +1842,14 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         const int jump = max(nd.skip, ii + 1);
         ii = D ? ii + 1 : jump;
     };
-    // four steps per round (the exit test between them, the fold count once per
-    // round); the compensated fold still comes after every CSG_FOLD_EVERY steps
-    for (;;) {
-        if (ii >= nnodes) break;
+    // four steps per round and one end test per round: a round that starts
+    // before the end may step onto up to three of the inert padding records
+    // after the last node (CSG_NODE_PAD); the compensated fold comes after every
+    // CSG_FOLD_EVERY steps
+    while (ii < nnodes) {
         step();
-        if (ii >= nnodes) break;
         step();
-        if (ii >= nnodes) break;
         step();
-        if (ii >= nnodes) break;
         step();
         if (++quads == CSG_FOLD_EVERY / 4) {
             quads = 0;
