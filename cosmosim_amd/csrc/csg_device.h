@@ -75,24 +75,32 @@ static_assert(sizeof(WNode) == 64, "WNode is one 64-byte record");
 // The same node for the FP32 walk (40 bytes).  The centre of mass stays FP64:
 // a lane forms its body-node offset as fp32(C - x) with the subtraction in FP64
 // (full rate on CDNA4), so the offset carries a single FP32 rounding however far
-// the pair sits from the origin.  Mass is FP32; s2 is (2h)^2 / theta^2 for the
-// opening angle the records were emitted for (divided in FP64, +inf at theta 0),
-// so the walk's opening test is s2 < d^2.
+// the pair sits from the origin.  Mass is FP32; s2 is (2h)^2 / theta^2 + eps^2
+// for the opening angle and softening the records were emitted for (formed in
+// FP64, +inf at theta 0), so the walk's opening test is s2 < d^2 + eps^2 with
+// the softened squared distance it needs for the force anyway.
 struct __align__(8) WNode32 {
     double cx, cy, cz;  // centre of mass
     float m;            // total mass
-    float s2;           // (2h)^2 / theta^2; -1 for a leaf
+    float s2;           // (2h)^2 / theta^2 + eps^2; -1 for a leaf
     int skip;
     int body;
 };
 static_assert(sizeof(WNode32) == 40, "WNode32 is one 40-byte record");
 
-// Dispatch order of the wave-mask walk: wave slot s walks wave order[s]
-// (s < n); cost (may be null) receives each wave's node-step count.
+// What the FP32 records fold into s2: 1 / theta^2 and eps^2.
+struct Open32 {
+    double inv_th2;
+    double eps2;
+};
+
+// Dispatch order of the walks: wave slot s walks wave order[s] (s < slots,
+// order[s] >= 0); cost (may be null) receives each wave's node-step count.
 struct WaveOrder {
     const int *order;
-    int n;
+    int n;        // waves
     int *cost;
+    int slots;    // entries of order (slots past it, or holding -1, walk nothing)
 };
 
 // One record of a walk tape: a node visited by a wavefront and the lanes
@@ -133,14 +141,14 @@ __device__ __forceinline__ void tape_close(const TapeOut &t, int wave, int lane,
 }
 
 __device__ __forceinline__ void store_nodes(WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
-                                            int id, const WNode &w, double inv_th2) {
+                                            int id, const WNode &w, Open32 op) {
     if (nodes) nodes[id] = w;   // FP64 records: skipped when only the FP32 walk reads the tree
     WNode32 f;
     f.cx = w.cx;
     f.cy = w.cy;
     f.cz = w.cz;
     f.m = (float)w.m;
-    f.s2 = w.body >= 0 ? -1.0f : (float)(w.s2 * inv_th2);  // a leaf passes every test
+    f.s2 = w.body >= 0 ? -1.0f : (float)(w.s2 * op.inv_th2 + op.eps2);  // a leaf passes every test
     f.skip = w.skip;
     f.body = w.body;
     n32[id] = f;
@@ -1279,7 +1287,7 @@ __device__ __forceinline__ double dd_range(const DD *__restrict__ pref, int cap1
 __device__ __forceinline__ void emit_internal(
     const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext, const int *__restrict__ nbase,
     const DD *__restrict__ pref, int cap1, const BuildParams *__restrict__ bp, int nl, int b,
-    int d, int id, WNode *__restrict__ nodes, WNode32 *__restrict__ n32, double inv_th2) {
+    int d, int id, WNode *__restrict__ nodes, WNode32 *__restrict__ n32, Open32 op) {
     const double H = bp->root[3];
     const int e = range_end(key, ext, b, d, nl);
     WNode w;
@@ -1292,7 +1300,7 @@ __device__ __forceinline__ void emit_internal(
     w.s2 = s * s;
     w.skip = nbase[e + 1];
     w.body = -1; w.depth = d; w.first = b; w.last = e; w.pad = 0;
-    store_nodes(nodes, n32, id, w, inv_th2);
+    store_nodes(nodes, n32, id, w, op);
 }
 
 __global__ __launch_bounds__(CSG_TPB) void k_emit(
@@ -1301,7 +1309,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const DD *__restrict__ pref, int cap1, const double *__restrict__ xs,
     const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ms,
     const BuildParams *__restrict__ bp, WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
-    double inv_th2, int *__restrict__ owner) {
+    Open32 op, int *__restrict__ owner) {
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
     if (bp->overflow) return;
@@ -1329,7 +1337,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
             const double s = 2.0 * H;
             r.s2 = s * s;
             r.skip = 1; r.body = -1; r.depth = 0; r.first = 0; r.last = -1; r.pad = 0;
-            store_nodes(nodes, n32, 0, r, inv_th2);
+            store_nodes(nodes, n32, 0, r, op);
         }
         return;
     }
@@ -1362,7 +1370,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
         w.s2 = s * s;
         w.skip = id + 1;
         w.body = i; w.depth = depth; w.first = b; w.last = e; w.pad = 0;
-        store_nodes(nodes, n32, id, w, inv_th2);
+        store_nodes(nodes, n32, id, w, op);
     }
 }
 
@@ -1371,7 +1379,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit_internal(
     const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext,
     const signed char *__restrict__ L, const int *__restrict__ nbase, const DD *__restrict__ pref,
     int cap1, const BuildParams *__restrict__ bp, WNode *__restrict__ nodes,
-    WNode32 *__restrict__ n32, double inv_th2, const int *__restrict__ owner) {
+    WNode32 *__restrict__ n32, Open32 op, const int *__restrict__ owner) {
     const int nl = bp->n_live, nn = bp->n_nodes;
     if (bp->overflow || nl == 0) return;
     for (int id = blockIdx.x * CSG_TPB + threadIdx.x; id < nn; id += gridDim.x * CSG_TPB) {
@@ -1379,7 +1387,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit_internal(
         if (b < 0) continue;
         const int lp = (b > 0) ? L[b - 1] : -1;
         const int d = min(lp, 63) + 1 + (id - nbase[b]);
-        emit_internal(key, ext, nbase, pref, cap1, bp, nl, b, d, id, nodes, n32, inv_th2);
+        emit_internal(key, ext, nbase, pref, cap1, bp, nl, b, d, id, nodes, n32, op);
     }
 }
 
@@ -1418,7 +1426,7 @@ __device__ __forceinline__ int xcd_tile(int grid, int chunk) {
 __device__ __forceinline__ int walk_wave(int nb8, int chunk, const WaveOrder &wo) {
     if (wo.order) {
         const int slot = (int)blockIdx.x * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
-        if (slot >= wo.n) return -1;
+        if (slot >= wo.slots) return -1;
         const int w = wo.order[slot];
         return (w < 0 || w >= wo.n) ? -1 : w;
     }
@@ -1598,7 +1606,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     int nrec = 0;
     const int me = act ? sidx[b] : -1;
     const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
-    const float eps2f = (float)eps2;  // theta is folded into the records' s2
+    const float eps2f = (float)eps2;  // enters d2; the records' s2 carries theta and eps^2
     (void)theta2;
     const unsigned long long gmask =
         (GS == 64) ? ~0ull : (((1ull << (GS & 63)) - 1ull) << (lane & ~(GS - 1)));
@@ -1622,12 +1630,12 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
         const float dx = (float)(nd.cx - x);
         const float dy = (float)(nd.cy - y);
         const float dz = (float)(nd.cz - z);
-        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));   // softened
         // leaves carry s2 = -1, so crit holds for them and only self is skipped
         const bool crit = nd.s2 < d2;
         const bool inter = here && crit && nd.body != me;
         const bool desc = here && !crit;
-        const float yv = __builtin_amdgcn_rsqf(d2 + eps2f);
+        const float yv = __builtin_amdgcn_rsqf(d2);
         const float f = inter ? nd.m * (yv * yv * yv) : 0.0f;
         bx = fmaf(dx, f, bx);
         by = fmaf(dy, f, by);
@@ -1658,85 +1666,113 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     if (lane == 0) atomicAdd(itotal, c);
 }
 
-// Longest-first dispatch order for the wave-mask walk, from each wave's node-step
-// count in the previous walk (longest processing time first: long traversals
-// start early and short ones fill the end of the grid, which otherwise drains
-// with few waves per SIMD).  Counting sort on the count quantized to 4096
-// levels, descending; the order inside a level is whatever the LDS atomics give,
-// which changes only the timing.  One block of 1024 threads.
+// Dispatch order of the walks, from each wave's node-step count in the previous
+// walk.  Block b of a walk grid runs on XCD b % 8 (workgroups are dealt out to
+// the XCDs round-robin), and wave slot s = 4 b + k of the grid walks wave
+// order[s].  The waves are cut into 8 contiguous ranges of sorted bodies, one
+// per XCD, of about equal total step count: each XCD's L2 then serves one
+// eighth of the space (the near-field nodes its waves share).  Each range is
+// dispatched longest first (longest processing time first: long traversals
+// start early and short ones fill the end, where an XCD otherwise drains with
+// few waves per SIMD).  A boundary moves from the equal-count one by at most
+// nw / 64 waves, so a range holds at most WO_CAP(nw) waves; the slots past a
+// range's end hold -1 (no wave).  Block r of 8 (1024 threads) orders range r
+// by a counting sort on the step count quantized to 4096 levels, descending;
+// the order inside a level is whatever the LDS atomics give, which changes only
+// the timing.  Every block computes the same boundaries.
+#define WO_XCDS 8
+#define WO_CAP(nw) ((((5 * (nw)) / 32 + 8) + 3) & ~3)
 __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cost, int nw,
-                                                     int *__restrict__ order) {
-    constexpr int PER = 16;   // loads in flight per thread and pass
+                                                     int capw, int *__restrict__ order) {
     __shared__ int hist[4096];
+    __shared__ long long wsum[16];
+    __shared__ int bnd[WO_XCDS + 1];
     __shared__ int part[16];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    int mx = 0;
-    for (int base = 0; base < nw; base += 1024 * PER) {
-        int v[PER];
+    const int r = blockIdx.x;
+    // chunk sums of the step counts (thread t: waves [c0, c1)) and their prefix
+    const int per = (nw + 1023) / 1024;
+    const int c0 = min(nw, t * per), c1 = min(nw, c0 + per);
+    long long s = 0;
+    for (int i = c0; i < c1; ++i) s += max(cost[i], 0);
+    long long inc = s;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = base + k * 1024 + t;
-            v[k] = i < nw ? cost[i] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < PER; ++k) mx = max(mx, v[k]);
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
     }
+    if (lane == 63) wsum[wid] = inc;
+    if (t <= WO_XCDS) bnd[t] = (int)(((long long)t * nw) / WO_XCDS);   // equal counts
+    __syncthreads();
+    long long ex = inc - s, T = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (k < wid) ex += wsum[k];
+        T += wsum[k];
+    }
+    // boundary k: one past the wave whose prefix reaches k T / 8 (one chunk holds it)
+    for (int k = 1; k < WO_XCDS; ++k) {
+        const long long target = (T * k) / WO_XCDS;
+        if (target > 0 && ex < target && ex + s >= target) {
+            long long run = ex;
+            int i = c0;
+            while (run < target) run += max(cost[i++], 0);
+            bnd[k] = i;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int k = 1; k < WO_XCDS; ++k) {
+            const int mid = (int)(((long long)k * nw) / WO_XCDS), slack = nw / 64;
+            const int bk = min(max(bnd[k], mid - slack), mid + slack);
+            bnd[k] = min(max(bk, bnd[k - 1]), nw);
+        }
+    }
+    for (int i = t; i < 4096; i += 1024) hist[i] = 0;
+    __syncthreads();
+    const int lo = bnd[r], hi = bnd[r + 1], m = hi - lo;
+    int mx = 0;
+    for (int i = lo + t; i < hi; i += 1024) mx = max(mx, cost[i]);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) part[wid] = mx;
-    for (int i = t; i < 4096; i += 1024) hist[i] = 0;
     __syncthreads();
     mx = 0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) mx = max(mx, part[k]);
     int shift = 0;
     while ((mx >> shift) >= 4096) ++shift;
-    // any count (stale ones included) lands in a bin: the order is always a
-    // permutation of [0, nw)
+    // any count (stale ones included) lands in a bin: the range's slots receive
+    // a permutation of its waves
     auto bin = [&](int c) { return 4095 - (max(c, 0) >> shift); };
-    for (int base = 0; base < nw; base += 1024 * PER) {
-        int v[PER];
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = base + k * 1024 + t;
-            v[k] = i < nw ? cost[i] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < PER; ++k)
-            if (base + k * 1024 + t < nw) atomicAdd(&hist[bin(v[k])], 1);
-    }
+    for (int i = lo + t; i < hi; i += 1024) atomicAdd(&hist[bin(cost[i])], 1);
     __syncthreads();
     // exclusive scan of the 4096 bins: 4 per thread, then across threads
     int h[4], sum = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) { h[k] = hist[4 * t + k]; sum += h[k]; }
-    int inc = sum;
+    int incl = sum;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += u;
+        const int u = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += u;
     }
     __syncthreads();
-    if (lane == 63) part[wid] = inc;
+    if (lane == 63) part[wid] = incl;
     __syncthreads();
     int base0 = 0;
     for (int k = 0; k < wid; ++k) base0 += part[k];
-    int run = base0 + inc - sum;
+    int run = base0 + incl - sum;
 #pragma unroll
     for (int k = 0; k < 4; ++k) { hist[4 * t + k] = run; run += h[k]; }
     __syncthreads();
-    for (int base = 0; base < nw; base += 1024 * PER) {
-        int v[PER];
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = base + k * 1024 + t;
-            v[k] = i < nw ? cost[i] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < PER; ++k)
-            if (base + k * 1024 + t < nw)
-                order[atomicAdd(&hist[bin(v[k])], 1)] = base + k * 1024 + t;
+    // position j of range r -> slot 4 (8 (j / 4) + r) + j % 4 (block 8 (j / 4) + r)
+    auto slot = [&](int j) { return 4 * (WO_XCDS * (j >> 2) + r) + (j & 3); };
+    for (int i = lo + t; i < hi; i += 1024) {
+        const int j = atomicAdd(&hist[bin(cost[i])], 1);
+        if (j < capw) order[slot(j)] = i;
     }
+    for (int j = m + t; j < capw; j += 1024) order[slot(j)] = -1;
 }
 
 // Lane select on a wave mask held in SGPRs: m's bit for this lane ? t : f.
@@ -1802,7 +1838,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
     int nrec = 0;
     const int me = act ? sidx[b] : -1;
     const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
-    const float eps2f = (float)eps2;  // theta is folded into the records' s2
+    const float eps2f = (float)eps2;  // enters d2; the records' s2 carries theta and eps^2
     (void)theta2;
     const char *const base = (const char *)nodes;
     int p = act ? 0 : 0x7fffffff;
@@ -1820,7 +1856,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         const float dx = (float)(nd.cx - x);
         const float dy = (float)(nd.cy - y);
         const float dz = (float)(nd.cz - z);
-        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));   // softened
         // compare predicates: 41 = signed <=, 4 = ordered <, 33 = !=
         const unsigned long long H = __builtin_amdgcn_sicmp(p, ii, 41);
         // leaves carry s2 = -1, so the criterion holds for them
@@ -1829,7 +1865,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         unsigned long long I = A;                    // interacting
         if constexpr (SELF) I &= __builtin_amdgcn_sicmp(nd.body, me, 33);
         const unsigned long long D = H & ~C;         // descending
-        const float yv = __builtin_amdgcn_rsqf(d2 + eps2f);
+        const float yv = __builtin_amdgcn_rsqf(d2);
         const float f = lane_sel_f32(I, nd.m * (yv * yv * yv), 0.0f);
         bx = fmaf(dx, f, bx);
         by = fmaf(dy, f, by);
