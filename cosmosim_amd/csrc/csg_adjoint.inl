@@ -870,7 +870,6 @@ int csg_record(csg_ctx *c, int k, double dt, double G, double softening, double 
     CSG_USE(c);
     if (k < 0 || every <= 0 || precision < 0 || precision > 2) return COSMOSIM_E_ARG;
     c->theta_hint = theta;
-    c->eps2_hint = softening * softening;
     int s = adj_reserve(c, c->tape_cap > 0 ? c->tape_cap : 2048);
     if (s) return s;
     frozen_free(c);
@@ -928,7 +927,6 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
     const int n = c->n, k = c->rec_k, every = c->rec_every;
     const double dt = c->rec_dt, G = c->rec_G, eps = c->rec_eps, th = c->rec_theta;
     c->theta_hint = th;
-    c->eps2_hint = eps * eps;
     const int prec = c->rec_prec;
     const size_t slot = 9 * (size_t)n;
     const int nck = k / every + 2;

@@ -82,9 +82,7 @@ struct csg_ctx {
     int n_frozen = 0;
     double rec_dt = 0, rec_G = 1, rec_eps = 0, rec_theta = 0;
     double theta_hint = 0.5;  // opening angle of the next walks (set by every walk entry)
-    double eps2_hint = 0.0;   // squared softening of the next walks (set with theta_hint)
     double th32 = -1.0;       // opening angle folded into the FP32 records (-1: none)
-    double eps32 = -1.0;      // squared softening folded into the FP32 records
     int walk_group = 8;     // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
     int walk_group32 = 0;   // FP32 walk: 0 = wave-mask traversal, else lanes per group
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
@@ -620,21 +618,19 @@ static int launch_build(csg_ctx *c, bool with64) {
                                         c->dd_part, c->tpref);
     k_dd_down2<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part, c->tpref,
                                        c->pref);
-    const Open32 op{c->theta_hint > 0.0 ? 1.0 / (c->theta_hint * c->theta_hint) : HUGE_VAL,
-                    c->eps2_hint};
+    const double inv_th2 = c->theta_hint > 0.0 ? 1.0 / (c->theta_hint * c->theta_hint) : HUGE_VAL;
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
         c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->xs, c->ys, c->zs, c->ms,
-        c->bp, with64 ? c->nodes : nullptr, c->nodes32, op, c->owner);
+        c->bp, with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->owner);
     k_emit_internal<<<at_least1(blocks_for(2 * (long long)n + 1, CSG_TPB)), CSG_TPB, 0, st>>>(
         c->key, c->ext, c->lcp, c->nbase, c->pref, c->cap + 1, c->bp,
-        with64 ? c->nodes : nullptr, c->nodes32, op, c->owner);
+        with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->owner);
     CSG_LAUNCHED();
     c->tree_valid = 1;
     c->geom_valid = 0;
     c->gq_valid = 0;
     c->nodes64_valid = with64 ? 1 : 0;
     c->th32 = c->theta_hint;
-    c->eps32 = c->eps2_hint;
     c->tape_valid = 0;   // a tape describes the tree it was recorded on
     return COSMOSIM_OK;
 }
@@ -677,9 +673,8 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     hipStream_t st = c->stream;
     const double eps2 = softening * softening, th2 = theta * theta;
     c->theta_hint = theta;
-    c->eps2_hint = eps2;
-    if (precision == COSMOSIM_PREC_FP32 && c->tree_valid && (c->th32 != theta || c->eps32 != eps2)) {
-        // the FP32 records carry the opening angle and softening: emit them again
+    if (precision == COSMOSIM_PREC_FP32 && c->tree_valid && c->th32 != theta) {
+        // the FP32 records carry the opening angle: emit them again for this one
         const int s = launch_build(c, c->nodes64_valid != 0);
         if (s) return s;
     }
@@ -811,7 +806,6 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
                  int precision) {
     CSG_USE(c);
     c->theta_hint = theta;
-    c->eps2_hint = softening * softening;
     if (precision < 0 || precision > 2 || k < 0) return COSMOSIM_E_ARG;
     if (k == 0) return COSMOSIM_OK;
     int s;

@@ -75,24 +75,20 @@ static_assert(sizeof(WNode) == 64, "WNode is one 64-byte record");
 // The same node for the FP32 walk (40 bytes).  The centre of mass stays FP64:
 // a lane forms its body-node offset as fp32(C - x) with the subtraction in FP64
 // (full rate on CDNA4), so the offset carries a single FP32 rounding however far
-// the pair sits from the origin.  Mass is FP32; s2 is (2h)^2 / theta^2 + eps^2
-// for the opening angle and softening the records were emitted for (formed in
-// FP64, +inf at theta 0), so the walk's opening test is s2 < d^2 + eps^2 with
-// the softened squared distance it needs for the force anyway.
+// the pair sits from the origin.  Mass is FP32; s2 is (2h)^2 / theta^2 for the
+// opening angle the records were emitted for (divided in FP64, +inf at theta 0),
+// so the walk's opening test is s2 < d^2.  Softening stays out of the test: with
+// eps^2 on both sides, FP32 resolves the criterion of nodes much smaller than
+// eps only to eps^2's rounding, and the opening decisions drift from the FP64
+// walk's.
 struct __align__(8) WNode32 {
     double cx, cy, cz;  // centre of mass
     float m;            // total mass
-    float s2;           // (2h)^2 / theta^2 + eps^2; -1 for a leaf
+    float s2;           // (2h)^2 / theta^2; -1 for a leaf
     int skip;
     int body;
 };
 static_assert(sizeof(WNode32) == 40, "WNode32 is one 40-byte record");
-
-// What the FP32 records fold into s2: 1 / theta^2 and eps^2.
-struct Open32 {
-    double inv_th2;
-    double eps2;
-};
 
 // Dispatch order of the walks: wave slot s walks wave order[s] (s < slots,
 // order[s] >= 0); cost (may be null) receives each wave's node-step count.
@@ -141,14 +137,14 @@ __device__ __forceinline__ void tape_close(const TapeOut &t, int wave, int lane,
 }
 
 __device__ __forceinline__ void store_nodes(WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
-                                            int id, const WNode &w, Open32 op) {
+                                            int id, const WNode &w, double inv_th2) {
     if (nodes) nodes[id] = w;   // FP64 records: skipped when only the FP32 walk reads the tree
     WNode32 f;
     f.cx = w.cx;
     f.cy = w.cy;
     f.cz = w.cz;
     f.m = (float)w.m;
-    f.s2 = w.body >= 0 ? -1.0f : (float)(w.s2 * op.inv_th2 + op.eps2);  // a leaf passes every test
+    f.s2 = w.body >= 0 ? -1.0f : (float)(w.s2 * inv_th2);  // a leaf passes every test
     f.skip = w.skip;
     f.body = w.body;
     n32[id] = f;
@@ -1287,7 +1283,7 @@ __device__ __forceinline__ double dd_range(const DD *__restrict__ pref, int cap1
 __device__ __forceinline__ void emit_internal(
     const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext, const int *__restrict__ nbase,
     const DD *__restrict__ pref, int cap1, const BuildParams *__restrict__ bp, int nl, int b,
-    int d, int id, WNode *__restrict__ nodes, WNode32 *__restrict__ n32, Open32 op) {
+    int d, int id, WNode *__restrict__ nodes, WNode32 *__restrict__ n32, double inv_th2) {
     const double H = bp->root[3];
     const int e = range_end(key, ext, b, d, nl);
     WNode w;
@@ -1300,7 +1296,7 @@ __device__ __forceinline__ void emit_internal(
     w.s2 = s * s;
     w.skip = nbase[e + 1];
     w.body = -1; w.depth = d; w.first = b; w.last = e; w.pad = 0;
-    store_nodes(nodes, n32, id, w, op);
+    store_nodes(nodes, n32, id, w, inv_th2);
 }
 
 __global__ __launch_bounds__(CSG_TPB) void k_emit(
@@ -1309,7 +1305,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const DD *__restrict__ pref, int cap1, const double *__restrict__ xs,
     const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ms,
     const BuildParams *__restrict__ bp, WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
-    Open32 op, int *__restrict__ owner) {
+    double inv_th2, int *__restrict__ owner) {
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
     if (bp->overflow) return;
@@ -1337,7 +1333,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
             const double s = 2.0 * H;
             r.s2 = s * s;
             r.skip = 1; r.body = -1; r.depth = 0; r.first = 0; r.last = -1; r.pad = 0;
-            store_nodes(nodes, n32, 0, r, op);
+            store_nodes(nodes, n32, 0, r, inv_th2);
         }
         return;
     }
@@ -1370,7 +1366,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
         w.s2 = s * s;
         w.skip = id + 1;
         w.body = i; w.depth = depth; w.first = b; w.last = e; w.pad = 0;
-        store_nodes(nodes, n32, id, w, op);
+        store_nodes(nodes, n32, id, w, inv_th2);
     }
 }
 
@@ -1379,7 +1375,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit_internal(
     const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext,
     const signed char *__restrict__ L, const int *__restrict__ nbase, const DD *__restrict__ pref,
     int cap1, const BuildParams *__restrict__ bp, WNode *__restrict__ nodes,
-    WNode32 *__restrict__ n32, Open32 op, const int *__restrict__ owner) {
+    WNode32 *__restrict__ n32, double inv_th2, const int *__restrict__ owner) {
     const int nl = bp->n_live, nn = bp->n_nodes;
     if (bp->overflow || nl == 0) return;
     for (int id = blockIdx.x * CSG_TPB + threadIdx.x; id < nn; id += gridDim.x * CSG_TPB) {
@@ -1387,7 +1383,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit_internal(
         if (b < 0) continue;
         const int lp = (b > 0) ? L[b - 1] : -1;
         const int d = min(lp, 63) + 1 + (id - nbase[b]);
-        emit_internal(key, ext, nbase, pref, cap1, bp, nl, b, d, id, nodes, n32, op);
+        emit_internal(key, ext, nbase, pref, cap1, bp, nl, b, d, id, nodes, n32, inv_th2);
     }
 }
 
@@ -1606,7 +1602,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
     int nrec = 0;
     const int me = act ? sidx[b] : -1;
     const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
-    const float eps2f = (float)eps2;  // enters d2; the records' s2 carries theta and eps^2
+    const float eps2f = (float)eps2;  // theta is folded into the records' s2
     (void)theta2;
     const unsigned long long gmask =
         (GS == 64) ? ~0ull : (((1ull << (GS & 63)) - 1ull) << (lane & ~(GS - 1)));
@@ -1630,12 +1626,12 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
         const float dx = (float)(nd.cx - x);
         const float dy = (float)(nd.cy - y);
         const float dz = (float)(nd.cz - z);
-        const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));   // softened
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         // leaves carry s2 = -1, so crit holds for them and only self is skipped
         const bool crit = nd.s2 < d2;
         const bool inter = here && crit && nd.body != me;
         const bool desc = here && !crit;
-        const float yv = __builtin_amdgcn_rsqf(d2);
+        const float yv = __builtin_amdgcn_rsqf(d2 + eps2f);
         const float f = inter ? nd.m * (yv * yv * yv) : 0.0f;
         bx = fmaf(dx, f, bx);
         by = fmaf(dy, f, by);
@@ -1838,7 +1834,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
     int nrec = 0;
     const int me = act ? sidx[b] : -1;
     const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
-    const float eps2f = (float)eps2;  // enters d2; the records' s2 carries theta and eps^2
+    const float eps2f = (float)eps2;  // theta is folded into the records' s2
     (void)theta2;
     const char *const base = (const char *)nodes;
     int p = act ? 0 : 0x7fffffff;
@@ -1856,7 +1852,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         const float dx = (float)(nd.cx - x);
         const float dy = (float)(nd.cy - y);
         const float dz = (float)(nd.cz - z);
-        const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));   // softened
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         // compare predicates: 41 = signed <=, 4 = ordered <, 33 = !=
         const unsigned long long H = __builtin_amdgcn_sicmp(p, ii, 41);
         // leaves carry s2 = -1, so the criterion holds for them
@@ -1865,7 +1861,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32w(
         unsigned long long I = A;                    // interacting
         if constexpr (SELF) I &= __builtin_amdgcn_sicmp(nd.body, me, 33);
         const unsigned long long D = H & ~C;         // descending
-        const float yv = __builtin_amdgcn_rsqf(d2);
+        const float yv = __builtin_amdgcn_rsqf(d2 + eps2f);
         const float f = lane_sel_f32(I, nd.m * (yv * yv * yv), 0.0f);
         bx = fmaf(dx, f, bx);
         by = fmaf(dy, f, by);

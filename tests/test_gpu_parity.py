@@ -286,23 +286,6 @@ def test_fp32_traversal_forms_agree(monkeypatch):
     assert np.all(res["0"][0][b["mass"] <= 0] == 0.0)
 
 
-def test_fp32_records_follow_softening():
-    """The FP32 node records carry eps^2 (with theta) in their opening threshold:
-    a walk with another softening emits them again, so its forces are those of a
-    fresh simulation with that softening, bit for bit."""
-    b = ca.generate(ca.IC_MERGER, 20000, 5)
-    sim = sim_for(b, 0.01, 0.5, precision="fp32")
-    a1, _ = forces_on(sim, 0.5, "fp32")
-    sim.set_params(softening=0.02)
-    a2, c2 = forces_on(sim, 0.5, "fp32")
-    fresh = sim_for(b, 0.02, 0.5, precision="fp32")
-    a3, c3 = forces_on(fresh, 0.5, "fp32")
-    assert same_bits(a2, a3) and np.array_equal(c2, c3)
-    assert not same_bits(a1, a2)
-    sim.close()
-    fresh.close()
-
-
 def test_counts_come_from_explicit_walks():
     """Substep loops keep only the total; per-body counts of a loop's walk are
     refused rather than returned stale."""
