@@ -111,6 +111,7 @@ struct csg_ctx {
     int wcost_n = 0;        // number of waves wcost describes (0: none)
     int wave_order = 1;     // dispatch the wave-mask walk longest first
     int xcd_chunk = 16;     // walk tiles per XCD chunk (0: one contiguous range per XCD)
+    int walk_block_waves = 1;  // waves per block of the ordered walks
 };
 
 #define CSG_CHECK(expr)                                                                      \
@@ -199,7 +200,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->lcp, C1) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
               dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->tpref, (size_t)4 * CSG_TPB * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
-              dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, WO_XCDS * (size_t)WO_CAP(C / 64 + 1)) &&
+              dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, 2 * (size_t)(C / 64 + 1) + 4096) &&
               dalloc(c->xyzm, C) && dalloc(c->owner, 8 * C + 4096) && dalloc(c->rs_ctl, 1) &&
               dalloc(c->rs_status, (size_t)256 * blocks_for((long long)C, RS_TILE));
     if (!ok) {
@@ -334,6 +335,10 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     }
     if (const char *g = getenv("COSMOSIM_XCD_CHUNK")) c->xcd_chunk = atoi(g);
     if (const char *g = getenv("COSMOSIM_WAVE_ORDER")) c->wave_order = atoi(g) != 0;
+    if (const char *g = getenv("COSMOSIM_WALK_BLOCK_WAVES")) {
+        const int v = atoi(g);
+        if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) c->walk_block_waves = v;
+    }
     if (const char *g = getenv("COSMOSIM_WALK_GROUP32")) {
         const int v = atoi(g);
         if (v == 0 || v == 8 || v == 16 || v == 32 || v == 64) c->walk_group32 = v;
@@ -640,14 +645,14 @@ extern "C++" {
 // the XCD-chunked mapping (grid = nb8); the grouped FP32 forms ignore the order.
 template <int GS>
 static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, double th2, int grid,
-                           int nb8, TapeOut tape, WaveOrder wo) {
+                           int nb8, TapeOut tape, WaveOrder wo, int tpb) {
     hipStream_t st = c->stream;
     if (precision == COSMOSIM_PREC_FP64_STRICT)
-        k_walk<WALK_FP64_STRICT, GS><<<grid, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y,
+        k_walk<WALK_FP64_STRICT, GS><<<grid, tpb, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y,
             c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape,
             wo);
     else if (precision == COSMOSIM_PREC_FP64)
-        k_walk<WALK_FP64, GS><<<grid, CSG_TPB, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
+        k_walk<WALK_FP64, GS><<<grid, tpb, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
             c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape, wo);
     else
         k_walk32<GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes32, c->bp, c->idx, c->x, c->y, c->z, c->ax, c->ay,
@@ -656,8 +661,8 @@ static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, dou
 
 template <bool LC, bool TP, bool O32, bool TOT, bool SELF>
 static void launch_walk32w(csg_ctx *c, double G, double eps2, double th2, int grid, int nb8,
-                           TapeOut tape, WaveOrder wo) {
-    k_walk32w<LC, TP, O32, TOT, SELF><<<grid, CSG_TPB, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x,
+                           TapeOut tape, WaveOrder wo, int tpb) {
+    k_walk32w<LC, TP, O32, TOT, SELF><<<grid, tpb, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x,
         c->y, c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk,
         tape, wo);
 }
@@ -692,13 +697,18 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     // wave-mask FP32 walk and the FP64 walks record them)
     const int nw = blocks_for(n, 64);
     WaveOrder wo{nullptr, nw, grouped32 ? nullptr : c->wcost, 0};
-    int grid = nb8;
+    int grid = nb8, tpb = CSG_TPB;
     if (!grouped32 && c->wave_order && c->wcost_n == nw) {
-        const int capw = WO_CAP(nw);
-        k_wave_order<<<WO_XCDS, 1024, 0, st>>>(c->wcost, nw, capw, c->worder);
+        // blocks of gw waves (one by default: the finest balance; larger blocks,
+        // whose waves share a CU's scalar cache, measured slower).  The FP64
+        // walks' kernels are bounded at CSG_TPB threads.
+        const int gw = wave_mask32 ? c->walk_block_waves : min(c->walk_block_waves, CSG_TPB / 64);
+        const int capg = WO_CAP(blocks_for(nw, gw));
+        k_wave_order<<<WO_XCDS, 1024, 0, st>>>(c->wcost, nw, gw, capg, c->worder);
         wo.order = c->worder;
-        wo.slots = WO_XCDS * capw;
-        grid = blocks_for(wo.slots, CSG_TPB / 64);
+        wo.slots = WO_XCDS * capg * gw;
+        grid = WO_XCDS * capg;
+        tpb = 64 * gw;
     }
     if (!grouped32) c->wcost_n = nw;
     if (wave_mask32) {
@@ -709,21 +719,21 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         if (v < 2 && !count_total) {
             // softened: the own leaf adds exactly 0, so it needs no exclusion
             if (off32 && eps2 > 0.0)
-                launch_walk32w<false, false, true, false, false>(c, G, eps2, th2, grid, nb8, tape, wo);
+                launch_walk32w<false, false, true, false, false>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
             else if (off32)
-                launch_walk32w<false, false, true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo);
+                launch_walk32w<false, false, true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
             else
-                launch_walk32w<false, false, false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo);
+                launch_walk32w<false, false, false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
         } else {
             switch (v) {
-            case 0: launch_walk32w<false, false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 1: launch_walk32w<false, false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 2: launch_walk32w<false, true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 3: launch_walk32w<false, true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 4: launch_walk32w<true, false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 5: launch_walk32w<true, false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 6: launch_walk32w<true, true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
-            case 7: launch_walk32w<true, true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo); break;
+            case 0: launch_walk32w<false, false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 1: launch_walk32w<false, false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 2: launch_walk32w<false, true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 3: launch_walk32w<false, true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 4: launch_walk32w<true, false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 5: launch_walk32w<true, false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 6: launch_walk32w<true, true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 7: launch_walk32w<true, true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
             }
         }
         CSG_LAUNCHED();
@@ -734,10 +744,10 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     const int gs = tape.rec ? 64 : (precision == COSMOSIM_PREC_FP32 ? c->walk_group32 : c->walk_group);
     c->counts_valid = 1;
     switch (gs) {
-    case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, grid, nb8, tape, wo); break;
-    case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, grid, nb8, tape, wo); break;
-    case 32: launch_walk_gs<32>(c, precision, G, eps2, th2, grid, nb8, tape, wo); break;
-    default: launch_walk_gs<64>(c, precision, G, eps2, th2, grid, nb8, tape, wo); break;
+    case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+    case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+    case 32: launch_walk_gs<32>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+    default: launch_walk_gs<64>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
     }
     CSG_LAUNCHED();
     return COSMOSIM_OK;

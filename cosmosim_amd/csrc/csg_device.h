@@ -1421,7 +1421,7 @@ __device__ __forceinline__ int xcd_tile(int grid, int chunk) {
 // mapping.  -1: a slot past the order's end (no wave).
 __device__ __forceinline__ int walk_wave(int nb8, int chunk, const WaveOrder &wo) {
     if (wo.order) {
-        const int slot = (int)blockIdx.x * (CSG_TPB / 64) + (int)(threadIdx.x >> 6);
+        const int slot = (int)blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
         if (slot >= wo.slots) return -1;
         const int w = wo.order[slot];
         return (w < 0 || w >= wo.n) ? -1 : w;
@@ -1663,34 +1663,47 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
 }
 
 // Dispatch order of the walks, from each wave's node-step count in the previous
-// walk.  Block b of a walk grid runs on XCD b % 8 (workgroups are dealt out to
-// the XCDs round-robin), and wave slot s = 4 b + k of the grid walks wave
-// order[s].  The waves are cut into 8 contiguous ranges of sorted bodies, one
-// per XCD, of about equal total step count: each XCD's L2 then serves one
-// eighth of the space (the near-field nodes its waves share).  Each range is
-// dispatched longest first (longest processing time first: long traversals
-// start early and short ones fill the end, where an XCD otherwise drains with
-// few waves per SIMD).  A boundary moves from the equal-count one by at most
-// nw / 64 waves, so a range holds at most WO_CAP(nw) waves; the slots past a
-// range's end hold -1 (no wave).  Block r of 8 (1024 threads) orders range r
-// by a counting sort on the step count quantized to 4096 levels, descending;
+// walk.  The walk runs blocks of gw waves (gw consecutive waves: 64 gw sorted,
+// spatially adjacent bodies that share a CU and its scalar cache); block b of
+// the grid runs on XCD b % 8 (workgroups are dealt out to the XCDs
+// round-robin), and wave slot s of the grid walks wave order[s].  The blocks'
+// groups are cut into 8 contiguous ranges of sorted bodies, one per XCD, of
+// about equal total step count: each XCD's L2 then serves one eighth of the
+// space (the near-field nodes its waves share).  Each range is dispatched
+// longest first (longest processing time first: long traversals start early
+// and short ones fill the end, where an XCD otherwise drains with few waves per
+// SIMD).  A boundary moves from the equal-count one by at most ng / 64 groups,
+// so a range holds at most capg = WO_CAP(ng) groups; the slots past a range's
+// end hold -1 (no wave).  Block r of 8 (1024 threads) orders range r by a
+// counting sort on the group's step count quantized to 4096 levels, descending;
 // the order inside a level is whatever the LDS atomics give, which changes only
 // the timing.  Every block computes the same boundaries.
 #define WO_XCDS 8
-#define WO_CAP(nw) ((((5 * (nw)) / 32 + 8) + 3) & ~3)
-__global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cost, int nw,
-                                                     int capw, int *__restrict__ order) {
+#define WO_CAP(ng) ((((5 * (ng)) / 32 + 8) + 3) & ~3)
+__global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cost, int nw, int gw,
+                                                     int capg, int *__restrict__ order) {
     __shared__ int hist[4096];
     __shared__ long long wsum[16];
     __shared__ int bnd[WO_XCDS + 1];
     __shared__ int part[16];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int r = blockIdx.x;
-    // chunk sums of the step counts (thread t: waves [c0, c1)) and their prefix
-    const int per = (nw + 1023) / 1024;
-    const int c0 = min(nw, t * per), c1 = min(nw, c0 + per);
+    const int ng = (nw + gw - 1) / gw;
+    // step count of group g (stale or garbage counts are clamped: any value
+    // still yields a permutation)
+    auto gcost = [&](int g) {
+        int c = 0;
+        for (int k = 0; k < gw; ++k) {
+            const int w = g * gw + k;
+            if (w < nw) c += min(max(cost[w], 0), 1 << 24);
+        }
+        return c;
+    };
+    // chunk sums of the group counts (thread t: groups [c0, c1)) and their prefix
+    const int per = (ng + 1023) / 1024;
+    const int c0 = min(ng, t * per), c1 = min(ng, c0 + per);
     long long s = 0;
-    for (int i = c0; i < c1; ++i) s += max(cost[i], 0);
+    for (int g = c0; g < c1; ++g) s += gcost(g);
     long long inc = s;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -1698,7 +1711,7 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
         if (lane >= o) inc += u;
     }
     if (lane == 63) wsum[wid] = inc;
-    if (t <= WO_XCDS) bnd[t] = (int)(((long long)t * nw) / WO_XCDS);   // equal counts
+    if (t <= WO_XCDS) bnd[t] = (int)(((long long)t * ng) / WO_XCDS);   // equal counts
     __syncthreads();
     long long ex = inc - s, T = 0;
 #pragma unroll
@@ -1706,29 +1719,29 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
         if (k < wid) ex += wsum[k];
         T += wsum[k];
     }
-    // boundary k: one past the wave whose prefix reaches k T / 8 (one chunk holds it)
+    // boundary k: one past the group whose prefix reaches k T / 8 (one chunk holds it)
     for (int k = 1; k < WO_XCDS; ++k) {
         const long long target = (T * k) / WO_XCDS;
         if (target > 0 && ex < target && ex + s >= target) {
             long long run = ex;
-            int i = c0;
-            while (run < target) run += max(cost[i++], 0);
-            bnd[k] = i;
+            int g = c0;
+            while (run < target) run += gcost(g++);
+            bnd[k] = g;
         }
     }
     __syncthreads();
     if (t == 0) {
         for (int k = 1; k < WO_XCDS; ++k) {
-            const int mid = (int)(((long long)k * nw) / WO_XCDS), slack = nw / 64;
+            const int mid = (int)(((long long)k * ng) / WO_XCDS), slack = ng / 64;
             const int bk = min(max(bnd[k], mid - slack), mid + slack);
-            bnd[k] = min(max(bk, bnd[k - 1]), nw);
+            bnd[k] = min(max(bk, bnd[k - 1]), ng);
         }
     }
     for (int i = t; i < 4096; i += 1024) hist[i] = 0;
     __syncthreads();
     const int lo = bnd[r], hi = bnd[r + 1], m = hi - lo;
     int mx = 0;
-    for (int i = lo + t; i < hi; i += 1024) mx = max(mx, cost[i]);
+    for (int g = lo + t; g < hi; g += 1024) mx = max(mx, gcost(g));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) part[wid] = mx;
@@ -1738,10 +1751,8 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
     for (int k = 0; k < 16; ++k) mx = max(mx, part[k]);
     int shift = 0;
     while ((mx >> shift) >= 4096) ++shift;
-    // any count (stale ones included) lands in a bin: the range's slots receive
-    // a permutation of its waves
-    auto bin = [&](int c) { return 4095 - (max(c, 0) >> shift); };
-    for (int i = lo + t; i < hi; i += 1024) atomicAdd(&hist[bin(cost[i])], 1);
+    auto bin = [&](int c) { return 4095 - (c >> shift); };
+    for (int g = lo + t; g < hi; g += 1024) atomicAdd(&hist[bin(gcost(g))], 1);
     __syncthreads();
     // exclusive scan of the 4096 bins: 4 per thread, then across threads
     int h[4], sum = 0;
@@ -1762,13 +1773,18 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
 #pragma unroll
     for (int k = 0; k < 4; ++k) { hist[4 * t + k] = run; run += h[k]; }
     __syncthreads();
-    // position j of range r -> slot 4 (8 (j / 4) + r) + j % 4 (block 8 (j / 4) + r)
-    auto slot = [&](int j) { return 4 * (WO_XCDS * (j >> 2) + r) + (j & 3); };
-    for (int i = lo + t; i < hi; i += 1024) {
-        const int j = atomicAdd(&hist[bin(cost[i])], 1);
-        if (j < capw) order[slot(j)] = i;
+    // position j of range r -> grid block 8 j + r, whose gw slots take the
+    // group's waves in order
+    for (int g = lo + t; g < hi; g += 1024) {
+        const int j = atomicAdd(&hist[bin(gcost(g))], 1);
+        if (j < capg)
+            for (int k = 0; k < gw; ++k) {
+                const int w = g * gw + k;
+                order[(WO_XCDS * j + r) * gw + k] = w < nw ? w : -1;
+            }
     }
-    for (int j = m + t; j < capw; j += 1024) order[slot(j)] = -1;
+    for (int j = m + t; j < capg; j += 1024)
+        for (int k = 0; k < gw; ++k) order[(WO_XCDS * j + r) * gw + k] = -1;
 }
 
 // Lane select on a wave mask held in SGPRs: m's bit for this lane ? t : f.
@@ -1812,7 +1828,7 @@ __device__ __forceinline__ int lane_sel_i32(unsigned long long m, int t, int f) 
 // depend on the dispatch order.  With WaveOrder::cost set, every wave stores its
 // node-step count there.
 template <bool LANE_COUNTS, bool TAPING, bool OFF32, bool TOTAL, bool SELF>
-__global__ __launch_bounds__(CSG_TPB) void k_walk32w(
+__global__ __launch_bounds__(1024) void k_walk32w(
     const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
     const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
