@@ -688,7 +688,8 @@ static int forces_frozen_dev(csg_ctx *c) {
     int s = read_params(c);
     if (s) return s;
     const int nl = c->hbp->n_live, nn = c->hbp->n_nodes;
-    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, nullptr);
+    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, nullptr,
+                                                            nullptr);
     if (nl == 0) return COSMOSIM_OK;
     k_gather_sorted<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, c->x, c->y, c->z, c->m,
                                                                 c->bp, c->xs, c->ys, c->zs, c->ms);
@@ -788,7 +789,8 @@ static int frozen_forces(csg_ctx *c, int e) {
     BuildParams hb;
     CSG_CHECK(hipMemcpyAsync(&hb, f.bp, sizeof(BuildParams), hipMemcpyDeviceToHost, st));
     CSG_CHECK(hipStreamSynchronize(st));
-    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, nullptr);
+    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, nullptr,
+                                                            nullptr);
     const int nl = hb.n_live, nn = hb.n_nodes;
     if (nl == 0) return COSMOSIM_OK;
     k_gather_sorted<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(f.idx, c->x, c->y, c->z, c->m, f.bp,

@@ -545,7 +545,7 @@ int csg_set_state(csg_ctx *c, const double *pos, const double *vel, const double
     return COSMOSIM_OK;
 }
 
-static int launch_build(csg_ctx *c, bool with64 = true);
+static int launch_build(csg_ctx *c, bool with64 = true, int bbox = 0);
 
 // The FP64 node records of the current tree, re-emitted by a full build when the
 // last build wrote only the FP32 records (same positions, so the same tree).
@@ -554,10 +554,25 @@ static int ensure_nodes64(csg_ctx *c, int precision) {
     return launch_build(c, true);
 }
 
-static int launch_kick_drift(csg_ctx *c, double dt, int kicks) {
+// Blocks of the bounding-box partials (k_bbox_partial's grid).
+static int bbox_blocks(int n) {
+    return n > 0 ? (blocks_for(n, CSG_TPB) < CSG_BBOX_BLOCKS ? blocks_for(n, CSG_TPB) : CSG_BBOX_BLOCKS)
+                 : 1;
+}
+
+// bbox (may be null): also leave the drifted positions' bounding-box partials
+// in c->bbox_part and their block count in *bbox, for the build that follows.
+static int launch_kick_drift(csg_ctx *c, double dt, int kicks, int *bbox = nullptr) {
+    if (bbox) *bbox = 0;
     if (c->n == 0) return COSMOSIM_OK;
-    k_kick_drift<<<blocks_for(c->n, CSG_TPB), CSG_TPB, 0, c->stream>>>(
-        c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->m, c->n, dt, kicks);
+    if (bbox) {
+        *bbox = bbox_blocks(c->n);
+        k_kick_drift_bbox<<<*bbox, CSG_TPB, 0, c->stream>>>(c->x, c->y, c->z, c->vx, c->vy, c->vz,
+            c->ax, c->ay, c->az, c->m, c->n, dt, kicks, c->bbox_part);
+    } else {
+        k_kick_drift<<<blocks_for(c->n, CSG_TPB), CSG_TPB, 0, c->stream>>>(
+            c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->m, c->n, dt, kicks);
+    }
     CSG_LAUNCHED();
     return COSMOSIM_OK;
 }
@@ -573,13 +588,13 @@ static int launch_kick(csg_ctx *c, double dt) {
 // Whole build on the stream; no host synchronisation.
 // with64: also write the FP64 node records (the FP64 walks, the adjoint and the
 // tree export read them; the FP32 walk reads only the 40-byte records).
-static int launch_build(csg_ctx *c, bool with64) {
+// bbox: the block count of bounding-box partials of the current positions that
+// the preceding kick-drift left in c->bbox_part (0: compute them here).
+static int launch_build(csg_ctx *c, bool with64, int bbox) {
     const int n = c->n;
     hipStream_t st = c->stream;
-    const int nbb = n > 0 ? (blocks_for(n, CSG_TPB) < CSG_BBOX_BLOCKS ? blocks_for(n, CSG_TPB)
-                                                                       : CSG_BBOX_BLOCKS)
-                          : 1;
-    k_bbox_partial<<<nbb, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bbox_part);
+    const int nbb = bbox > 0 ? bbox : bbox_blocks(n);
+    if (bbox <= 0) k_bbox_partial<<<nbb, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bbox_part);
     k_bbox_final<<<1, CSG_TPB, 0, st>>>(c->bbox_part, nbb, c->bp, c->node_cap, c->rs_ctl);
     CSG_LAUNCHED();
     if (n > 0) {
@@ -683,9 +698,12 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         const int s = launch_build(c, c->nodes64_valid != 0);
         if (s) return s;
     }
-    k_set_u64<<<1, 1, 0, st>>>(c->itotal, 0ull);
-    if (n == 0) return COSMOSIM_OK;
-    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, c->icount);
+    if (n == 0) {
+        k_set_u64<<<1, 1, 0, st>>>(c->itotal, 0ull);
+        return COSMOSIM_OK;
+    }
+    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, c->icount,
+                                                            c->itotal);
     int nb8 = blocks_for(n, CSG_TPB);
     const int gran = c->xcd_chunk > 0 ? 8 * c->xcd_chunk : 8;
     nb8 = (nb8 + gran - 1) / gran * gran;
@@ -818,13 +836,13 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
     c->theta_hint = theta;
     if (precision < 0 || precision > 2 || k < 0) return COSMOSIM_E_ARG;
     if (k == 0) return COSMOSIM_OK;
-    int s;
+    int s, bbox = 0;
     timer_start(c);
-    if ((s = launch_kick_drift(c, dt, 1))) return s;
+    if ((s = launch_kick_drift(c, dt, 1, &bbox))) return s;
     timer_stop(c, PH_KDK);
     for (int sub = 0; sub < k; ++sub) {
         timer_start(c);
-        if ((s = launch_build(c, precision != COSMOSIM_PREC_FP32))) return s;
+        if ((s = launch_build(c, precision != COSMOSIM_PREC_FP32, bbox))) return s;
         timer_stop(c, PH_BUILD);
         timer_start(c);
         if ((s = launch_walk(c, precision, G, softening, theta, TapeOut{}, false, sub + 1 == k)))
@@ -832,7 +850,8 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
         timer_stop(c, PH_WALK);
         timer_start(c);
         // end-of-substep half kick fused with the next substep's kick + drift
-        s = (sub + 1 < k) ? launch_kick_drift(c, dt, 2) : launch_kick(c, dt);
+        // (which also bounds the drifted positions for the next build)
+        s = (sub + 1 < k) ? launch_kick_drift(c, dt, 2, &bbox) : launch_kick(c, dt);
         if (s) return s;
         timer_stop(c, PH_KDK);
     }

@@ -1931,8 +1931,10 @@ __global__ __launch_bounds__(CSG_TPB) void k_zero_dead(const double *__restrict_
                                                        double *__restrict__ ax,
                                                        double *__restrict__ ay,
                                                        double *__restrict__ az,
-                                                       int *__restrict__ icount) {
+                                                       int *__restrict__ icount,
+                                                       unsigned long long *__restrict__ total) {
     const int i = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (i == 0 && total) *total = 0ull;   // the walk's interaction total starts here
     if (i >= n) return;
     if (!live_mass(m[i])) {
         ax[i] = 0.0; ay[i] = 0.0; az[i] = 0.0;
@@ -1966,6 +1968,53 @@ __global__ __launch_bounds__(CSG_TPB) void k_kick_drift(double *__restrict__ x,
     x[i] = x[i] + u * dt;
     y[i] = y[i] + v * dt;
     z[i] = z[i] + w * dt;
+}
+
+// k_kick_drift over a grid-stride loop that also leaves k_bbox_partial's
+// per-block bounds of the drifted positions in part (same layout), so that the
+// build that follows starts from k_bbox_final.
+__global__ __launch_bounds__(CSG_TPB) void k_kick_drift_bbox(
+    double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+    double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
+    const double *__restrict__ ax, const double *__restrict__ ay, const double *__restrict__ az,
+    const double *__restrict__ m, int n, double dt, int kicks, double *__restrict__ part) {
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double cnt = 0.0;
+    for (int i = blockIdx.x * CSG_TPB + threadIdx.x; i < n; i += gridDim.x * CSG_TPB) {
+        if (!live_mass(m[i])) continue;
+        double u = vx[i], v = vy[i], w = vz[i];
+        const double hx = 0.5 * ax[i], hy = 0.5 * ay[i], hz = 0.5 * az[i];
+        for (int k = 0; k < kicks; ++k) {
+            u = u + hx * dt;
+            v = v + hy * dt;
+            w = w + hz * dt;
+        }
+        vx[i] = u; vy[i] = v; vz[i] = w;
+        const double p[3] = {x[i] + u * dt, y[i] + v * dt, z[i] + w * dt};
+        x[i] = p[0]; y[i] = p[1]; z[i] = p[2];
+        cnt += 1.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (p[a] < lo[a]) lo[a] = p[a];
+            if (p[a] > hi[a]) hi[a] = p[a];
+        }
+    }
+    __shared__ double s[7][CSG_TPB];
+    for (int a = 0; a < 3; ++a) { s[a][threadIdx.x] = lo[a]; s[3 + a][threadIdx.x] = hi[a]; }
+    s[6][threadIdx.x] = cnt;
+    __syncthreads();
+    for (int st = CSG_TPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const int o = threadIdx.x + st;
+            for (int a = 0; a < 3; ++a) {
+                if (s[a][o] < s[a][threadIdx.x]) s[a][threadIdx.x] = s[a][o];
+                if (s[3 + a][o] > s[3 + a][threadIdx.x]) s[3 + a][threadIdx.x] = s[3 + a][o];
+            }
+            s[6][threadIdx.x] += s[6][o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) part[blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x][0];
 }
 
 // v += (0.5*a)*dt for live bodies (src/integrator.c:45-51).
