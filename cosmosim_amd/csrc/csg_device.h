@@ -1672,23 +1672,28 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
 // space (the near-field nodes its waves share).  Each range is dispatched
 // longest first (longest processing time first: long traversals start early
 // and short ones fill the end, where an XCD otherwise drains with few waves per
-// SIMD).  A boundary moves from the equal-count one by at most ng / 64 groups,
-// so a range holds at most capg = WO_CAP(ng) groups; the slots past a range's
-// end hold -1 (no wave).  Block r of 8 (1024 threads) orders range r by a
+// SIMD).  With split > 1 an XCD's range is cut further into split sub-ranges
+// of equal step count, dispatched one after the other (each longest first), so
+// that the waves running together on an XCD span a smaller part of the space.
+// A boundary moves from the equal-count one by at most ng / 64 groups, so an
+// XCD's range holds at most capg = WO_CAP(ng) groups; the slots past its end
+// hold -1 (no wave).  Block r of 8 (1024 threads) orders XCD r's range by a
 // counting sort on the group's step count quantized to 4096 levels, descending;
 // the order inside a level is whatever the LDS atomics give, which changes only
 // the timing.  Every block computes the same boundaries.
 #define WO_XCDS 8
+#define WO_MAX_SPLIT 4
 #define WO_CAP(ng) ((((5 * (ng)) / 32 + 8) + 3) & ~3)
 __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cost, int nw, int gw,
-                                                     int capg, int *__restrict__ order) {
+                                                     int split, int capg, int *__restrict__ order) {
     __shared__ int hist[4096];
     __shared__ long long wsum[16];
-    __shared__ int bnd[WO_XCDS + 1];
+    __shared__ int bnd[WO_XCDS * WO_MAX_SPLIT + 1];
     __shared__ int part[16];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int r = blockIdx.x;
     const int ng = (nw + gw - 1) / gw;
+    const int R = WO_XCDS * split;   // ranges: XCD x takes ranges x split .. x split + split - 1
     // step count of group g (stale or garbage counts are clamped: any value
     // still yields a permutation)
     auto gcost = [&](int g) {
@@ -1711,7 +1716,7 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
         if (lane >= o) inc += u;
     }
     if (lane == 63) wsum[wid] = inc;
-    if (t <= WO_XCDS) bnd[t] = (int)(((long long)t * ng) / WO_XCDS);   // equal counts
+    if (t <= R) bnd[t] = (int)(((long long)t * ng) / R);   // equal counts
     __syncthreads();
     long long ex = inc - s, T = 0;
 #pragma unroll
@@ -1719,9 +1724,9 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
         if (k < wid) ex += wsum[k];
         T += wsum[k];
     }
-    // boundary k: one past the group whose prefix reaches k T / 8 (one chunk holds it)
-    for (int k = 1; k < WO_XCDS; ++k) {
-        const long long target = (T * k) / WO_XCDS;
+    // boundary k: one past the group whose prefix reaches k T / R (one chunk holds it)
+    for (int k = 1; k < R; ++k) {
+        const long long target = (T * k) / R;
         if (target > 0 && ex < target && ex + s >= target) {
             long long run = ex;
             int g = c0;
@@ -1731,60 +1736,63 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
     }
     __syncthreads();
     if (t == 0) {
-        for (int k = 1; k < WO_XCDS; ++k) {
-            const int mid = (int)(((long long)k * ng) / WO_XCDS), slack = ng / 64;
+        for (int k = 1; k < R; ++k) {
+            const int mid = (int)(((long long)k * ng) / R), slack = ng / 64;
             const int bk = min(max(bnd[k], mid - slack), mid + slack);
             bnd[k] = min(max(bk, bnd[k - 1]), ng);
         }
     }
-    for (int i = t; i < 4096; i += 1024) hist[i] = 0;
     __syncthreads();
-    const int lo = bnd[r], hi = bnd[r + 1], m = hi - lo;
-    int mx = 0;
-    for (int g = lo + t; g < hi; g += 1024) mx = max(mx, gcost(g));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
-    if (lane == 0) part[wid] = mx;
-    __syncthreads();
-    mx = 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) mx = max(mx, part[k]);
-    int shift = 0;
-    while ((mx >> shift) >= 4096) ++shift;
-    auto bin = [&](int c) { return 4095 - (c >> shift); };
-    for (int g = lo + t; g < hi; g += 1024) atomicAdd(&hist[bin(gcost(g))], 1);
-    __syncthreads();
-    // exclusive scan of the 4096 bins: 4 per thread, then across threads
-    int h[4], sum = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) { h[k] = hist[4 * t + k]; sum += h[k]; }
-    int incl = sum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += u;
-    }
-    __syncthreads();
-    if (lane == 63) part[wid] = incl;
-    __syncthreads();
-    int base0 = 0;
-    for (int k = 0; k < wid; ++k) base0 += part[k];
-    int run = base0 + incl - sum;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) { hist[4 * t + k] = run; run += h[k]; }
-    __syncthreads();
-    // position j of range r -> grid block 8 j + r, whose gw slots take the
+    const int xlo = bnd[r * split], xhi = bnd[r * split + split];
+    // position j of XCD r's list -> grid block 8 j + r, whose gw slots take the
     // group's waves in order
-    for (int g = lo + t; g < hi; g += 1024) {
-        const int j = atomicAdd(&hist[bin(gcost(g))], 1);
+    auto place = [&](int j, int g) {
         if (j < capg)
             for (int k = 0; k < gw; ++k) {
                 const int w = g * gw + k;
-                order[(WO_XCDS * j + r) * gw + k] = w < nw ? w : -1;
+                order[(WO_XCDS * j + r) * gw + k] = (g >= 0 && w < nw) ? w : -1;
             }
+    };
+    for (int q = 0; q < split; ++q) {
+        const int lo = bnd[r * split + q], hi = bnd[r * split + q + 1];
+        for (int i = t; i < 4096; i += 1024) hist[i] = 0;
+        int mx = 0;
+        for (int g = lo + t; g < hi; g += 1024) mx = max(mx, gcost(g));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+        if (lane == 0) part[wid] = mx;
+        __syncthreads();
+        mx = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) mx = max(mx, part[k]);
+        int shift = 0;
+        while ((mx >> shift) >= 4096) ++shift;
+        auto bin = [&](int c) { return 4095 - (c >> shift); };
+        for (int g = lo + t; g < hi; g += 1024) atomicAdd(&hist[bin(gcost(g))], 1);
+        __syncthreads();
+        // exclusive scan of the 4096 bins: 4 per thread, then across threads
+        int h[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { h[k] = hist[4 * t + k]; sum += h[k]; }
+        int incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        __syncthreads();
+        if (lane == 63) part[wid] = incl;
+        __syncthreads();
+        int base0 = lo - xlo;   // the sub-range's first position in the XCD's list
+        for (int k = 0; k < wid; ++k) base0 += part[k];
+        int run = base0 + incl - sum;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { hist[4 * t + k] = run; run += h[k]; }
+        __syncthreads();
+        for (int g = lo + t; g < hi; g += 1024) place(atomicAdd(&hist[bin(gcost(g))], 1), g);
+        __syncthreads();
     }
-    for (int j = m + t; j < capg; j += 1024)
-        for (int k = 0; k < gw; ++k) order[(WO_XCDS * j + r) * gw + k] = -1;
+    for (int j = (xhi - xlo) + t; j < capg; j += 1024) place(j, -1);
 }
 
 // Lane select on a wave mask held in SGPRs: m's bit for this lane ? t : f.
