@@ -10,7 +10,7 @@
 //   xbar_k += (m_k / M_J) cbar_J for every node J containing k (c_J = sum m x / M)
 // with T = G M r^-3 (I - 3 d d^T / r^2), r^2 = |d|^2 + eps^2.
 //   1. k_adj_walk replays each wave's tape: per-lane -T w, and per tape record the
-//      wave's lane sum of T w (a fixed DPP reduction tree).
+//      wave's lane sum of T w (a fixed order through LDS).
 //   2. The records are stable-radix-sorted by node id: the transposed
 //      interaction lists.  k_seg_short / k_seg_long sum each node's records in a
 //      fixed order: cbar without atomics on the sums, reproducibly.
@@ -61,45 +61,29 @@ __global__ __launch_bounds__(CSG_TPB) void k_tape_scan(const int *__restrict__ c
     if (threadIdx.x == 0) *total = carry;
 }
 
-// One DPP move of an FP64 value (as two 32-bit halves); lanes the control does
-// not write, or whose source lies outside the row, read 0.
-template <int CTRL, int ROWS, int BANKS>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, BANKS, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, BANKS, true);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-// Sum over the 64 lanes of a full wavefront, left in lane 63: shifts by 1, 2, 4, 8
-// inside each row of 16 lanes, then row 0 into row 1 and row 2 into row 3
-// (row_bcast:15), then rows 0-1 into rows 2-3 (row_bcast:31).  A fixed tree, so
-// the rounding is the same on every call; no LDS traffic.
-__device__ __forceinline__ double wave_sum63(double s) {
-    s += dpp_f64<0x111, 0xF, 0xF>(s);  // row_shr:1
-    s += dpp_f64<0x112, 0xF, 0xF>(s);  // row_shr:2
-    s += dpp_f64<0x114, 0xF, 0xE>(s);  // row_shr:4, banks 1-3
-    s += dpp_f64<0x118, 0xF, 0xC>(s);  // row_shr:8, banks 2-3
-    s += dpp_f64<0x142, 0xA, 0xF>(s);  // row_bcast:15 into rows 1 and 3
-    s += dpp_f64<0x143, 0xC, 0xF>(s);  // row_bcast:31 into rows 2 and 3
-    return s;
-}
-
-// Step 1: per-lane -T w and per-record lane sums (one wave per 64 sorted bodies).
-// The wave index is made uniform, so tape records and node records come by
-// scalar loads, four records per round (their latencies overlap).
-__global__ __launch_bounds__(CSG_TPB) void k_adj_walk(
+// Step 1: per-lane -T w and per-record lane sums (one wave per 64 sorted bodies,
+// one wave per workgroup).  The wave index is uniform, so tape records and node
+// records come by scalar loads, four records per round (their latencies
+// overlap).  The lane sums of a round go through LDS, two records (six sums) at
+// a time: every lane stores its six values, lane 8 s + p adds the values of
+// lanes p, p + 8, ..., p + 56 for sum s in that order, and the eight partials of
+// a sum are combined by a fixed butterfly (xor 4, 2, 1).  The order is the same
+// on every call, so the sums are reproducible.  Rows are padded to 72 values so
+// that the six sums' reads fall in different banks.
+#define ADJ_WAVE_TPB 64
+__global__ __launch_bounds__(ADJ_WAVE_TPB) void k_adj_walk(
     const WNode *__restrict__ nodes, const TapeRec *__restrict__ tape,
     const int *__restrict__ tcount, const int *__restrict__ toff, int cap,
     const BuildParams *__restrict__ bp, const double *__restrict__ xs,
     const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ws,
     double G, double eps2, double *__restrict__ xbar_s, uint64_t *__restrict__ rkey,
     int *__restrict__ rval, double *__restrict__ rsum) {
+    __shared__ double sums[6][72];
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * CSG_TPB + threadIdx.x) >> 6));
+    const int wave = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
     const int nl = bp->n_live;
     const int b0 = wave * 64;
-    if (b0 >= nl) return;
+    if (b0 >= nl) return;   // the whole workgroup (one wave) leaves together
     const int b = b0 + lane;
     const bool act = b < nl;
     const double x = act ? xs[b] : 0.0, y = act ? ys[b] : 0.0, z = act ? zs[b] : 0.0;
@@ -108,6 +92,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_adj_walk(
     double ax = 0.0, ay = 0.0, az = 0.0;
     const int cnt = tcount[wave], off = toff[wave];
     const TapeRec *t = tape + (size_t)wave * cap;
+    const int s_of = lane >> 3, part = lane & 7;   // this lane's sum and its part of it
     for (int r0 = 0; r0 < cnt; r0 += 4) {
         TapeRec rr[4];
         double cm[4][4];
@@ -118,20 +103,44 @@ __global__ __launch_bounds__(CSG_TPB) void k_adj_walk(
             const WNode *np = nodes + rr[k].node;
             cm[k][0] = np->cx; cm[k][1] = np->cy; cm[k][2] = np->cz; cm[k][3] = np->m;
         }
+        double g[4][3];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (r0 + k >= cnt) break;
             double gx = 0.0, gy = 0.0, gz = 0.0;
-            if ((rr[k].imask >> lane) & 1ull)
+            if (r0 + k < cnt && ((rr[k].imask >> lane) & 1ull))
                 tidal_apply(cm[k][0] - x, cm[k][1] - y, cm[k][2] - z, wx, wy, wz, G * cm[k][3],
                             eps2, gx, gy, gz);
             ax -= gx; ay -= gy; az -= gz;
-            gx = wave_sum63(gx); gy = wave_sum63(gy); gz = wave_sum63(gz);
-            if (lane == 63) {
-                const size_t q = (size_t)off + r0 + k;
-                rkey[q] = (uint64_t)rr[k].node;
-                rval[q] = (int)q;
-                rsum[3 * q] = gx; rsum[3 * q + 1] = gy; rsum[3 * q + 2] = gz;
+            g[k][0] = gx; g[k][1] = gy; g[k][2] = gz;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (r0 + 2 * h >= cnt) break;
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sums[3 * kk + c][lane] = g[2 * h + kk][c];
+            __syncthreads();
+            double acc = 0.0;
+            if (s_of < 6) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) acc += sums[s_of][part + 8 * i];
+            }
+            acc += __shfl_xor(acc, 4, 8);
+            acc += __shfl_xor(acc, 2, 8);
+            acc += __shfl_xor(acc, 1, 8);
+            __syncthreads();   // every read is done before the next stores
+            if (s_of < 6 && part == 0) {
+                const int kk = s_of / 3, c = s_of - 3 * kk;
+                const int rec = r0 + 2 * h + kk;
+                if (rec < cnt) {
+                    const size_t q = (size_t)off + rec;
+                    rsum[3 * q + c] = acc;
+                    if (c == 0) {
+                        rkey[q] = (uint64_t)rr[2 * h + kk].node;
+                        rval[q] = (int)q;
+                    }
+                }
             }
         }
     }
@@ -635,7 +644,7 @@ static int force_vjp_dev(csg_ctx *c, const double *w_dev, double *xbar_dev, int 
     CSG_CHECK(hipMemcpyAsync(&R, c->tape_total, sizeof(int), hipMemcpyDeviceToHost, st));
     CSG_CHECK(hipStreamSynchronize(st));
     if ((s = adj_reserve_records(c, (size_t)R + 1))) return s;
-    k_adj_walk<<<blocks_for((long long)nw * 64, CSG_TPB), CSG_TPB, 0, st>>>(
+    k_adj_walk<<<nw, ADJ_WAVE_TPB, 0, st>>>(
         c->nodes, c->tape, c->tcount, c->toff, c->tape_cap, c->bp, c->xs, c->ys, c->zs, c->ws,
         c->tape_G, c->tape_eps2, c->xbar_s, c->rkey, c->rval, c->rsum);
     CSG_LAUNCHED();
