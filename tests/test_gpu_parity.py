@@ -309,6 +309,24 @@ def test_walk_dispatch_settings_agree(monkeypatch, precision):
         assert np.array_equal(c, c0), key
 
 
+@pytest.mark.parametrize("n", [5, 130, 1000])
+def test_walk_dispatch_small_trees(monkeypatch, n):
+    """Fewer waves than XCD ranges and sub-ranges: most ranges are empty, and the
+    ordered walks still visit every body once (bit-identical to the unordered
+    dispatch, FP32 and FP64)."""
+    b = ca.generate(ca.IC_MERGER, max(n, 4), 2)[:n].copy()
+    for precision in ["fp32", "fp64"]:
+        res = []
+        for order in ["1", "0"]:
+            monkeypatch.setenv("COSMOSIM_WAVE_ORDER", order)
+            sim = sim_for(b, 0.01, 0.5, precision=precision, dt=1e-3)
+            sim.advance(2)
+            res.append(forces_on(sim, 0.5, precision))
+            sim.close()
+        assert same_bits(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+        assert np.all(res[0][1] > 0) or n == 1
+
+
 def test_counts_come_from_explicit_walks():
     """Substep loops keep only the total; per-body counts of a loop's walk are
     refused rather than returned stale."""
