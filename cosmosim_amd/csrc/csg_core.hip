@@ -83,7 +83,12 @@ struct csg_ctx {
     double rec_dt = 0, rec_G = 1, rec_eps = 0, rec_theta = 0;
     double theta_hint = 0.5;  // opening angle of the next walks (set by every walk entry)
     double th32 = -1.0;       // opening angle folded into the FP32 records (-1: none)
-    int walk_group = 8;     // FP64 walks: lanes per traversal group (8, 16, 32 or 64)
+    // FP64 walks: 0 = wave-mask traversal, else lanes per traversal group of the
+    // grouped walk (8, 16, 32 or 64); -1 (default) = wave-mask for FP64 and
+    // groups of 8 for FP64-strict, whose correctly rounded divide and square
+    // root make each step dear (1M-body merger walk: FP64 3.36 ms wave-mask vs
+    // 3.71 grouped; strict 5.42 vs 4.45)
+    int walk_group = -1;
     int walk_group32 = 0;   // FP32 walk: 0 = wave-mask traversal, else lanes per group
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
     int nodes64_valid = 0;  // the FP64 node records belong to the current tree
@@ -333,7 +338,7 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     c->device = device;
     if (const char *g = getenv("COSMOSIM_WALK_GROUP")) {
         const int v = atoi(g);
-        if (v == 8 || v == 16 || v == 32 || v == 64) c->walk_group = v;
+        if (v == 0 || v == 8 || v == 16 || v == 32 || v == 64) c->walk_group = v;
     }
     if (const char *g = getenv("COSMOSIM_XCD_CHUNK")) c->xcd_chunk = atoi(g);
     if (const char *g = getenv("COSMOSIM_WAVE_ORDER")) c->wave_order = atoi(g) != 0;
@@ -687,6 +692,13 @@ static void launch_walk32w(csg_ctx *c, double G, double eps2, double th2, int gr
         c->y, c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk,
         tape, wo);
 }
+
+template <int PREC, bool TP>
+static void launch_walk64w(csg_ctx *c, double G, double eps2, double th2, int grid, int nb8,
+                           TapeOut tape, WaveOrder wo, int tpb) {
+    k_walk64w<PREC, TP><<<grid, tpb, 0, c->stream>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
+        c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape, wo);
+}
 }  // extern "C++"
 
 // lane_counts: record per-body interaction counts (c->icount); walks inside the
@@ -764,9 +776,26 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         c->counts_valid = lane_counts ? 1 : 0;
         return COSMOSIM_OK;
     }
-    // a taped walk runs with whole-wave traversal (the tape is per wavefront)
-    const int gs = tape.rec ? 64 : (precision == COSMOSIM_PREC_FP32 ? c->walk_group32 : c->walk_group);
     c->counts_valid = 1;
+    // FP64: the wave-mask walk (also for tapes, which are per wavefront) when the
+    // records span < 4 GiB, else the grouped walk
+    const int g64 = c->walk_group >= 0 ? c->walk_group
+                                       : (precision == COSMOSIM_PREC_FP64_STRICT ? 8 : 0);
+    if (precision != COSMOSIM_PREC_FP32 && (g64 == 0 || tape.rec) &&
+        (size_t)c->node_cap * sizeof(WNode) < ((size_t)1 << 32)) {
+        const bool taping = tape.rec != nullptr;
+        if (precision == COSMOSIM_PREC_FP64_STRICT) {
+            if (taping) launch_walk64w<WALK_FP64_STRICT, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
+            else launch_walk64w<WALK_FP64_STRICT, false>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
+        } else {
+            if (taping) launch_walk64w<WALK_FP64, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
+            else launch_walk64w<WALK_FP64, false>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
+        }
+        CSG_LAUNCHED();
+        return COSMOSIM_OK;
+    }
+    // a taped walk runs with whole-wave traversal (the tape is per wavefront)
+    const int gs = tape.rec ? 64 : (precision == COSMOSIM_PREC_FP32 ? c->walk_group32 : g64);
     switch (gs) {
     case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
     case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;

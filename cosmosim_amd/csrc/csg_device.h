@@ -1323,6 +1323,15 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
             f.skip = nn + k + 1;
             f.body = -1;
             n32[nn + k] = f;
+            if (nodes) {   // the same for the FP64 wave-mask walk (no mass: never interacts)
+                WNode w;
+                w.cx = w.cy = w.cz = 0.0;
+                w.m = 0.0;
+                w.s2 = __builtin_inf();
+                w.skip = nn + k + 1;
+                w.body = -1; w.depth = 0; w.first = 0; w.last = -1; w.pad = 0;
+                nodes[nn + k] = w;
+            }
         }
     }
     if (nl == 0) {
@@ -1932,6 +1941,120 @@ __global__ __launch_bounds__(1024) void k_walk32w(
     if constexpr (TOTAL) {
         if (lane == 0) atomicAdd(itotal, wcnt);
     }
+}
+
+// 64-bit select on a wave mask held in SGPRs: m's bit for this lane ? t : f.
+__device__ __forceinline__ double lane_sel_f64(unsigned long long m, double t, double f) {
+    const long long tb = __double_as_longlong(t), fb = __double_as_longlong(f);
+    const int lo = lane_sel_i32(m, (int)tb, (int)fb);
+    const int hi = lane_sel_i32(m, (int)(tb >> 32), (int)(fb >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// FP64 walks, one traversal per wavefront on lane masks: k_walk32w's scheme
+// (node index in a scalar register, records by scalar loads, participation,
+// acceptance and descent as SGPR lane masks, four steps per round over the inert
+// padding records after the tree) with walk_visit's per-lane arithmetic.  A
+// lane visits exactly its own depth-first sequence in pre-order and adds the
+// same terms in the same order as k_walk, so the accelerations and counts are
+// k_walk's bit for bit.  Every lane evaluates the opening test and the force at
+// every node of its wave's union; lanes outside the interacting mask add a
+// zero term (x + 0 = x for the sums, which never hold -0).  The node array
+// spans < 4 GiB (32-bit byte offsets).
+template <int PREC, bool TAPING>
+__global__ __launch_bounds__(1024) void k_walk64w(
+    const WNode *__restrict__ nodes, const BuildParams *__restrict__ bp,
+    const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
+    double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
+    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape, WaveOrder wo) {
+    const int n_live = bp->n_live;
+    const int nnodes = bp->n_nodes;
+    if (bp->overflow) return;
+    const int lane = threadIdx.x & 63;
+    const int wv = walk_wave(nb8, chunk, wo);
+    if (wv < 0) return;
+    const int b0 = wv * 64;
+    if (b0 >= n_live) {
+        if (wo.cost && lane == 0 && wv < wo.n) wo.cost[wv] = 0;
+        return;
+    }
+    const int b = b0 + lane;
+    const bool act = b < n_live;
+    int nrec = 0;
+    const int me = act ? sidx[b] : -1;
+    const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
+    const char *const base = (const char *)nodes;
+    int p = act ? 0 : 0x7fffffff;
+    int cnt = 0;
+    unsigned long long wcnt = 0;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    int ii = 0, quads = 0;
+    auto step = [&]() {
+        const WNode nd = *(const WNode *)(base + (unsigned)ii * (unsigned)sizeof(WNode));
+        const double dx = nd.cx - x, dy = nd.cy - y, dz = nd.cz - z;
+        double d2;
+        if constexpr (PREC == WALK_FP64_STRICT) d2 = dx * dx + dy * dy + dz * dz;
+        else d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        // compare predicates: 41 = signed <=, 33 = !=
+        const unsigned long long H = __builtin_amdgcn_sicmp(p, ii, 41);
+        const bool leaf = nd.body >= 0;
+        // internal node: the reference's opening test; a leaf interacts unless
+        // it is the body itself
+        const bool accept = open_accept_exact(nd.s2, d2, theta2);
+        const unsigned long long C = leaf ? ~0ull : __ballot(accept);
+        unsigned long long D = H & ~C;   // descending
+        unsigned long long I = H & C;    // interacting
+        if (leaf) I &= __builtin_amdgcn_sicmp(nd.body, me, 33);
+        if constexpr (PREC == WALK_FP64_STRICT) {
+            if (nd.m == 0.0) { D = 0ull; I = 0ull; }   // an empty cell is passed over
+        }
+        const unsigned long long V = H & ~D;   // lanes that move past the node
+        const double r2 = d2 + eps2;
+        if constexpr (PREC == WALK_FP64_STRICT) {
+            // reference arithmetic and evaluation order, no contraction; the
+            // correctly rounded divide and square root only on steps where a
+            // lane interacts (a wave-uniform branch)
+            if (I) {
+                const double w = lane_sel_f64(I, 1.0 / (r2 * sqrt(r2)), 0.0);
+                const double gm = G * nd.m;
+                sx += gm * dx * w;
+                sy += gm * dy * w;
+                sz += gm * dz * w;
+            }
+        } else {
+            const double yv = rsq64(r2);
+            const double f = lane_sel_f64(I, nd.m * (yv * yv * yv), 0.0);
+            sx = fma(dx, f, sx);
+            sy = fma(dy, f, sy);
+            sz = fma(dz, f, sz);
+        }
+        wcnt += (unsigned long long)__builtin_popcountll(I);
+        cnt += (int)((I >> lane) & 1ull);
+        if constexpr (TAPING) tape_push(tape, b0 >> 6, lane, nrec, I, ii);
+        p = lane_sel_i32(V, nd.skip, p);
+        // strictly increasing even for a malformed skip: the loop always ends
+        const int jump = max(nd.skip, ii + 1);
+        ii = D ? ii + 1 : jump;
+    };
+    while (ii < nnodes) {
+        step();
+        step();
+        step();
+        step();
+        ++quads;
+    }
+    if (wo.cost && lane == 0) wo.cost[wv] = quads * 4;
+    if (act) {
+        if constexpr (PREC == WALK_FP64_STRICT) {
+            ax[me] = sx; ay[me] = sy; az[me] = sz;
+        } else {
+            ax[me] = G * sx; ay[me] = G * sy; az[me] = G * sz;
+        }
+        if (icount) icount[me] = cnt;
+    }
+    if constexpr (TAPING) tape_close(tape, b0 >> 6, lane, nrec);
+    if (lane == 0) atomicAdd(itotal, wcnt);
 }
 
 // Dead bodies get a = 0 (src/octree.c:227-231).

@@ -286,6 +286,36 @@ def test_fp32_traversal_forms_agree(monkeypatch):
     assert np.all(res["0"][0][b["mass"] <= 0] == 0.0)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp64_strict"])
+@pytest.mark.parametrize("theta", [0.0, 0.6])
+def test_fp64_traversal_forms_agree(monkeypatch, precision, theta):
+    """The FP64 walks' two traversals, one per wavefront on lane masks (default
+    for FP64) and groups of 8 to 64 lanes (default 8 for FP64-strict), give each
+    body the same interactions in the same order with the same arithmetic:
+    bit-identical accelerations and equal counts, ordered or not."""
+    b = ca.generate(ca.IC_MERGER, 20000, 13)
+    b["mass"][::89] = 0.0
+    res = {}
+    for gs in ["default", "0", "8", "16", "32", "64", "0-unordered"]:
+        if gs == "default":
+            monkeypatch.delenv("COSMOSIM_WALK_GROUP", raising=False)
+        else:
+            monkeypatch.setenv("COSMOSIM_WALK_GROUP", gs.split("-")[0])
+        monkeypatch.setenv("COSMOSIM_WAVE_ORDER", "0" if "unordered" in gs else "1")
+        sim = sim_for(b, 0.005, theta, precision=precision, dt=5e-4)
+        sim.advance(2)
+        res[gs] = forces_on(sim, theta, precision)
+        sim.close()
+    a0, c0 = res["0"]
+    for gs, (a, c) in res.items():
+        assert same_bits(a, a0), gs
+        assert np.array_equal(c, c0), gs
+    assert np.all(a0[b["mass"] <= 0] == 0.0)
+    if theta == 0.0:
+        live = int((b["mass"] > 0).sum())
+        assert np.all(c0[b["mass"] > 0] == live - 1)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_walk_dispatch_settings_agree(monkeypatch, precision):
     """The ordered dispatch (per-XCD ranges, sub-ranges, waves per workgroup)
