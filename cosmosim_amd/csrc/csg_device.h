@@ -1691,6 +1691,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
 // the order inside a level is whatever the LDS atomics give, which changes only
 // the timing.  Every block computes the same boundaries.
 #define WO_XCDS 8
+#define WO_REG 16
 #define WO_MAX_SPLIT 4
 #define WO_CAP(ng) ((((5 * (ng)) / 32 + 8) + 3) & ~3)
 __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cost, int nw, int gw,
@@ -1716,8 +1717,19 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
     // chunk sums of the group counts (thread t: groups [c0, c1)) and their prefix
     const int per = (ng + 1023) / 1024;
     const int c0 = min(ng, t * per), c1 = min(ng, c0 + per);
+    // chunks of up to WO_REG one-wave groups (up to 1M bodies) are loaded at
+    // once and kept in registers for the boundary search
+    int cv[WO_REG];
+    const bool regs = gw == 1 && per <= WO_REG;
     long long s = 0;
-    for (int g = c0; g < c1; ++g) s += gcost(g);
+    if (regs) {
+#pragma unroll
+        for (int k = 0; k < WO_REG; ++k) cv[k] = c0 + k < c1 ? min(max(cost[c0 + k], 0), 1 << 24) : 0;
+#pragma unroll
+        for (int k = 0; k < WO_REG; ++k) s += cv[k];
+    } else {
+        for (int g = c0; g < c1; ++g) s += gcost(g);
+    }
     long long inc = s;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -1739,7 +1751,13 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
         if (target > 0 && ex < target && ex + s >= target) {
             long long run = ex;
             int g = c0;
-            while (run < target) run += gcost(g++);
+            if (regs) {
+#pragma unroll
+                for (int q = 0; q < WO_REG; ++q)
+                    if (run < target) { run += cv[q]; g = c0 + q + 1; }
+            } else {
+                while (run < target) run += gcost(g++);
+            }
             bnd[k] = g;
         }
     }
@@ -1765,8 +1783,11 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
     for (int q = 0; q < split; ++q) {
         const int lo = bnd[r * split + q], hi = bnd[r * split + q + 1];
         for (int i = t; i < 4096; i += 1024) hist[i] = 0;
+        // this thread's first group of the sub-range, loaded once
+        const int myc = lo + t < hi ? gcost(lo + t) : 0;
+        auto gc = [&](int g) { return g == lo + t ? myc : gcost(g); };
         int mx = 0;
-        for (int g = lo + t; g < hi; g += 1024) mx = max(mx, gcost(g));
+        for (int g = lo + t; g < hi; g += 1024) mx = max(mx, gc(g));
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
         if (lane == 0) part[wid] = mx;
@@ -1777,7 +1798,7 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
         int shift = 0;
         while ((mx >> shift) >= 4096) ++shift;
         auto bin = [&](int c) { return 4095 - (c >> shift); };
-        for (int g = lo + t; g < hi; g += 1024) atomicAdd(&hist[bin(gcost(g))], 1);
+        for (int g = lo + t; g < hi; g += 1024) atomicAdd(&hist[bin(gc(g))], 1);
         __syncthreads();
         // exclusive scan of the 4096 bins: 4 per thread, then across threads
         int h[4], sum = 0;
@@ -1798,7 +1819,7 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
 #pragma unroll
         for (int k = 0; k < 4; ++k) { hist[4 * t + k] = run; run += h[k]; }
         __syncthreads();
-        for (int g = lo + t; g < hi; g += 1024) place(atomicAdd(&hist[bin(gcost(g))], 1), g);
+        for (int g = lo + t; g < hi; g += 1024) place(atomicAdd(&hist[bin(gc(g))], 1), g);
         __syncthreads();
     }
     for (int j = (xhi - xlo) + t; j < capg; j += 1024) place(j, -1);
