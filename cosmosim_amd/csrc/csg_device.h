@@ -164,6 +164,14 @@ __device__ __forceinline__ double rsq64(double r2) {
     return fma(0.5 * y, e, y);
 }
 
+// The FP64 walks' 1/sqrt(r2): the hardware estimate refined by one Newton step
+// on the same residual.
+__device__ __forceinline__ double rsq64_walk(double r2) {
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double e = fma(-r2 * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
 // Octant code: bit0 x >= c, bit1 y < c, bit2 z < c.
 __device__ __forceinline__ int cell_octant(double px, double py, double pz, double cx, double cy,
                                            double cz) {
@@ -1484,7 +1492,7 @@ __device__ __forceinline__ bool walk_visit(WalkLane &L, const WNode &nd, int i, 
         L.inter = inter;
         if (inter) {
             const double r2 = d2 + eps2;
-            const double yv = rsq64(r2);
+            const double yv = rsq64_walk(r2);
             const double f = nd.m * (yv * yv * yv);
             L.sx = fma(dx, f, L.sx);
             L.sy = fma(dy, f, L.sy);
@@ -2044,7 +2052,7 @@ __global__ __launch_bounds__(1024) void k_walk64w(
                 sz += gm * dz * w;
             }
         } else {
-            const double yv = rsq64(r2);
+            const double yv = rsq64_walk(r2);
             const double f = lane_sel_f64(I, nd.m * (yv * yv * yv), 0.0);
             sx = fma(dx, f, sx);
             sy = fma(dy, f, sy);
