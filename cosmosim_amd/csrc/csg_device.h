@@ -32,9 +32,10 @@
 // A live key is never all ones in those bits (that cell lies in the 1% margin
 // of the root cube), so the all-ones key of dead bodies sorts last.
 #define CSG_KEY_LEVELS 16
-// FP32 node records past the last node: padding the FP32 walk may step onto
-// (it tests for the end once per four steps); never accepted, no mass
-#define CSG_NODE_PAD 4
+// Node records past the last node: padding the wave-mask walks may step onto
+// (they test for the end once per round of eight FP32 or four FP64 steps);
+// never accepted, no mass
+#define CSG_NODE_PAD 8
 #define CSG_KEY_BITS (3 * CSG_KEY_LEVELS)
 #define CSG_SORT_PASSES ((CSG_KEY_BITS + 7) / 8)
 // collision words: ext[0] levels L..L+20, ext[1] L+21..L+41, ext[2] L+42..63
@@ -1320,7 +1321,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const double H = bp->root[3];
     if (b == 0) {
         // padding records after the tree (the FP32 walk's unrolled rounds may
-        // step onto up to three of them): opening criterion never met
+        // step onto up to seven of them): opening criterion never met
         // (s2 = +inf), so no lane interacts, descends past them one by one
         const int nn = bp->n_nodes;
         for (int k = 0; k < CSG_NODE_PAD; ++k) {
@@ -1936,8 +1937,8 @@ __global__ __launch_bounds__(1024) void k_walk32w(
         const int jump = max(nd.skip, ii + 1);
         ii = D ? ii + 1 : jump;
     };
-    // four steps per round and one end test per round: a round that starts
-    // before the end may step onto up to three of the inert padding records
+    // eight steps per round and one end test per round: a round that starts
+    // before the end may step onto up to seven of the inert padding records
     // after the last node (CSG_NODE_PAD); the compensated fold comes after every
     // CSG_FOLD_EVERY steps
     while (ii < nnodes) {
@@ -1945,7 +1946,11 @@ __global__ __launch_bounds__(1024) void k_walk32w(
         step();
         step();
         step();
-        if (++quads == CSG_FOLD_EVERY / 4) {
+        step();
+        step();
+        step();
+        step();
+        if (++quads == CSG_FOLD_EVERY / 8) {
             quads = 0;
             ++folds;
             if constexpr (TOTAL) {
@@ -1958,7 +1963,7 @@ __global__ __launch_bounds__(1024) void k_walk32w(
     }
     wcnt += wc32;
     // node steps of this wave, to within the last round (the dispatch order's key)
-    if (wo.cost && lane == 0) wo.cost[wv] = folds * CSG_FOLD_EVERY + quads * 4;
+    if (wo.cost && lane == 0) wo.cost[wv] = folds * CSG_FOLD_EVERY + quads * 8;
     kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
     if (act) {
         ax[me] = G * ((double)sx - (double)kx);
