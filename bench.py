@@ -86,11 +86,57 @@ def accuracy(ca, sim, idx):
     return r
 
 
+def per_step_accuracy(ca, bodies, idx, theta, steps):
+    """FP32 substeps of the 1M merger (untimed): after every substep, the FP32
+    walk's accelerations of the 4096-body subset against the FP64 direct sum on
+    the same state (error (c)); errors (a) and (b) are taken at the first and
+    last state by accuracy()."""
+    sim = ca.Simulation(bodies, softening=EPS, theta=theta, dt=DT, precision="fp32")
+    rms, mx = [], []
+    for _ in range(steps):
+        sim.advance(1)
+        a = sim.accelerations()
+        r, m = rel_err(a[idx], sim.direct_forces(idx))
+        rms.append(r)
+        mx.append(m)
+    sim.close()
+    return {"steps": steps, "fp32_vs_direct_rms_mean": float(np.mean(rms)),
+            "fp32_vs_direct_rms_max": float(np.max(rms)),
+            "fp32_vs_direct_max_max": float(np.max(mx))}
+
+
+def config_line(ca, recipe, n, seed, theta, eps, dt, steps, precision):
+    """Substeps/s, interactions/s and the walk's roofline fraction of one of the
+    smaller configurations, with its energy and momentum drift over the run."""
+    sim = ca.Simulation(ca.generate(recipe, n, seed), softening=eps, theta=theta, dt=dt,
+                        precision=precision)
+    ke0, pe0, p0 = sim.energy()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.advance(steps)
+    t1 = time.perf_counter()
+    st = sim.stats()
+    ke1, pe1, p1 = sim.energy()
+    sim.close()
+    walk_ms = st.ms_walk / max(st.timed_walks, 1)
+    peak = FP32_PEAK_TFLOPS if precision == "fp32" else FP64_PEAK_TFLOPS
+    tflops = st.interactions * FLOP_PER_INTERACTION / (walk_ms * 1e-3) / 1e12
+    return {"bodies": n, "substeps": steps, "precision": precision,
+            "substeps_per_s": steps / (t1 - t0), "walk_ms": walk_ms,
+            "build_ms": st.ms_build / steps, "kdk_ms": st.ms_kdk / steps,
+            "interactions": int(st.interactions),
+            "interactions_per_s": st.interactions / (walk_ms * 1e-3),
+            "walk_tflops": tflops, "walk_valu_frac": tflops / peak,
+            "energy_drift": abs((ke1 + pe1 - ke0 - pe0) / (ke0 + pe0)),
+            "momentum_drift": float(np.linalg.norm(p1 - p0))}
+
+
 def cpu_baseline(ca, bodies, theta):
-    """The reference's own C (oracle/_ref) when it was built here, else the
-    oracle's restatement, on the host cores: median of 3 KDK substeps."""
-    from oracle.cpu_ref import CpuRef, RefBuild
-    impl = RefBuild() if RefBuild.available() else CpuRef()
+    """The oracle's restatement of the reference (oracle/libcpuref.so: serial
+    insertion build, the walk under the reference's OpenMP schedule, serial KDK)
+    on the host cores: median of 3 KDK substeps."""
+    from oracle.cpu_ref import CpuRef
+    impl = CpuRef()
     b = bodies.copy()
     impl.init_acc(b, 1.0, EPS, theta)
     pool = np.zeros(8 * len(b) + 16, dtype=ca.NODE_DTYPE)
@@ -110,42 +156,63 @@ def cpu_baseline(ca, bodies, theta):
     }
 
 
-def adjoint_runs(ca, big_n, big_steps):
-    """Adjoint throughput.  Config 3: 8,192-body merger, 50 substeps, dt 1e-3,
-    eps 0.02, theta 0.5, FP64 walk, checkpoints every 10.  Config 4 (shortened to
-    big_steps substeps): big_n-body merger (seed 5), FP32 walk, theta 0.7,
-    eps 0.003, dt 5e-4, checkpoints every 10; both reverse modes."""
+def _adjoint_problem(ca, n, seed, steps, eps, th, dt, prec):
+    """Bodies and the target snapshot: the same simulator run from the orbit
+    parameters perturbed by 1%."""
+    b = ca.generate(ca.IC_MERGER, n, seed)
+    t = ca.Simulation(ca.generate(ca.IC_MERGER, n, seed, orbit=ca.default_orbit() * 1.01),
+                      softening=eps, theta=th, dt=dt, precision=prec)
+    t.advance(steps)
+    target = t.state()[0]
+    t.close()
+    return b, target
+
+
+def adjoint_runs(ca, big_n, big_steps, everies=(10, 25, 50)):
+    """Adjoint throughput.  Config 3: 8,192-body merger (seed 4), 50 substeps,
+    dt 1e-3, eps 0.02, theta 0.5, FP64 walk, checkpoints every 10.  Config 4:
+    big_n-body merger (seed 5), FP32 walk, theta 0.7, eps 0.003, dt 5e-4,
+    big_steps forward + big_steps reverse substeps with checkpoints every K
+    (segments recomputed forward) for each K, and the time-reversed leapfrog.
+    Loss: mean squared distance to the target snapshot."""
     out = {}
-    for name, n, seed, steps, eps, th, dt, prec in [
-            ("config3_8192_fp64_theta0.5", 8192, 4, 50, 0.02, 0.5, 1e-3, "fp64"),
-            (f"config4_{big_n}_fp32_theta0.7", big_n, 5, big_steps, 0.003, 0.7, 5e-4, "fp32")]:
-        free0, total = ca.device_memory(0)
-        b = ca.generate(ca.IC_MERGER, n, seed)
-        orbit = ca.default_orbit()
-        target = ca.positions(ca.generate(ca.IC_MERGER, n, seed, orbit=orbit * 1.01))
-        sim = ca.Simulation(b, softening=eps, theta=th, dt=dt, precision=prec)
-        t0 = time.perf_counter()
-        sim.record(steps, 10)
-        t1 = time.perf_counter()
-        xk = sim.state()[0]
-        g = 2 * (xk - target) / n
-        r = {"bodies": n, "substeps": steps, "checkpoint_every": 10,
-             "forward_substeps_per_s": steps / (t1 - t0)}
-        grads = []
-        for mode, label in [(0, "checkpoint"), (1, "reversal")]:
+    for name, n, seed, steps, eps, th, dt, prec, ks in [
+            ("config3_8192_fp64_theta0.5", 8192, 4, 50, 0.02, 0.5, 1e-3, "fp64", (10,)),
+            (f"config4_{big_n}_fp32_theta0.7", big_n, 5, big_steps, 0.003, 0.7, 5e-4, "fp32",
+             everies)]:
+        b, target = _adjoint_problem(ca, n, seed, steps, eps, th, dt, prec)
+        r = {"bodies": n, "substeps": steps}
+        grads = {}
+        sim = None
+        for k in ks:
+            # a handle of its own per K, so the device memory is that K's
+            if sim is not None:
+                sim.close()
+            free0, total = ca.device_memory(0)
+            sim = ca.Simulation(b, softening=eps, theta=th, dt=dt, precision=prec)
+            t0 = time.perf_counter()
+            sim.record(steps, k)
+            t1 = time.perf_counter()
+            xk = sim.state()[0]
+            g = 2 * (xk - target) / n
             t2 = time.perf_counter()
-            xb, vb, st = sim.backward(g, mode=mode)
+            xb, vb, st = sim.backward(g, mode=0)
             t3 = time.perf_counter()
-            r[f"adjoint_substeps_per_s_{label}"] = steps / (t3 - t2)
-            if mode == 1:
-                r["reversal_max_position_deviation"] = float(st[0])
-            grads.append(xb)
-        r["reversal_vs_checkpoint_grad_rel"] = float(
-            np.max(np.abs(grads[0] - grads[1])) / max(np.max(np.abs(grads[0])), 1e-300))
-        free1, _ = ca.device_memory(0)
-        r["adjoint_device_bytes"] = float(st[1])
-        r["hbm_used_bytes"] = int(total - free1)
-        r["hbm_added_bytes"] = int(free0 - free1)
+            free1, _ = ca.device_memory(0)
+            r[f"K{k}"] = {"forward_substeps_per_s": steps / (t1 - t0),
+                          "adjoint_substeps_per_s": steps / (t3 - t2),
+                          "adjoint_device_bytes": float(st[1]),
+                          "hbm_used_bytes": int(total - free1),
+                          "hbm_added_bytes": int(free0 - free1)}
+            grads[k] = xb
+        t2 = time.perf_counter()
+        xb, vb, st = sim.backward(g, mode=1)
+        t3 = time.perf_counter()
+        r["reversal"] = {"adjoint_substeps_per_s": steps / (t3 - t2),
+                         "max_position_deviation": float(st[0])}
+        r["reversal_vs_checkpoint_grad_rel"] = max(
+            float(np.max(np.abs(grads[k] - xb)) / max(np.max(np.abs(xb)), 1e-300)) for k in ks)
+        r["loss"] = float(np.sum((xk - target) ** 2) / n)
         sim.close()
         out[name] = r
     return out
@@ -176,9 +243,9 @@ def sph_runs(ca, bodies, cpu=True):
     """SPH on the device tree of the 1M merger with every other star made gas:
     smoothing lengths first settle over four density passes (each moves h by
     x1.2 / x0.8 toward 20-48 neighbours), then one density and one force pass
-    are timed with the core's events.  The reference's own sph.c (oracle/_ref,
-    else the oracle's restatement) runs the same two passes on the host cores
-    from the same state, and its densities and accelerations are compared."""
+    are timed with the core's events.  The oracle's restatement of sph.c runs
+    the same two passes on the host cores from the same state, and its
+    densities and accelerations are compared."""
     b = bodies.copy()
     gas = (np.arange(len(b)) % 2 == 0) & (b["type"] == 0)
     b["type"] = np.where(gas, 1, b["type"])
@@ -201,8 +268,8 @@ def sph_runs(ca, bodies, cpu=True):
            "gas_bodies_per_s_kernel": ngas / (st.ms_sph * 1e-3),
            "call_s_density_forces": t1 - t0}
     if cpu:
-        from oracle.cpu_ref import CpuRef, RefBuild
-        impl = RefBuild() if RefBuild.available() else CpuRef()
+        from oracle.cpu_ref import CpuRef
+        impl = CpuRef()
         cb = start.copy()
         pool = impl.build(cb)
         c0 = time.perf_counter()
@@ -212,7 +279,7 @@ def sph_runs(ca, bodies, cpu=True):
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         same = all(np.array_equal(cb[f].view(np.uint64), got[f].view(np.uint64))
                    for f in ("density", "pressure", "smoothing_h", "ax", "ay", "az"))
-        out.update({"cpu_kind": impl.kind, "cpu_threads": threads if impl.kind == "reference" else 1,
+        out.update({"cpu_kind": impl.kind, "cpu_threads": threads,
                     "cpu_s_density_forces": c1 - c0, "cpu_gas_bodies_per_s": ngas / (c1 - c0),
                     "bit_identical_to_cpu": bool(same)})
     return out
@@ -255,8 +322,8 @@ def quasar_scene(n, seed=7):
 
 def quasar_runs(ca, n=1_000_000, steps=10, cpu=True):
     """quasar_step on the device (accretion passes, feedback, jets) for a
-    1M-body quasar scene, per step, against the reference's quasar_step
-    (oracle/_ref, else the oracle) on the host for the same first steps."""
+    1M-body quasar scene, per step, against the oracle's restatement of
+    quasar_step on one host thread for the same first steps."""
     b, n0 = quasar_scene(n)
     sim = ca.Simulation(b[:n0], softening=0.01, theta=0.5, precision="fp32")
     sim.reserve(len(b))
@@ -270,16 +337,11 @@ def quasar_runs(ca, n=1_000_000, steps=10, cpu=True):
            "bodies_after": sim.count(), "jet_count": cfg.jet_count}
     sim.close()
     if cpu:
-        from oracle.cpu_ref import CpuRef, QuasarConfig, RefBuild
+        from oracle.cpu_ref import CpuRef, QuasarConfig
         c = b.copy()
         rc = QuasarConfig.default(jet_count=400, jet_cap=n // 10)
-        if RefBuild.available():
-            from oracle.cpu_ref import RefQuasar
-            R, kind = RefQuasar(), "reference"
-            step = lambda k: R.quasar_step(c, k, rc, 5e-4)
-        else:
-            R, kind, st = CpuRef(), "port", [1, 0]
-            step = lambda k: R.quasar_step(c, k, rc, 5e-4, st)
+        R, kind, st = CpuRef(), "port", [1, 0]
+        step = lambda k: R.quasar_step(c, k, rc, 5e-4, st)
         k = step(n0)
         c0 = time.perf_counter()
         for _ in range(3):
@@ -287,6 +349,12 @@ def quasar_runs(ca, n=1_000_000, steps=10, cpu=True):
         c1 = time.perf_counter()
         out.update({"cpu_kind": kind, "cpu_threads": 1, "cpu_ms_per_step": (c1 - c0) / 3 * 1e3})
     return out
+
+
+def _rounded(v):
+    if isinstance(v, dict):
+        return {k: _rounded(x) for k, x in v.items()}
+    return float(f"{v:.4g}") if isinstance(v, float) else v
 
 
 def host_cpu():
@@ -333,8 +401,11 @@ def main():
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-bodies", type=int, default=4_000_000)
-    ap.add_argument("--adjoint-steps", type=int, default=20)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--no-neighbors", action="store_true")
+    ap.add_argument("--no-per-step", action="store_true",
+                    help="skip the per-substep subset accuracy runs")
+    ap.add_argument("--per-step-steps", type=int, default=200)
     ap.add_argument("--no-sph", action="store_true")
     ap.add_argument("--no-quasar", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -375,6 +446,15 @@ def main():
         results[f"theta{th}_accuracy_step0"] = accuracy(ca, s0, idx)
         s0.close()
 
+    if not args.no_per_step:
+        results["per_step"] = {f"theta{th}": per_step_accuracy(ca, bodies, idx, th,
+                                                               args.per_step_steps)
+                               for th in thetas}
+    if not args.headline_only:
+        results["config0_20001_disk"] = config_line(ca, ca.IC_DISK, 20001, 1, 0.5, 0.01, 1e-3,
+                                                    100, "fp64_strict")
+        results["config1_100000_merger"] = config_line(ca, ca.IC_MERGER, 100000, 2, 0.5, 0.01,
+                                                       1e-3, 1000, "fp64_strict")
     walk_flop = head["interactions"] * FLOP_PER_INTERACTION
     achieved = walk_flop / (head["walk_ms"] * 1e-3) / 1e12
     traffic = load_traffic()
@@ -435,10 +515,9 @@ def main():
     line["accuracy_theta0.5"] = {k: float(f"{v:.3e}") for k, v in acc.items()}
     if not args.no_adjoint:
         adj = adjoint_runs(ca, args.adjoint_bodies, args.adjoint_steps)
-        line["adjoint"] = {k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv)
-                               for kk, vv in v.items()} for k, v in adj.items()}
+        line["adjoint"] = _rounded(adj)
         big = adj[f"config4_{args.adjoint_bodies}_fp32_theta0.7"]
-        line["adjoint_substeps_per_s"] = round(big["adjoint_substeps_per_s_checkpoint"], 3)
+        line["adjoint_substeps_per_s"] = round(big["K10"]["adjoint_substeps_per_s"], 3)
     if not args.no_neighbors:
         line["neighbors"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
                              for k, v in neighbor_runs(ca, bodies).items()}
@@ -453,9 +532,12 @@ def main():
         line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v)
                                 for k, v in cb.items()}
         line["speedup_vs_cpu"] = round(head["substeps_per_s"] / cb["value"], 1)
-    line["results"] = {k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv)
-                           for kk, vv in v.items() if not isinstance(vv, dict)}
-                       for k, v in results.items() if "accuracy" not in k}
+    line["results"] = {k: _rounded(v) for k, v in results.items()
+                       if "accuracy" not in k and k != "per_step"}
+    if "per_step" in results:
+        line["accuracy_per_step"] = {k: {kk: (float(f"{vv:.3e}") if isinstance(vv, float) else vv)
+                                         for kk, vv in v.items()}
+                                     for k, v in results["per_step"].items()}
     line["accuracy"] = {k: {kk: float(f"{vv:.3e}") for kk, vv in v.items()}
                         for k, v in results.items() if "accuracy" in k}
     print(json.dumps(line))

@@ -74,11 +74,13 @@ _SIGS = {
     "cosmosim_ext_quasar_step": (_I, [_VP, _VP, _D]),
     "cosmosim_ext_quasar_compact": (_I, [_VP, _VP]),
     "cosmosim_ext_quasar_state": (_I, [_VP, _VP, _VP, _I]),
+    "cosmosim_ext_quasar_config": (_I, [_VP, _VP, _I]),
     "cosmosim_ext_interaction_counts": (_I, [_VP, _VP, _I]),
     "cosmosim_ext_get_stats": (_I, [_VP, ctypes.POINTER(ExtStats)]),
     "cosmosim_ext_set_timing": (_I, [_VP, _I]),
     "cosmosim_ext_export_tree": (_I, [_VP, _VP, _I, _VP]),
     "cosmosim_ext_handle_error": (_I, [_VP]),
+    "cosmosim_ext_debug_fail_build": (_I, [_VP, _I]),
     "cosmosim_ext_last_error": (_I, []),
     "cosmosim_ext_clear_error": (None, []),
     "cosmosim_ext_error_string": (ctypes.c_char_p, [_I]),
@@ -125,6 +127,7 @@ _SIGS = {
     "csg_get_stats": (_I, [_VP, _VP]),
     "csg_set_timing": (_I, [_VP, _I]),
     "csg_synchronize": (_I, [_VP]),
+    "csg_debug_fail_build": (_I, [_VP, _I]),
     "csg_walk_taped": (_I, [_VP, _I, _D, _D, _D]),
     "csg_force_vjp": (_I, [_VP, _VP, _VP]),
     "csg_forces_frozen": (_I, [_VP, _VP, _VP]),
@@ -138,30 +141,31 @@ _SIGS = {
 }
 EXPORTED_SYMBOLS = tuple(_SIGS)
 
-_lib = None
+_libs = {}
 
 
 class CosmosimError(RuntimeError):
     def __init__(self, code, where=""):
         self.code = code
-        msg = lib().cosmosim_ext_error_string(code).decode() if _lib is not None else str(code)
+        msg = lib().cosmosim_ext_error_string(code).decode() if _libs else str(code)
         super().__init__(f"{where}: {msg} (code {code})")
 
 
-def lib():
-    """The loaded libcosmosim.so (raises if it was not built)."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(library_path):
+def lib(path=None):
+    """The loaded libcosmosim.so, or another build of it at `path` (a variant
+    linked with the caller's initial-condition generators); raises if missing."""
+    path = library_path if path is None else os.path.abspath(path)
+    if path not in _libs:
+        if not os.path.exists(path):
             raise ImportError(
-                f"{library_path} is missing: build it with `python -m cosmosim_amd.build`")
-        L = ctypes.CDLL(library_path)
+                f"{path} is missing: build it with `python -m cosmosim_amd.build`")
+        L = ctypes.CDLL(path)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _check(code, where):
@@ -256,8 +260,8 @@ def orbit_coefficients(n, recipe=IC_MERGER):
 class Handle:
     """The eight legacy exports of src/cosmosim_api.h on one handle."""
 
-    def __init__(self, config=None, **kw):
-        L = lib()
+    def __init__(self, config=None, library=None, **kw):
+        self.L = L = lib(library)
         cfg = L.cosmosim_default_config() if config is None else config
         for k, v in kw.items():
             setattr(cfg, k, v)
@@ -266,28 +270,50 @@ class Handle:
         if not self.h:
             raise CosmosimError(L.cosmosim_ext_last_error() or -3, "cosmosim_create")
 
+    def quasar_state(self, rng=None, phase=None):
+        """Get [rng, phase] of the quasar random stream; or set them when given."""
+        r, p = ctypes.c_uint64(0), ctypes.c_int(0)
+        if rng is None and phase is None:
+            _check(self.L.cosmosim_ext_quasar_state(self.h, ctypes.byref(r), ctypes.byref(p), 0),
+                   "quasar_state")
+            return [r.value, p.value]
+        r.value, p.value = int(rng), int(phase)
+        _check(self.L.cosmosim_ext_quasar_state(self.h, ctypes.byref(r), ctypes.byref(p), 1),
+               "quasar_state")
+        return [r.value, p.value]
+
+    def quasar_config(self, cfg=None):
+        """The QuasarConfig the quasar-mode step uses (a copy); or set it."""
+        if cfg is None:
+            cfg = QuasarConfig()
+            _check(self.L.cosmosim_ext_quasar_config(self.h, ctypes.byref(cfg), 0),
+                   "quasar_config")
+            return cfg
+        _check(self.L.cosmosim_ext_quasar_config(self.h, ctypes.byref(cfg), 1), "quasar_config")
+        return cfg
+
     def step(self):
-        lib().cosmosim_step(self.h)
-        _check(lib().cosmosim_ext_handle_error(self.h), "cosmosim_step")
+        self.L.cosmosim_step(self.h)
+        _check(self.L.cosmosim_ext_handle_error(self.h), "cosmosim_step")
 
     def bodies(self):
         n = self.count()
-        p = lib().cosmosim_get_bodies(self.h)
+        p = self.L.cosmosim_get_bodies(self.h)
         buf = (ctypes.c_char * (168 * n)).from_address(p)
         return np.frombuffer(buf, dtype=BODY_DTYPE, count=n).copy()
 
     def count(self):
-        return lib().cosmosim_get_count(self.h)
+        return self.L.cosmosim_get_count(self.h)
 
     def active_count(self):
-        return lib().cosmosim_get_active_count(self.h)
+        return self.L.cosmosim_get_active_count(self.h)
 
     def sim_time(self):
-        return lib().cosmosim_get_sim_time(self.h)
+        return self.L.cosmosim_get_sim_time(self.h)
 
     def close(self):
         if self.h:
-            lib().cosmosim_destroy(self.h)
+            self.L.cosmosim_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -302,7 +328,7 @@ class Simulation(Handle):
 
     def __init__(self, bodies, G=1.0, softening=0.01, theta=0.5, dt=1e-3, precision="fp64",
                  device=0):
-        L = lib()
+        self.L = L = lib()
         self.params = ExtParams(G, softening, theta, dt, PRECISIONS.get(precision, precision),
                                 device)
         self.n = len(bodies)
@@ -317,55 +343,57 @@ class Simulation(Handle):
             if k == "precision":
                 v = PRECISIONS.get(v, v)
             setattr(self.params, k, v)
-        _check(lib().cosmosim_ext_set_params(self.h, ctypes.byref(self.params)), "set_params")
+        _check(self.L.cosmosim_ext_set_params(self.h, ctypes.byref(self.params)), "set_params")
 
     def advance(self, substeps):
-        _check(lib().cosmosim_ext_step(self.h, substeps), "cosmosim_ext_step")
+        _check(self.L.cosmosim_ext_step(self.h, substeps), "cosmosim_ext_step")
 
     def compute_forces(self):
-        _check(lib().cosmosim_ext_compute_forces(self.h), "compute_forces")
+        _check(self.L.cosmosim_ext_compute_forces(self.h), "compute_forces")
 
     def state(self):
         pos, vel, acc = (np.zeros((self.n, 3)) for _ in range(3))
         mass = np.zeros(self.n)
-        _check(lib().cosmosim_ext_get_state(self.h, ptr(pos), ptr(vel), ptr(acc), ptr(mass)),
+        _check(self.L.cosmosim_ext_get_state(self.h, ptr(pos), ptr(vel), ptr(acc), ptr(mass)),
                "get_state")
         return pos, vel, acc, mass
 
     def set_state(self, pos=None, vel=None, acc=None, mass=None):
         arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
                for a in (pos, vel, acc, mass)]
-        _check(lib().cosmosim_ext_set_state(self.h, *(ptr(a) for a in arr)), "set_state")
+        _check(self.L.cosmosim_ext_set_state(self.h, *(ptr(a) for a in arr)), "set_state")
 
     def accelerations(self):
-        return self.state()[2]
+        acc = np.zeros((self.n, 3))
+        _check(self.L.cosmosim_ext_get_state(self.h, None, None, ptr(acc), None), "get_state")
+        return acc
 
     def direct_forces(self, idx):
         idx = np.ascontiguousarray(idx, dtype=np.int32)
         acc = np.zeros((len(idx), 3))
-        _check(lib().cosmosim_ext_direct_forces(self.h, ptr(idx), len(idx), ptr(acc)),
+        _check(self.L.cosmosim_ext_direct_forces(self.h, ptr(idx), len(idx), ptr(acc)),
                "direct_forces")
         return acc
 
     def energy(self):
         ke, pe = ctypes.c_double(), ctypes.c_double()
         p = np.zeros(3)
-        _check(lib().cosmosim_ext_energy(self.h, ctypes.byref(ke), ctypes.byref(pe), ptr(p)),
+        _check(self.L.cosmosim_ext_energy(self.h, ctypes.byref(ke), ctypes.byref(pe), ptr(p)),
                "energy")
         return ke.value, pe.value, p
 
     def interaction_counts(self):
         c = np.zeros(self.n, dtype=np.int32)
-        _check(lib().cosmosim_ext_interaction_counts(self.h, ptr(c), self.n), "counts")
+        _check(self.L.cosmosim_ext_interaction_counts(self.h, ptr(c), self.n), "counts")
         return c
 
     def stats(self):
         s = ExtStats()
-        _check(lib().cosmosim_ext_get_stats(self.h, ctypes.byref(s)), "stats")
+        _check(self.L.cosmosim_ext_get_stats(self.h, ctypes.byref(s)), "stats")
         return s
 
     def set_timing(self, on=True):
-        _check(lib().cosmosim_ext_set_timing(self.h, int(on)), "set_timing")
+        _check(self.L.cosmosim_ext_set_timing(self.h, int(on)), "set_timing")
 
     def find_neighbors(self, centres, radius, cap=128):
         """Sphere queries over the device tree: a list of index arrays, one per
@@ -376,57 +404,45 @@ class Simulation(Handle):
         rad = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, dtype=np.float64), (k,)))
         out = np.zeros((k, max(cap, 1)), dtype=np.int32)
         counts = np.zeros(k, dtype=np.int32)
-        _check(lib().cosmosim_ext_find_neighbors(self.h, ptr(centres), ptr(rad), k, cap, ptr(out),
+        _check(self.L.cosmosim_ext_find_neighbors(self.h, ptr(centres), ptr(rad), k, cap, ptr(out),
                                                  ptr(counts)), "find_neighbors")
         return [out[q, :counts[q]].copy() for q in range(k)]
 
     # ------------------------------------------------------------ SPH
     def sph_density(self):
         """sph_compute_density on the device (src/sph.c:45-95)."""
-        _check(lib().cosmosim_ext_sph_density(self.h), "sph_density")
+        _check(self.L.cosmosim_ext_sph_density(self.h), "sph_density")
 
     def sph_forces(self):
         """sph_compute_forces on the device (src/sph.c:99-196): adds to a."""
-        _check(lib().cosmosim_ext_sph_forces(self.h), "sph_forces")
+        _check(self.L.cosmosim_ext_sph_forces(self.h), "sph_forces")
 
     def sph_cooling(self, dt):
         """sph_apply_cooling on the device (src/sph.c:200-231)."""
-        _check(lib().cosmosim_ext_sph_cooling(self.h, dt), "sph_cooling")
+        _check(self.L.cosmosim_ext_sph_cooling(self.h, dt), "sph_cooling")
 
     def advance_sph(self, substeps):
         """Substeps of integrator_step with SPH enabled, device-resident."""
-        _check(lib().cosmosim_ext_step_sph(self.h, substeps), "cosmosim_ext_step_sph")
+        _check(self.L.cosmosim_ext_step_sph(self.h, substeps), "cosmosim_ext_step_sph")
 
     # ------------------------------------------------------------ quasar
     def reserve(self, capacity):
         """Grow the body capacity (room for bodies a quasar step appends)."""
-        _check(lib().cosmosim_ext_reserve(self.h, int(capacity)), "reserve")
+        _check(self.L.cosmosim_ext_reserve(self.h, int(capacity)), "reserve")
 
     def quasar_step(self, cfg, dt):
         """quasar_step on the device (src/quasar.c:477-510); cfg (QuasarConfig) is
         updated in place; returns the new body count."""
-        _check(lib().cosmosim_ext_quasar_step(self.h, ctypes.byref(cfg), dt), "quasar_step")
+        _check(self.L.cosmosim_ext_quasar_step(self.h, ctypes.byref(cfg), dt), "quasar_step")
         self.n = self.count()
         return self.n
 
     def quasar_compact(self):
         """quasar_compact on the device (src/quasar.c:512-523); returns the new count."""
         k = ctypes.c_int(0)
-        _check(lib().cosmosim_ext_quasar_compact(self.h, ctypes.byref(k)), "quasar_compact")
+        _check(self.L.cosmosim_ext_quasar_compact(self.h, ctypes.byref(k)), "quasar_compact")
         self.n = self.count()
         return k.value
-
-    def quasar_state(self, rng=None, phase=None):
-        """Get [rng, phase]; or set them when given."""
-        r, p = ctypes.c_uint64(0), ctypes.c_int(0)
-        if rng is None and phase is None:
-            _check(lib().cosmosim_ext_quasar_state(self.h, ctypes.byref(r), ctypes.byref(p), 0),
-                   "quasar_state")
-            return [r.value, p.value]
-        r.value, p.value = int(rng), int(phase)
-        _check(lib().cosmosim_ext_quasar_state(self.h, ctypes.byref(r), ctypes.byref(p), 1),
-               "quasar_state")
-        return [r.value, p.value]
 
     # ------------------------------------------------------------ snapshots
     def snapshot_ue(self, max_bodies=None):
@@ -434,14 +450,14 @@ class Simulation(Handle):
         m = self.n if max_bodies is None else int(max_bodies)
         out = np.zeros((max(m, 1), 16), dtype=np.float32)
         k = ctypes.c_int(0)
-        _check(lib().cosmosim_ext_snapshot_ue(self.h, ptr(out), m, ctypes.byref(k)), "snapshot_ue")
+        _check(self.L.cosmosim_ext_snapshot_ue(self.h, ptr(out), m, ctypes.byref(k)), "snapshot_ue")
         return out[: k.value].copy()
 
     def snapshot_render(self):
         """The renderer's upload records of the live bodies: float32 [count, 10]."""
         out = np.zeros((max(self.n, 1), 10), dtype=np.float32)
         k = ctypes.c_int(0)
-        _check(lib().cosmosim_ext_snapshot_render(self.h, ptr(out), ctypes.byref(k)),
+        _check(self.L.cosmosim_ext_snapshot_render(self.h, ptr(out), ctypes.byref(k)),
                "snapshot_render")
         return out[: k.value].copy()
 
@@ -449,35 +465,39 @@ class Simulation(Handle):
         cap = 8 * self.n + 4096
         pool = np.zeros(cap, dtype=NODE_DTYPE)
         used = ctypes.c_int(0)
-        _check(lib().cosmosim_ext_export_tree(self.h, ptr(pool), cap, ctypes.byref(used)),
+        _check(self.L.cosmosim_ext_export_tree(self.h, ptr(pool), cap, ctypes.byref(used)),
                "export_tree")
         return pool[: used.value].copy()
 
     def sync(self):
-        _check(lib().cosmosim_ext_sync(self.h), "sync")
+        _check(self.L.cosmosim_ext_sync(self.h), "sync")
+
+    def debug_fail_build(self, nth):
+        """Test hook: the nth tree build from now fails as an over-capacity tree."""
+        _check(self.L.cosmosim_ext_debug_fail_build(self.h, int(nth)), "debug_fail_build")
 
     # ------------------------------------------------------------ adjoint
     def force_vjp(self, w):
         """(d a / d x)^T w at the current positions (tree and decisions frozen)."""
         w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.n, 3)
         out = np.zeros((self.n, 3))
-        _check(lib().cosmosim_ext_force_vjp(self.h, ptr(w), ptr(out)), "force_vjp")
+        _check(self.L.cosmosim_ext_force_vjp(self.h, ptr(w), ptr(out)), "force_vjp")
         return out
 
     def forces_frozen(self, pos):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
         out = np.zeros((self.n, 3))
-        _check(lib().cosmosim_ext_forces_frozen(self.h, ptr(pos), ptr(out)), "forces_frozen")
+        _check(self.L.cosmosim_ext_forces_frozen(self.h, ptr(pos), ptr(out)), "forces_frozen")
         return out
 
     def direct_vjp(self, w):
         w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.n, 3)
         out = np.zeros((self.n, 3))
-        _check(lib().cosmosim_ext_direct_vjp(self.h, ptr(w), ptr(out)), "direct_vjp")
+        _check(self.L.cosmosim_ext_direct_vjp(self.h, ptr(w), ptr(out)), "direct_vjp")
         return out
 
     def record(self, substeps, checkpoint_every=10, keep_trees=False):
-        _check(lib().cosmosim_ext_record(self.h, substeps, checkpoint_every, int(keep_trees)),
+        _check(self.L.cosmosim_ext_record(self.h, substeps, checkpoint_every, int(keep_trees)),
                "record")
 
     def backward(self, gpos, gvel=None, mode=0):
@@ -485,7 +505,7 @@ class Simulation(Handle):
         gp = np.ascontiguousarray(gpos, dtype=np.float64).reshape(self.n, 3)
         gv = None if gvel is None else np.ascontiguousarray(gvel, dtype=np.float64).reshape(self.n, 3)
         xb, vb, st = np.zeros((self.n, 3)), np.zeros((self.n, 3)), np.zeros(2)
-        _check(lib().cosmosim_ext_backward(self.h, ptr(gp), ptr(gv), mode, ptr(xb), ptr(vb),
+        _check(self.L.cosmosim_ext_backward(self.h, ptr(gp), ptr(gv), mode, ptr(xb), ptr(vb),
                                            ptr(st)), "backward")
         return xb, vb, st
 
@@ -493,6 +513,6 @@ class Simulation(Handle):
         p0 = np.ascontiguousarray(pos0, dtype=np.float64).reshape(self.n, 3)
         v0 = np.ascontiguousarray(vel0, dtype=np.float64).reshape(self.n, 3)
         p, v = np.zeros((self.n, 3)), np.zeros((self.n, 3))
-        _check(lib().cosmosim_ext_replay(self.h, ptr(p0), ptr(v0), ptr(p), ptr(v)), "replay")
+        _check(self.L.cosmosim_ext_replay(self.h, ptr(p0), ptr(v0), ptr(p), ptr(v)), "replay")
         return p, v
 

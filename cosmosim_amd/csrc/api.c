@@ -54,8 +54,9 @@ static QuasarConfigABI quasar_defaults(void)
     return q;
 }
 
-/* Initial conditions are the caller's code (src/initial_conditions.c): weak
- * references. */
+/* Initial conditions are the caller's code (src/initial_conditions.c), linked
+ * into this library by the build's --ic-source option as the reference's
+ * shared-library target links them: weak references, NULL when absent. */
 extern void generate_spiral_galaxy(Body *, int, double, double, double, double, double, double)
     __attribute__((weak));
 extern void generate_merger(Body *, int, double, double) __attribute__((weak));
@@ -87,24 +88,6 @@ static void state_free(SimState *s)
     csg_host_free(s->bodies);
     free(s->pool);
     free(s);
-}
-
-/* Seeded stand-in used only when the caller's generators are not linked: the
- * library's own recipes, scaled to the reference API's galaxy mass 100 and disk
- * radius 15 (mass x100, length x3, velocity x sqrt(100/3)). */
-static int fallback_ics(Body *bodies, int n, int merger)
-{
-    int st = cosmosim_ext_generate(merger && n >= 4 ? COSMOSIM_IC_MERGER : COSMOSIM_IC_DISK, n,
-                                   0x5eedull + (unsigned long long)n, NULL, bodies);
-    if (st)
-        return st;
-    const double L = API_DISK_RADIUS / 5.0, Mf = API_GALAXY_MASS, V = sqrt(Mf / L);
-    for (int i = 0; i < n; ++i) {
-        bodies[i].x *= L; bodies[i].y *= L; bodies[i].z *= L;
-        bodies[i].vx *= V; bodies[i].vy *= V; bodies[i].vz *= V;
-        bodies[i].mass *= Mf;
-    }
-    return COSMOSIM_OK;
 }
 
 COSMOSIM_API CosmosimConfig cosmosim_default_config(void)
@@ -159,13 +142,19 @@ COSMOSIM_API SimHandle cosmosim_create(CosmosimConfig config)
     s->config = config;
     default_params(&s->params, &config);
 
+    const int have_ics = config.quasar
+                             ? (generate_quasar_merger && generate_quasar_galaxy &&
+                                generate_merger_dust)
+                             : (config.merger ? generate_merger != NULL
+                                              : generate_spiral_galaxy != NULL);
+    if (!have_ics) {
+        cs_set_error(COSMOSIM_E_UNSUPPORTED,
+                     "cosmosim_create (the library was built without the caller's "
+                     "initial_conditions.c: build with --ic-source)");
+        state_free(s);
+        return NULL;
+    }
     if (config.quasar) {
-        if (!generate_quasar_merger || !generate_quasar_galaxy || !generate_merger_dust) {
-            cs_set_error(COSMOSIM_E_UNSUPPORTED,
-                         "cosmosim_create (quasar mode needs initial_conditions.c)");
-            state_free(s);
-            return NULL;
-        }
         s->pool = (OctreeNode *)malloc(8 * (size_t)n_alloc * sizeof(OctreeNode));
         if (!s->pool) {
             state_free(s);
@@ -180,13 +169,10 @@ COSMOSIM_API SimHandle cosmosim_create(CosmosimConfig config)
             generate_quasar_galaxy(s->bodies, n, 0.0, 0.0, (double)n * 2.0, API_DISK_RADIUS, 0.0,
                                    0.0, config.smbh_mass_frac);
         }
-    } else if (config.merger && generate_merger) {
+    } else if (config.merger) {
         generate_merger(s->bodies, n, API_SEPARATION, API_APPROACH_VEL);
-    } else if (!config.merger && generate_spiral_galaxy) {
+    } else {
         generate_spiral_galaxy(s->bodies, n, 0.0, 0.0, API_GALAXY_MASS, API_DISK_RADIUS, 0.0, 0.0);
-    } else if (fallback_ics(s->bodies, n, config.merger) != COSMOSIM_OK) {
-        state_free(s);
-        return NULL;
     }
 
     const int initial_n = n + n_dust;
@@ -673,6 +659,18 @@ COSMOSIM_API int cosmosim_ext_quasar_state(SimHandle h, unsigned long long *rng,
     return h ? csg_quasar_state(((SimState *)h)->ctx, rng, phase, set) : COSMOSIM_E_ARG;
 }
 
+COSMOSIM_API int cosmosim_ext_quasar_config(SimHandle h, CosmosimQuasarConfig *cfg, int set)
+{
+    if (!h || !cfg)
+        return COSMOSIM_E_ARG;
+    SimState *s = (SimState *)h;
+    if (set)
+        memcpy(&s->qcfg, cfg, sizeof(s->qcfg));
+    else
+        memcpy(cfg, &s->qcfg, sizeof(s->qcfg));
+    return COSMOSIM_OK;
+}
+
 COSMOSIM_API int cosmosim_ext_snapshot_ue(SimHandle h, float *out, int max_bodies, int *count)
 {
     return snapshot(h, 0, out, max_bodies, count);
@@ -681,6 +679,11 @@ COSMOSIM_API int cosmosim_ext_snapshot_ue(SimHandle h, float *out, int max_bodie
 COSMOSIM_API int cosmosim_ext_snapshot_render(SimHandle h, float *out, int *count)
 {
     return snapshot(h, 1, out, h ? ((SimState *)h)->current_n : 0, count);
+}
+
+COSMOSIM_API int cosmosim_ext_debug_fail_build(SimHandle h, int nth)
+{
+    return h ? csg_debug_fail_build(((SimState *)h)->ctx, nth) : COSMOSIM_E_ARG;
 }
 
 COSMOSIM_API int cosmosim_ext_handle_error(SimHandle h)

@@ -921,6 +921,9 @@ int csg_record(csg_ctx *c, int k, double dt, double G, double softening, double 
     if (k % every == 0 && (s = state_copy(c, c->ckpt + slot * (k / every), 1))) return s;
     if ((s = state_copy(c, c->ckpt + slot * (nck - 1), 1))) return s;
     CSG_CHECK(hipStreamSynchronize(c->stream));
+    // a build that overflowed anywhere in the run leaves empty tapes and stale
+    // accelerations behind it: the recording is not valid then
+    if ((s = check_overflow(c))) return s;
     c->rec_valid = 1;
     return COSMOSIM_OK;
 }
@@ -1027,6 +1030,7 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
     if ((s = state_copy(c, c->ckpt + slot * (nck - 1), 0))) return s;
     if ((s = launch_build(c))) return s;
     CSG_CHECK(hipStreamSynchronize(st));
+    if ((s = check_overflow(c))) return s;
     if (stats) {
         stats[0] = xs_dev;
         size_t bytes = (size_t)c->tape_waves * c->tape_cap * sizeof(TapeRec) +

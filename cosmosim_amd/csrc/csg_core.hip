@@ -46,6 +46,10 @@ struct csg_ctx {
     int *icount = nullptr;
     unsigned long long *itotal = nullptr;
     int tree_valid = 0;
+    int fail_seen = 0;         // BuildParams::fail_count as of the last overflow check,
+                               // including the build that check examined
+    int debug_fail_build = 0;  // test hook: the n-th build from now runs with no node
+                               // capacity (overflows like a tree that does not fit)
     // export / foreign pool / diagnostics
     OctreeNode *pool_dev = nullptr;
     size_t pool_dev_cap = 0;
@@ -369,6 +373,8 @@ int csg_create(int device, int capacity, csg_ctx **out) {
         return COSMOSIM_E_ALLOC;
     }
     memset(c->hbp, 0, sizeof(BuildParams));
+    // the build counters start at zero (k_bbox_final counts earlier failures)
+    clear_words((int *)c->bp, sizeof(BuildParams) / sizeof(int), c->stream);
     int s = alloc_buffers(c, capacity);
     if (s != COSMOSIM_OK) {
         csg_destroy(c);
@@ -605,8 +611,10 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const int nbb = bbox > 0 ? bbox : bbox_blocks(n);
+    int node_cap = c->node_cap;
+    if (c->debug_fail_build > 0 && --c->debug_fail_build == 0) node_cap = 0;
     if (bbox <= 0) k_bbox_partial<<<nbb, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bbox_part);
-    k_bbox_final<<<1, CSG_TPB, 0, st>>>(c->bbox_part, nbb, c->bp, c->node_cap, c->rs_ctl);
+    k_bbox_final<<<1, CSG_TPB, 0, st>>>(c->bbox_part, nbb, c->bp, node_cap, c->rs_ctl);
     CSG_LAUNCHED();
     if (n > 0) {
         const int nt = blocks_for(n, RS_TILE);
@@ -753,8 +761,10 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         const bool taping = tape.rec != nullptr;
         const int v = (lane_counts ? 4 : 0) | (taping ? 2 : 0) | (off32 ? 1 : 0);
         if (v < 2 && !count_total) {
-            // softened: the own leaf adds exactly 0, so it needs no exclusion
-            if (off32 && eps2 > 0.0)
+            // softened: the own leaf's offset is exactly 0 and its factor
+            // m / eps^3 is finite in FP32 (eps^2 >= 1e-12 keeps it below
+            // m * 1e18), so it adds exactly 0 and needs no exclusion
+            if (off32 && (float)eps2 >= 1e-12f)
                 launch_walk32w<false, false, true, false, false>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
             else if (off32)
                 launch_walk32w<false, false, true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
@@ -826,11 +836,17 @@ int csg_kick(csg_ctx *c, double dt) {
     return COSMOSIM_OK;
 }
 
+// After a run of builds: fails when the last build or any build since the
+// previous check overflowed or stalled (the walks after such a build do
+// nothing, so the accelerations the run went on with are stale).
 static int check_overflow(csg_ctx *c) {
     int s = read_params(c);
     if (s) return s;
     collect_timers(c);
-    if (c->hbp->overflow) {
+    const int pending = (c->hbp->overflow || c->hbp->sort_stall) ? 1 : 0;
+    const bool earlier = c->hbp->fail_count != c->fail_seen;
+    c->fail_seen = c->hbp->fail_count + pending;
+    if (pending || earlier) {
         c->tree_valid = 0;
         c->counts_valid = 0;
         if (c->hbp->sort_stall) {
@@ -1078,6 +1094,13 @@ int csg_set_timing(csg_ctx *c, int enabled) {
     c->ms_build = c->ms_walk = c->ms_kdk = c->ms_nb = c->ms_sph = 0.0;
     c->timed_walks = 0;
     c->nb_queries = 0;
+    return COSMOSIM_OK;
+}
+
+int csg_debug_fail_build(csg_ctx *c, int nth) {
+    CSG_USE(c);
+    if (nth < 0) return COSMOSIM_E_ARG;
+    c->debug_fail_build = nth;
     return COSMOSIM_OK;
 }
 

@@ -54,7 +54,9 @@ struct BuildParams {
     int overflow;    // 1 when n_nodes exceeds the node capacity
     int node_cap;
     int sort_stall;  // 1 when a radix pass gave up waiting for an earlier tile
-    int pad;
+    int fail_count;  // builds that overflowed or stalled before the current one (never
+                     // reset by a build: a failure in the middle of a substep run stays
+                     // visible to the check at the run's end)
     unsigned long long interactions;  // body-node interactions of the last walk
 };
 
@@ -413,6 +415,8 @@ __global__ __launch_bounds__(CSG_TPB) void k_bbox_final(const double *__restrict
     }
     if (threadIdx.x == 0) {
         const int live = (int)s[6][0];
+        // the previous build's failure is counted before its flags are cleared
+        if (bp->overflow || bp->sort_stall) bp->fail_count += 1;
         bp->n_live = live;
         bp->n_runs = 0;
         bp->max_depth = 0;

@@ -253,30 +253,22 @@ void octree_compute_forces(const OctreeNode *pool, int root, Body *bodies, int n
 /* Sphere query for the SPH consumer (src/octree.c:236-318).  It runs on the
  * host because its callers (src/sph.c:57,120) issue one small query per
  * particle from host OpenMP loops over a host pool; the gravity hot path does
- * not use it. */
-static void nb_visit(const OctreeNode *pool, int v, const Body *bodies, double qx, double qy,
-                     double qz, double r2, int *buf, int *count, int cap)
+ * not use it, and batched queries run on the device
+ * (cosmosim_ext_find_neighbors).
+ *
+ * Depth-first over an explicit stack: children are pushed in reverse slot
+ * order, so they pop in slot order, which is the order the reported bodies
+ * must follow (the cap cuts the list).  A node whose cell misses the sphere is
+ * dropped when popped; a leaf reports its body when it is live and inside.
+ * Once the list is full nothing more can be reported, so the walk stops. */
+#define NB_STACK (8 * 65 + 8) /* 8 children per level, depth <= 64 */
+
+static int cell_misses_sphere(const OctreeNode *nd, double qx, double qy, double qz, double r2)
 {
-    const OctreeNode *nd = &pool[v];
-    double ex = fmax(0.0, fabs(qx - nd->center_x) - nd->half_size);
-    double ey = fmax(0.0, fabs(qy - nd->center_y) - nd->half_size);
-    double ez = fmax(0.0, fabs(qz - nd->center_z) - nd->half_size);
-    if (ex * ex + ey * ey + ez * ez > r2)
-        return;
-    if (nd->body_index >= 0) {
-        if (*count >= cap)
-            return;
-        const Body *b = &bodies[nd->body_index];
-        if (b->mass <= 0.0)
-            return;
-        double dx = b->x - qx, dy = b->y - qy, dz = b->z - qz;
-        if (dx * dx + dy * dy + dz * dz <= r2)
-            buf[(*count)++] = nd->body_index;
-        return;
-    }
-    for (int c = 0; c < 8; ++c)
-        if (nd->children[c] >= 0 && *count < cap)
-            nb_visit(pool, nd->children[c], bodies, qx, qy, qz, r2, buf, count, cap);
+    const double gx = fmax(0.0, fabs(qx - nd->center_x) - nd->half_size);
+    const double gy = fmax(0.0, fabs(qy - nd->center_y) - nd->half_size);
+    const double gz = fmax(0.0, fabs(qz - nd->center_z) - nd->half_size);
+    return gx * gx + gy * gy + gz * gz > r2;
 }
 
 int octree_find_neighbors(const OctreeNode *pool, int root, const Body *bodies, int n, double cx,
@@ -284,11 +276,33 @@ int octree_find_neighbors(const OctreeNode *pool, int root, const Body *bodies, 
                           int buf_capacity)
 {
     (void)n;
-    int count = 0;
     if (!pool || !bodies || !neighbor_buf)
         return 0;
-    nb_visit(pool, root, bodies, cx, cy, cz, radius * radius, neighbor_buf, &count, buf_capacity);
-    return count;
+    const double r2 = radius * radius;
+    int stack[NB_STACK];
+    int top = 0, found = 0;
+    stack[top++] = root;
+    while (top > 0 && found < buf_capacity) {
+        const OctreeNode *nd = &pool[stack[--top]];
+        if (cell_misses_sphere(nd, cx, cy, cz, r2))
+            continue;
+        const int bi = nd->body_index;
+        if (bi >= 0) {
+            const Body *b = &bodies[bi];
+            if (b->mass > 0.0) {
+                const double ox = b->x - cx, oy = b->y - cy, oz = b->z - cz;
+                if (ox * ox + oy * oy + oz * oz <= r2)
+                    neighbor_buf[found++] = bi;
+            }
+            continue;
+        }
+        for (int slot = 7; slot >= 0; --slot) {
+            const int child = nd->children[slot];
+            if (child >= 0 && top < NB_STACK)
+                stack[top++] = child;
+        }
+    }
+    return found;
 }
 
 /* ------------------------------------------------------------ src/integrator.h */

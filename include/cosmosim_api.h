@@ -29,17 +29,24 @@
 #define COSMOSIM_API __attribute__((visibility("default")))
 #endif
 
+/* Field order and types are the ABI; the meaning of each field:
+ *   n_bodies        bodies the initial conditions make (at least 2)
+ *   merger, quasar  flags: two galaxies instead of one; black-hole physics
+ *                   (SPH, accretion, jets) between substeps
+ *   dt, theta       substep length; the cell-acceptance angle of the walk
+ *   smbh_mass_frac .. feedback_strength: quasar-mode black-hole parameters
+ *   substeps        KDK substeps advanced by one cosmosim_step */
 typedef struct {
     int n_bodies;
-    int merger;            /* 0 or 1 */
-    int quasar;            /* 0 or 1 */
-    double dt;             /* timestep */
-    double theta;          /* Barnes-Hut opening angle */
-    double smbh_mass_frac; /* SMBH mass fraction (quasar mode) */
+    int merger;
+    int quasar;
+    double dt;
+    double theta;
+    double smbh_mass_frac;
     double accretion_radius;
     double jet_speed;
     double feedback_strength;
-    int substeps;          /* integration substeps per cosmosim_step call */
+    int substeps;
 } CosmosimConfig;
 
 COSMOSIM_STATIC_ASSERT(sizeof(CosmosimConfig) == 72, "CosmosimConfig must stay 72 bytes");

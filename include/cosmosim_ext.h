@@ -172,6 +172,14 @@ int cosmosim_ext_reserve(SimHandle h, int capacity);
 int cosmosim_ext_quasar_step(SimHandle h, CosmosimQuasarConfig *cfg, double dt);
 int cosmosim_ext_quasar_compact(SimHandle h, int *n);
 int cosmosim_ext_quasar_state(SimHandle h, unsigned long long *rng, int *phase, int set);
+/* Get (set = 0) or set (set = 1) the QuasarConfig that the quasar-mode
+ * cosmosim_step passes to its quasar steps (cosmosim_create fills it as
+ * src/cosmosim_api.c:86-94 does; jet_count is updated by every step). */
+int cosmosim_ext_quasar_config(SimHandle h, CosmosimQuasarConfig *cfg, int set);
+/* Test hook for the capacity error path: the nth tree build from now (1 = the
+ * next) fails as a tree needing more than the node capacity would, so a call
+ * that runs through it must return COSMOSIM_E_CAPACITY; 0 cancels. */
+int cosmosim_ext_debug_fail_build(SimHandle h, int nth);
 /* Last error recorded on the handle by a void legacy call (cosmosim_step). */
 int cosmosim_ext_handle_error(SimHandle h);
 /* Last error recorded on the calling thread by octree_* / integrator_*. */

@@ -123,6 +123,9 @@ int  csg_get_stats(csg_ctx *ctx, csg_stats *out);
  * recorded without host waits and collected by csg_get_stats. */
 int  csg_set_timing(csg_ctx *ctx, int enabled);
 int  csg_synchronize(csg_ctx *ctx);
+/* Test hook: the nth build from now (1 = the next) runs with no node capacity,
+ * so it fails as a tree that does not fit would; 0 cancels. */
+int  csg_debug_fail_build(csg_ctx *ctx, int nth);
 
 /* Adjoint (no counterpart in the reference).
  * csg_walk_taped: the walk, also recording its tape (the visited nodes and the

@@ -5,14 +5,27 @@ C0: 20,001-body disk (seed 1), theta 0.5, eps 0.01, dt 1e-3, 100 substeps.  The
     10 and 100 substeps), and so are the energy and momentum drifts.
 C1: 100,000-body merger (seed 2), theta 0.5, eps 0.01, dt 1e-3, 1000 substeps:
     per-body accelerations against the oracle walk at step 0 and on the GPU's
-    state at step 1000, and energy / momentum drift over the run.
+    state at step 1000; the energy and momentum every 10 substeps against the
+    curves the reference's own integrator_step produced for the same bodies
+    (tests/golden/c1_drift.npz, written by gen_golden_c1_drift.py from the
+    compiled reference).  The two runs differ only in the last bits of the
+    centres of mass, so their energies agree to 1e-10 of |E| and their drift
+    curves to 1e-3 of the reference's drift; momenta to 1e-12 absolute.
+C2: 1,000,000-body merger (seed 3), eps 0.005, theta 0.4 / 0.5 / 0.7: the FP32
+    walk against the FP64 walk on the same tree over every body (RMS <= 1e-5;
+    max <= 5e-3, reached only by bodies whose FP32 opening decisions differ from
+    FP64 at a node within rounding of the opening boundary: bodies with the same
+    interaction count stay within 1e-5 (measured 0.6-1.5e-6), and fewer than
+    1e-4 of the bodies differ (measured 2e-6 - 3e-5)),
+    and both against the FP64 direct sum on a random 4096-body subset (the
+    reference method's own error).
 """
 import numpy as np
 import pytest
 
 import cosmosim_amd as ca
 from oracle.cpu_ref import CpuRef
-from tests._util import acc, rel_err
+from tests._util import acc, golden, rel_err
 
 pytestmark = pytest.mark.gpu
 ref = CpuRef()
@@ -77,3 +90,64 @@ def test_config1_merger_1000_substeps():
     # Barnes-Hut monopole forces are not pairwise symmetric: momentum drifts at
     # the level of the method's force error (the oracle shows the same at C0)
     assert np.max(np.abs(e1[2] - e0[2])) < 1e-3
+
+
+def test_config1_drift_curves_match_reference():
+    g = golden("c1_drift.npz")
+    b = ca.generate(ca.IC_MERGER, 100000, 2)
+    sim = ca.Simulation(b, softening=0.01, theta=0.5, dt=1e-3, precision="fp64_strict")
+    E_ref = g["kinetic"] + g["potential"]
+    P_ref = g["momentum"]
+    E, P = [], []
+    for k, s in enumerate(g["steps"]):
+        if k:
+            sim.advance(int(s - g["steps"][k - 1]))
+        ke, pe, p = sim.energy()
+        E.append(ke + pe)
+        P.append(p)
+    E, P = np.array(E), np.array(P)
+    dE = np.max(np.abs(E - E_ref)) / abs(E_ref[0])
+    drift_ref = (E_ref - E_ref[0]) / abs(E_ref[0])
+    drift = (E - E[0]) / abs(E[0])
+    dd = np.max(np.abs(drift - drift_ref))
+    dP = np.max(np.abs((P - P[0]) - (P_ref - P_ref[0])))
+    print(f"\nC1 over 1000 substeps: reference energy drift {np.max(np.abs(drift_ref)):.3e}, "
+          f"GPU {np.max(np.abs(drift)):.3e}, curves differ by {dd:.3e}; |E - E_ref| / |E| "
+          f"{dE:.3e}; momentum drift {np.max(np.abs(P_ref - P_ref[0])):.3e}, differs by "
+          f"{dP:.3e}", flush=True)
+    assert dE <= 1e-10
+    assert dd <= 1e-3 * np.max(np.abs(drift_ref)) + 1e-13
+    assert dP <= 1e-12
+    # sampled positions after 1000 substeps
+    pos = sim.state()[0][g["final_index"]]
+    print(f"sampled positions after 1000 substeps differ by {np.max(np.abs(pos - g['final_pos'])):.3e}",
+          flush=True)
+
+
+@pytest.mark.parametrize("theta", [0.4, 0.5, 0.7])
+def test_config2_fp32_accuracy_at_1m(theta):
+    n = 1_000_000
+    b = ca.generate(ca.IC_MERGER, n, 3)
+    sim = ca.Simulation(b, softening=0.005, theta=theta, dt=5e-4, precision="fp64")
+    a64, c64 = sim.accelerations(), sim.interaction_counts()
+    sim.set_params(precision="fp32")
+    sim.compute_forces()
+    a32, c32 = sim.accelerations(), sim.interaction_counts()
+    e = rel_err(a32, a64)
+    same = c32 == c64
+    rms = float(np.sqrt(np.mean(e ** 2)))
+    idx = np.sort(np.random.default_rng(12345).choice(n, 4096, replace=False)).astype(np.int32)
+    d = sim.direct_forces(idx)
+    eb, ec = rel_err(a64[idx], d), rel_err(a32[idx], d)
+    rb, rc = float(np.sqrt(np.mean(eb ** 2))), float(np.sqrt(np.mean(ec ** 2)))
+    print(f"\ntheta {theta}: FP32 vs FP64 rms {rms:.3e} max {e.max():.3e}; bodies with other "
+          f"counts {np.mean(~same):.2e}, max over the rest {e[same].max():.3e}; vs direct sum: "
+          f"FP64 rms {rb:.3e} max {eb.max():.3e}, FP32 rms {rc:.3e} max {ec.max():.3e}",
+          flush=True)
+    assert rms <= 1e-5
+    assert e.max() <= 5e-3
+    assert e[same].max() <= 1e-5
+    assert np.mean(~same) <= 1e-4
+    # the method's own error (the reference's monopole criterion at this theta)
+    assert rb <= {0.4: 1e-2, 0.5: 2e-2, 0.7: 4e-2}[theta]
+    assert abs(rc - rb) <= 1e-5

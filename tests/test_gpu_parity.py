@@ -396,36 +396,6 @@ def test_fp64_consumers_after_fp32_substeps():
     assert same_bits(sim.force_vjp(w), fresh3.force_vjp(w))
 
 
-# ------------------------------------------------------------ handle API
-def test_api_handle_like_reference_tests():
-    """tests/test_physics.c:1048-1111 restated."""
-    h = ca.Handle(n_bodies=1000, merger=0, quasar=0)
-    assert h.count() == 1000 and h.active_count() == 1000 and h.sim_time() == 0.0
-    b0 = h.bodies()
-    h.step()
-    assert abs(h.sim_time() - h.config.dt * h.config.substeps) < 1e-12
-    b1 = h.bodies()
-    assert np.any(b1["vx"] != b0["vx"])
-    h.close()
-    h = ca.Handle(n_bodies=1, merger=1)     # n < 2 becomes 2 (src/cosmosim_api.c:49-51)
-    assert h.count() == 2
-    h.close()
-
-
-def test_api_step_equals_legacy_substeps():
-    """cosmosim_step's device-resident loop == integrator_step per substep."""
-    h = ca.Handle(n_bodies=2000, merger=1, quasar=0, substeps=3)
-    start = h.bodies()
-    h.step()
-    got = h.bodies()
-    bb = start.copy()
-    pool = np.zeros(8 * len(bb), dtype=ca.NODE_DTYPE)
-    for _ in range(3):
-        ca.integrator_step(bb, h.config.dt, 1.0, 1.5, h.config.theta, pool)
-    assert same_bits(hot(got), hot(bb))
-    h.close()
-
-
 # ------------------------------------------------------------ full-size properties
 def test_million_body_merger_properties():
     """At the benchmark size (1M merger): tree invariants, interaction counts,
