@@ -435,10 +435,14 @@ static double cref_spline_dr(double r, double h)       /* src/sph.c:27-41 */
 
 static inline int cref_gas(const Body *b) { return b->type == BODY_GAS && !(b->mass <= 0.0); }
 
+/* Both passes run their bodies under OpenMP as src/sph.c:50,106 does; each body
+ * writes only its own fields and sums its neighbours in list order, so the
+ * results do not depend on the thread count. */
 void cref_sph_density(Body *bodies, int n, const OctreeNode *pool)
 {
-    int list[CREF_SPH_CAP];
+#pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < n; ++i) {
+        int list[CREF_SPH_CAP];
         Body *bi = &bodies[i];
         if (!cref_gas(bi))
             continue;
@@ -469,8 +473,9 @@ void cref_sph_density(Body *bodies, int n, const OctreeNode *pool)
 
 void cref_sph_forces(Body *bodies, int n, const OctreeNode *pool)
 {
-    int list[CREF_SPH_CAP];
+#pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < n; ++i) {
+        int list[CREF_SPH_CAP];
         Body *bi = &bodies[i];
         if (!cref_gas(bi))
             continue;
