@@ -84,6 +84,8 @@ _SIGS = {
     "cosmosim_ext_last_error": (_I, []),
     "cosmosim_ext_clear_error": (None, []),
     "cosmosim_ext_error_string": (ctypes.c_char_p, [_I]),
+    "cosmosim_ext_forces_noalloc": (_I, [_VP, _VP, _VP, _VP, _I, _D, _D, _D, _I, _VP, _VP,
+                                         _VP, _VP, _VP]),
     "cosmosim_ext_device_info": (_I, [_I, ctypes.c_char_p, _I, _VP]),
     "cosmosim_ext_device_memory": (_I, [_I, _VP, _VP]),
     "cosmosim_ext_force_vjp": (_I, [_VP, _VP, _VP]),

@@ -7,6 +7,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "csg_device.h"
 #include "cosmosim_ext.h"
 #include "csg.h"
@@ -1101,6 +1103,150 @@ int csg_debug_fail_build(csg_ctx *c, int nth) {
     CSG_USE(c);
     if (nth < 0) return COSMOSIM_E_ARG;
     c->debug_fail_build = nth;
+    return COSMOSIM_OK;
+}
+
+}  // extern "C"
+
+// ================================================ allocation-free force call
+// cosmosim_ext_forces_noalloc: the tree build and walk of
+// cosmosim_ext_compute_forces on caller-supplied device arrays, with every
+// other buffer carved out of one file-scope device array instead of hipMalloc:
+// keys and sort buffers, collision words, sorted copies, prefix sums, node
+// records, dispatch and counter words.  The array holds the regions of at most
+// CSG_NOALLOC_MAX_BODIES bodies; each GPU the library runs on keeps its own
+// copy until the process ends, and every call on that GPU reuses it.
+#define CSG_NOALLOC_MAX_BODIES 16384
+enum {
+    NA_BBOX, NA_KEY, NA_KEY2, NA_EXT, NA_EXT_TMP, NA_IDX, NA_IDX2, NA_XS, NA_YS, NA_ZS, NA_MS,
+    NA_DEPTH, NA_RUNS, NA_LCP, NA_CNT, NA_NBASE, NA_SCAN, NA_PREF, NA_DDPART, NA_TPREF,
+    NA_NODES, NA_NODES32, NA_ICOUNT, NA_WCOST, NA_WORDER, NA_XYZM, NA_OWNER, NA_RSCTL,
+    NA_RSSTATUS, NA_BP, NA_ITOTAL, NA_REGIONS
+};
+struct NoallocLayout {
+    size_t off[NA_REGIONS];
+    size_t total;
+};
+// Byte offset of every region for `cap` bodies, each 256-byte aligned; the
+// sizes are alloc_buffers' for the buffers launch_build and launch_walk use.
+constexpr NoallocLayout noalloc_layout(int cap) {
+    NoallocLayout L{};
+    const size_t C = (size_t)(cap < 1 ? 1 : cap), C1 = C + 1;
+    const size_t nb = (C1 + CSG_TILE - 1) / CSG_TILE;
+    const size_t nodes = 8 * C + 4096, waves = C / 64 + 1;
+    const size_t bytes[NA_REGIONS] = {
+        (size_t)CSG_BBOX_BLOCKS * 8 * sizeof(double),
+        C * sizeof(uint64_t), C * sizeof(uint64_t), 3 * C * sizeof(uint64_t),
+        3 * C * sizeof(uint64_t), C * sizeof(int), C * sizeof(int),
+        C * sizeof(double), C * sizeof(double), C * sizeof(double), C * sizeof(double),
+        ((C1 + CSG_TPB - 1) / CSG_TPB) * sizeof(int), C * sizeof(int), C1 * sizeof(signed char),
+        C1 * sizeof(int), C1 * sizeof(int), nb * sizeof(int), 4 * C1 * sizeof(DD),
+        4 * nb * sizeof(DD), 4 * (size_t)CSG_TPB * nb * sizeof(DD),
+        nodes * sizeof(WNode), nodes * sizeof(WNode32), C * sizeof(int), waves * sizeof(int),
+        (2 * waves + 4096) * sizeof(int), C * sizeof(double4), nodes * sizeof(int),
+        sizeof(RsCtl), 256 * ((C + RS_TILE - 1) / RS_TILE) * sizeof(uint64_t),
+        sizeof(BuildParams), sizeof(unsigned long long)};
+    size_t o = 0;
+    for (int r = 0; r < NA_REGIONS; ++r) {
+        L.off[r] = o;
+        o += (bytes[r] + 255) / 256 * 256;
+    }
+    L.total = o;
+    return L;
+}
+#define CSG_NOALLOC_BYTES (noalloc_layout(CSG_NOALLOC_MAX_BODIES).total)
+// The host must not use this name as a device address: kernels get the address
+// hipGetSymbolAddress returns for the current device (noalloc_scratch below).
+static __device__ __align__(256) unsigned char scratch[CSG_NOALLOC_BYTES];
+
+#define CSG_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[CSG_MAX_DEVICES];
+
+// The scratch array's device address on `dev` (the current device), looked up
+// once per device.
+static int noalloc_scratch(int dev, unsigned char **out) {
+    if (dev < 0 || dev >= CSG_MAX_DEVICES) return COSMOSIM_E_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "cosmosim: hipGetSymbolAddress(scratch) failed: %s\n",
+                    hipGetErrorString(e));
+            (void)hipGetLastError();
+            return COSMOSIM_E_HIP;
+        }
+        g_scratch_ptr[dev] = (unsigned char *)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return COSMOSIM_OK;
+}
+
+// status: 0, or the code a build that overflowed (or stalled) would return.
+__global__ void k_noalloc_status(const BuildParams *bp, int *status) {
+    *status = bp->sort_stall ? COSMOSIM_E_HIP : (bp->overflow ? COSMOSIM_E_CAPACITY : 0);
+}
+
+extern "C" {
+
+int cosmosim_ext_forces_noalloc(const double *x, const double *y, const double *z,
+                                const double *m, int n, double G, double softening,
+                                double theta, int precision, double *ax, double *ay,
+                                double *az, int *status, void *stream) {
+    if (n < 0 || (n > 0 && (!x || !y || !z || !m || !ax || !ay || !az)) || !status ||
+        precision < COSMOSIM_PREC_FP64_STRICT || precision > COSMOSIM_PREC_FP32 ||
+        !(theta >= 0.0) || !(softening >= 0.0))
+        return COSMOSIM_E_ARG;
+    if (n > CSG_NOALLOC_MAX_BODIES) {
+        fprintf(stderr, "cosmosim: cosmosim_ext_forces_noalloc takes at most %d bodies (%d)\n",
+                CSG_NOALLOC_MAX_BODIES, n);
+        return COSMOSIM_E_CAPACITY;
+    }
+    int dev = 0;
+    CSG_CHECK(hipGetDevice(&dev));
+    unsigned char *base = nullptr;
+    int s = noalloc_scratch(dev, &base);
+    if (s) return s;
+    const NoallocLayout L = noalloc_layout(n);
+    if (L.total > CSG_NOALLOC_BYTES) return COSMOSIM_E_CAPACITY;   // not reached: n <= max
+    auto at = [&](int r) { return (void *)(base + L.off[r]); };
+    // a context on the stack whose buffers are the caller's arrays and the scratch
+    csg_ctx c;
+    c.device = dev;
+    c.stream = (hipStream_t)stream;
+    c.cap = n > 0 ? n : 1;
+    c.n = n;
+    c.node_cap = 8 * c.cap + 4096;
+    c.x = (double *)x; c.y = (double *)y; c.z = (double *)z; c.m = (double *)m;
+    c.ax = ax; c.ay = ay; c.az = az;
+    c.bbox_part = (double *)at(NA_BBOX);
+    c.key = (uint64_t *)at(NA_KEY); c.key2 = (uint64_t *)at(NA_KEY2);
+    c.ext = (uint64_t *)at(NA_EXT); c.ext_tmp = (uint64_t *)at(NA_EXT_TMP);
+    c.idx = (int *)at(NA_IDX); c.idx2 = (int *)at(NA_IDX2);
+    c.xs = (double *)at(NA_XS); c.ys = (double *)at(NA_YS); c.zs = (double *)at(NA_ZS);
+    c.ms = (double *)at(NA_MS);
+    c.depth_part = (int *)at(NA_DEPTH); c.runs = (int *)at(NA_RUNS);
+    c.lcp = (signed char *)at(NA_LCP); c.cnt = (int *)at(NA_CNT); c.nbase = (int *)at(NA_NBASE);
+    c.scan_part = (int *)at(NA_SCAN); c.pref = (DD *)at(NA_PREF); c.dd_part = (DD *)at(NA_DDPART);
+    c.tpref = (DD *)at(NA_TPREF); c.nodes = (WNode *)at(NA_NODES);
+    c.nodes32 = (WNode32 *)at(NA_NODES32); c.icount = (int *)at(NA_ICOUNT);
+    c.wcost = (int *)at(NA_WCOST); c.worder = (int *)at(NA_WORDER);
+    c.xyzm = (double4 *)at(NA_XYZM); c.owner = (int *)at(NA_OWNER);
+    c.rs_ctl = (RsCtl *)at(NA_RSCTL); c.rs_status = (uint64_t *)at(NA_RSSTATUS);
+    c.bp = (BuildParams *)at(NA_BP); c.itotal = (unsigned long long *)at(NA_ITOTAL);
+    // the array keeps the last call's contents: start the words the build reads
+    // before writing them from zero (fill kernels, no memset nodes)
+    clear_words((int *)c.bp, sizeof(BuildParams) / sizeof(int), c.stream);
+    clear_words(c.rs_status, 256 * (size_t)blocks_for(c.cap, RS_TILE), c.stream);
+    clear_words(c.idx, (size_t)c.cap, c.stream);
+    clear_words(c.idx2, (size_t)c.cap, c.stream);
+    CSG_LAUNCHED();
+    c.theta_hint = theta;
+    if ((s = launch_build(&c, precision != COSMOSIM_PREC_FP32))) return s;
+    if ((s = launch_walk(&c, precision, G, softening, theta))) return s;
+    k_noalloc_status<<<1, 1, 0, c.stream>>>(c.bp, status);
+    CSG_LAUNCHED();
     return COSMOSIM_OK;
 }
 

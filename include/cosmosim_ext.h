@@ -207,6 +207,23 @@ int cosmosim_ext_backward(SimHandle h, const double *gpos, const double *gvel, i
 int cosmosim_ext_replay(SimHandle h, const double *pos0, const double *vel0, double *pos,
                         double *vel);
 
+/* Allocation-free force evaluation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory while they run (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The variant of cosmosim_ext_compute_forces (tree build + walk at the given
+ * positions) on caller-owned DEVICE arrays: x, y, z, m [n] in, ax, ay, az [n]
+ * out (SoA, FP64), accelerations of bodies with m <= 0 set to 0.  All scratch
+ * lives in one device array inside the library (about 19 MB per GPU, held
+ * until the process ends), so n is at most 16384 (COSMOSIM_E_CAPACITY beyond).
+ * Work is queued on `stream` (a hipStream_t; NULL = the default stream) and
+ * the call returns without waiting; *status (a device int) receives 0, or
+ * COSMOSIM_E_CAPACITY / COSMOSIM_E_HIP when the tree did not fit or the sort
+ * stalled.  Calls that overlap on one GPU (other threads or streams) share the
+ * scratch and race: order them on one stream. */
+int cosmosim_ext_forces_noalloc(const double *x, const double *y, const double *z,
+                                const double *m, int n, double G, double softening,
+                                double theta, int precision, double *ax, double *ay,
+                                double *az, int *status, void *stream);
 /* Name and compute-unit count of the HIP device (name buffer of len bytes). */
 int cosmosim_ext_device_info(int device, char *name, int len, int *compute_units);
 /* Free and total device memory in bytes. */
