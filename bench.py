@@ -406,6 +406,8 @@ def main():
     ap.add_argument("--no-per-step", action="store_true",
                     help="skip the per-substep subset accuracy runs")
     ap.add_argument("--per-step-steps", type=int, default=200)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 20K-disk and 100K-merger lines")
     ap.add_argument("--no-sph", action="store_true")
     ap.add_argument("--no-quasar", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -450,7 +452,7 @@ def main():
         results["per_step"] = {f"theta{th}": per_step_accuracy(ca, bodies, idx, th,
                                                                args.per_step_steps)
                                for th in thetas}
-    if not args.headline_only:
+    if not args.headline_only and not args.no_configs:
         results["config0_20001_disk"] = config_line(ca, ca.IC_DISK, 20001, 1, 0.5, 0.01, 1e-3,
                                                     100, "fp64_strict")
         results["config1_100000_merger"] = config_line(ca, ca.IC_MERGER, 100000, 2, 0.5, 0.01,
