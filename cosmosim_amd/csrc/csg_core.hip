@@ -40,6 +40,7 @@ struct csg_ctx {
     int *depth_part = nullptr;
     int *runs = nullptr;
     signed char *lcp = nullptr;
+    signed char *lmin1 = nullptr, *lmin2 = nullptr;  // lcp minima per 64 / 4096 sorted bodies
     int *cnt = nullptr, *nbase = nullptr, *scan_part = nullptr;
     DD *pref = nullptr, *dd_part = nullptr;
     DD *tpref = nullptr;   // build: each thread's exclusive prefix inside its tile
@@ -99,7 +100,6 @@ struct csg_ctx {
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
     int nodes64_valid = 0;  // the FP64 node records belong to the current tree
     double4 *xyzm = nullptr;  // build scratch: (x, y, z, m) interleaved, body order
-    int *owner = nullptr;     // build scratch: body that opens each internal node id (-1: leaf)
     double4 *geom = nullptr;  // neighbour queries: cell centre and half size per node
     size_t geom_cap = 0;
     int geom_valid = 0;       // geom describes the current tree
@@ -174,9 +174,9 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->bbox_part); dfree(c->key); dfree(c->key2); dfree(c->ext); dfree(c->ext_tmp);
     dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
     dfree(c->depth_part);
-    dfree(c->runs); dfree(c->lcp); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
+    dfree(c->runs); dfree(c->lcp); dfree(c->lmin1); dfree(c->lmin2); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->tpref); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
-    dfree(c->wcost); dfree(c->worder); dfree(c->xyzm); dfree(c->owner);
+    dfree(c->wcost); dfree(c->worder); dfree(c->xyzm);
     c->wcost_n = 0;
     c->cap = 0;
     c->node_cap = 0;
@@ -195,6 +195,13 @@ static void clear_words(T *p, size_t n, hipStream_t st) {
     k_clear<T><<<(unsigned)(b < 4096 ? b : 4096), CSG_TPB, 0, st>>>(p, n);
 }
 
+// Chunk minima of the common-prefix depths: k_lcp writes 4 per block of
+// CSG_TPB bodies (one per wavefront of 64), k_iscan_top one per 64 of those;
+// padded so 16-byte reads past the last entry stay inside.
+static constexpr size_t lmin1_count(size_t cap) { return 4 * ((cap + 1 + CSG_TPB - 1) / CSG_TPB); }
+static constexpr size_t lmin1_bytes(size_t cap) { return (lmin1_count(cap) + 63) / 64 * 64 + 16; }
+static constexpr size_t lmin2_bytes(size_t cap) { return (lmin1_count(cap) + 63) / 64 + 32; }
+
 // Device buffers for `cap` bodies; about 900 bytes per body (the node records,
 // 8 per body at 64 B, dominate; 4096 extra records cover deep chains of
 // near-coincident bodies in small inputs).
@@ -210,11 +217,12 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->idx, C) && dalloc(c->idx2, C) && dalloc(c->rs_hist, (size_t)256 * nb) &&
               dalloc(c->rs_tot, 256) && dalloc(c->xs, C) && dalloc(c->ys, C) && dalloc(c->zs, C) &&
               dalloc(c->ms, C) && dalloc(c->depth_part, (size_t)blocks_for((long long)C1, CSG_TPB)) && dalloc(c->runs, C) &&
-              dalloc(c->lcp, C1) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
+              dalloc(c->lcp, C1 + 64) && dalloc(c->lmin1, lmin1_bytes(C)) &&
+              dalloc(c->lmin2, lmin2_bytes(C)) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
               dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->tpref, (size_t)4 * CSG_TPB * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
               dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, 2 * (size_t)(C / 64 + 1) + 4096) &&
-              dalloc(c->xyzm, C) && dalloc(c->owner, 8 * C + 4096) && dalloc(c->rs_ctl, 1) &&
+              dalloc(c->xyzm, C) && dalloc(c->rs_ctl, 1) &&
               dalloc(c->rs_status, (size_t)256 * blocks_for((long long)C, RS_TILE));
     if (!ok) {
         (void)hipGetLastError();
@@ -650,10 +658,12 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
     }
     const int n1 = n + 1;
     const int nlb = blocks_for(n1, CSG_TPB);
-    k_lcp<<<nlb, CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt, c->depth_part);
+    k_lcp<<<nlb, CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt, c->depth_part,
+                                   c->lmin1);
     const int nt = blocks_for(n1, CSG_TILE);
     k_iscan_reduce<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->bp, c->scan_part);
-    k_iscan_top<<<1, CSG_TPB, 0, st>>>(c->scan_part, nt, c->depth_part, nlb, c->bp);
+    k_iscan_top<<<1, CSG_TPB, 0, st>>>(c->scan_part, nt, c->depth_part, nlb, c->bp, c->lmin1,
+                                       4 * nlb, c->lmin2);
     k_iscan_down<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->scan_part, c->bp, c->nbase);
     k_gather_dd<<<nt, CSG_TPB, 0, st>>>(c->idx, c->xyzm, c->bp, c->xs, c->ys, c->zs, c->ms,
                                         c->dd_part, c->tpref);
@@ -661,11 +671,8 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
                                        c->pref);
     const double inv_th2 = c->theta_hint > 0.0 ? 1.0 / (c->theta_hint * c->theta_hint) : HUGE_VAL;
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
-        c->key, c->ext, c->idx, c->lcp, c->nbase, c->pref, c->cap + 1, c->xs, c->ys, c->zs, c->ms,
-        c->bp, with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->owner);
-    k_emit_internal<<<at_least1(blocks_for(2 * (long long)n + 1, CSG_TPB)), CSG_TPB, 0, st>>>(
-        c->key, c->ext, c->lcp, c->nbase, c->pref, c->cap + 1, c->bp,
-        with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->owner);
+        c->key, c->ext, c->idx, c->lcp, c->lmin1, c->lmin2, c->nbase, c->pref, c->cap + 1, c->xs,
+        c->ys, c->zs, c->ms, c->bp, with64 ? c->nodes : nullptr, c->nodes32, inv_th2);
     CSG_LAUNCHED();
     c->tree_valid = 1;
     c->geom_valid = 0;
@@ -1119,8 +1126,9 @@ int csg_debug_fail_build(csg_ctx *c, int nth) {
 #define CSG_NOALLOC_MAX_BODIES 16384
 enum {
     NA_BBOX, NA_KEY, NA_KEY2, NA_EXT, NA_EXT_TMP, NA_IDX, NA_IDX2, NA_XS, NA_YS, NA_ZS, NA_MS,
-    NA_DEPTH, NA_RUNS, NA_LCP, NA_CNT, NA_NBASE, NA_SCAN, NA_PREF, NA_DDPART, NA_TPREF,
-    NA_NODES, NA_NODES32, NA_ICOUNT, NA_WCOST, NA_WORDER, NA_XYZM, NA_OWNER, NA_RSCTL,
+    NA_DEPTH, NA_RUNS, NA_LCP, NA_LMIN1, NA_LMIN2, NA_CNT, NA_NBASE, NA_SCAN, NA_PREF,
+    NA_DDPART, NA_TPREF,
+    NA_NODES, NA_NODES32, NA_ICOUNT, NA_WCOST, NA_WORDER, NA_XYZM, NA_RSCTL,
     NA_RSSTATUS, NA_BP, NA_ITOTAL, NA_REGIONS
 };
 struct NoallocLayout {
@@ -1139,12 +1147,11 @@ constexpr NoallocLayout noalloc_layout(int cap) {
         C * sizeof(uint64_t), C * sizeof(uint64_t), 3 * C * sizeof(uint64_t),
         3 * C * sizeof(uint64_t), C * sizeof(int), C * sizeof(int),
         C * sizeof(double), C * sizeof(double), C * sizeof(double), C * sizeof(double),
-        ((C1 + CSG_TPB - 1) / CSG_TPB) * sizeof(int), C * sizeof(int), C1 * sizeof(signed char),
-        C1 * sizeof(int), C1 * sizeof(int), nb * sizeof(int), 4 * C1 * sizeof(DD),
+        ((C1 + CSG_TPB - 1) / CSG_TPB) * sizeof(int), C * sizeof(int), C1 + 64,
+        lmin1_bytes(C), lmin2_bytes(C), C1 * sizeof(int), C1 * sizeof(int), nb * sizeof(int), 4 * C1 * sizeof(DD),
         4 * nb * sizeof(DD), 4 * (size_t)CSG_TPB * nb * sizeof(DD),
         nodes * sizeof(WNode), nodes * sizeof(WNode32), C * sizeof(int), waves * sizeof(int),
-        (2 * waves + 4096) * sizeof(int), C * sizeof(double4), nodes * sizeof(int),
-        sizeof(RsCtl), 256 * ((C + RS_TILE - 1) / RS_TILE) * sizeof(uint64_t),
+        (2 * waves + 4096) * sizeof(int), C * sizeof(double4), sizeof(RsCtl), 256 * ((C + RS_TILE - 1) / RS_TILE) * sizeof(uint64_t),
         sizeof(BuildParams), sizeof(unsigned long long)};
     size_t o = 0;
     for (int r = 0; r < NA_REGIONS; ++r) {
@@ -1227,12 +1234,14 @@ int cosmosim_ext_forces_noalloc(const double *x, const double *y, const double *
     c.xs = (double *)at(NA_XS); c.ys = (double *)at(NA_YS); c.zs = (double *)at(NA_ZS);
     c.ms = (double *)at(NA_MS);
     c.depth_part = (int *)at(NA_DEPTH); c.runs = (int *)at(NA_RUNS);
-    c.lcp = (signed char *)at(NA_LCP); c.cnt = (int *)at(NA_CNT); c.nbase = (int *)at(NA_NBASE);
+    c.lcp = (signed char *)at(NA_LCP);
+    c.lmin1 = (signed char *)at(NA_LMIN1); c.lmin2 = (signed char *)at(NA_LMIN2);
+    c.cnt = (int *)at(NA_CNT); c.nbase = (int *)at(NA_NBASE);
     c.scan_part = (int *)at(NA_SCAN); c.pref = (DD *)at(NA_PREF); c.dd_part = (DD *)at(NA_DDPART);
     c.tpref = (DD *)at(NA_TPREF); c.nodes = (WNode *)at(NA_NODES);
     c.nodes32 = (WNode32 *)at(NA_NODES32); c.icount = (int *)at(NA_ICOUNT);
     c.wcost = (int *)at(NA_WCOST); c.worder = (int *)at(NA_WORDER);
-    c.xyzm = (double4 *)at(NA_XYZM); c.owner = (int *)at(NA_OWNER);
+    c.xyzm = (double4 *)at(NA_XYZM);
     c.rs_ctl = (RsCtl *)at(NA_RSCTL); c.rs_status = (uint64_t *)at(NA_RSSTATUS);
     c.bp = (BuildParams *)at(NA_BP); c.itotal = (unsigned long long *)at(NA_ITOTAL);
     // the array keeps the last call's contents: start the words the build reads

@@ -893,20 +893,24 @@ __global__ __launch_bounds__(CSG_TPB) void k_runs_fix(
 // L[b] = levels shared by sorted bodies b and b+1 (-1 past the end).  Body b
 // opens the internal nodes of depths (min(L[b-1],63), min(L[b],63)] and starts
 // a leaf unless it shares all 64 levels with b-1.
+// Also M1[c] = min of L over the 64 bodies of chunk c (127 past the live
+// bodies): the summary the node emission searches for a cell's last member.
 __global__ __launch_bounds__(CSG_TPB) void k_lcp(const uint64_t *__restrict__ key,
                                                  const uint64_t *__restrict__ ext,
                                                  const BuildParams *__restrict__ bp,
                                                  signed char *__restrict__ L,
                                                  int *__restrict__ cnt,
-                                                 int *__restrict__ depth_part) {
+                                                 int *__restrict__ depth_part,
+                                                 signed char *__restrict__ M1) {
     __shared__ int wmax[4];
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
-    int leafdepth = 0;
+    int leafdepth = 0, lmin = 127;
     if (b < nl) {
         const int lc = (b + 1 < nl) ? lcp_sorted(key, ext, b, b + 1) : -1;
         const int lp = (b > 0) ? lcp_sorted(key, ext, b - 1, b) : -1;
         L[b] = (signed char)lc;
+        lmin = lc;
         const int opened = max(0, min(lc, 63) - min(lp, 63));
         cnt[b] = opened + (lp < 64 ? 1 : 0);
         leafdepth = min(max(lp, lc) + 1, 64);
@@ -914,7 +918,11 @@ __global__ __launch_bounds__(CSG_TPB) void k_lcp(const uint64_t *__restrict__ ke
         cnt[b] = 0;
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) leafdepth = max(leafdepth, __shfl_xor(leafdepth, o, 64));
+    for (int o = 32; o > 0; o >>= 1) {
+        leafdepth = max(leafdepth, __shfl_xor(leafdepth, o, 64));
+        lmin = min(lmin, __shfl_xor(lmin, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) M1[b >> 6] = (signed char)lmin;
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = leafdepth;
     __syncthreads();
     if (threadIdx.x == 0)
@@ -960,10 +968,31 @@ __global__ __launch_bounds__(CSG_TPB) void k_iscan_reduce(const int *__restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
+// Also M2[s] = min of M1 over the 64 chunks of superchunk s (4096 bodies), for
+// the nm1 chunk minima k_lcp wrote.
 __global__ __launch_bounds__(CSG_TPB) void k_iscan_top(int *__restrict__ part, int nb,
                                                        const int *__restrict__ depth_part,
-                                                       int ndp, BuildParams *bp) {
+                                                       int ndp, BuildParams *bp,
+                                                       const signed char *__restrict__ M1,
+                                                       int nm1, signed char *__restrict__ M2) {
     __shared__ int lds4[4];
+    for (int sc = threadIdx.x; sc < (nm1 + 63) / 64; sc += CSG_TPB) {
+        int m = 127;
+        const int c0 = sc * 64, c1 = min(c0 + 64, nm1);
+        if (c1 - c0 == 64) {
+            const int4 *q = (const int4 *)(M1 + c0);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int4 v = q[w];
+                const int vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 16; ++j) m = min(m, (int)(signed char)(vv[j >> 2] >> (8 * (j & 3))));
+            }
+        } else {
+            for (int c = c0; c < c1; ++c) m = min(m, (int)M1[c]);
+        }
+        M2[sc] = (signed char)m;
+    }
     int carry = 0;
     for (int c0 = 0; c0 < nb; c0 += CSG_TPB) {
         const int i = c0 + threadIdx.x;
@@ -1286,19 +1315,59 @@ __device__ __forceinline__ double dd_range(const DD *__restrict__ pref, int cap1
 }
 
 // ================================================================== emit
+// First i in [from, to) with a[i] < d, else to.  Reads the whole 16-byte words
+// that overlap [from, to): the arrays are 16-byte aligned and padded to 16 bytes.
+__device__ __forceinline__ int first_below(const signed char *__restrict__ a, int from, int to,
+                                           int d) {
+    for (int w = from & ~15; w < to; w += 16) {
+        const int4 q = *(const int4 *)(a + w);
+        const int vv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int i = w + j;
+            const int v = (signed char)(vv[j >> 2] >> (8 * (j & 3)));
+            if (i >= from && i < to && v < d) return i;
+        }
+    }
+    return to;
+}
+
+// Last member of the cell of depth d whose first member is sorted body b: the
+// first k >= b whose common prefix with k + 1 is shallower than d (L[k] < d;
+// L[nl - 1] = -1 ends every search).  Three levels: the rest of b's chunk of
+// 64 bodies in L, the rest of its superchunk of 4096 in the chunk minima M1,
+// then the superchunk minima M2, descending again to the chunk and the body.
+__device__ int cell_last(const signed char *__restrict__ L, const signed char *__restrict__ M1,
+                         const signed char *__restrict__ M2, int b, int d, int nl) {
+    const int cend = min((b | 63) + 1, nl);
+    const int k = first_below(L, b, cend, d);
+    if (k < cend) return k;
+    const int nch = (nl + 63) >> 6;
+    const int sc = b >> 12;
+    const int send = min((sc + 1) << 6, nch);
+    int c = first_below(M1, (b >> 6) + 1, send, d);
+    if (c == send) {
+        const int s2 = first_below(M2, sc + 1, (nch + 63) >> 6, d);
+        c = first_below(M1, s2 << 6, min((s2 + 1) << 6, nch), d);
+    }
+    return first_below(L, c << 6, min((c + 1) << 6, nl), d);
+}
+
 // Node records.  The nodes a sorted body b opens are depths (min(L[b-1],63),
 // min(L[b],63)] (internal cells whose first member is b) plus its leaf, with
 // consecutive pre-order ids from nbase[b].  k_emit writes the leaves and, for
-// every internal id, the body that opens it (owner); k_emit_internal then gives
-// each internal node its own thread, whose one galloping search finds the cell's
-// last member (a body that opens many nested cells, like body 0 with all the
-// ancestors of the first leaf, no longer searches for them one after another).
+// k_emit writes both: each body its leaf, and the internal nodes that the 64
+// bodies of a wavefront open are dealt out over the wave's lanes (lane j takes
+// the wave's j-th one), so a body that opens many nested cells, like body 0 with
+// the ancestors of the first leaf, does not walk them one after another.  Each
+// internal node finds its last member with cell_last.
 __device__ __forceinline__ void emit_internal(
-    const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext, const int *__restrict__ nbase,
+    const signed char *__restrict__ L, const signed char *__restrict__ M1,
+    const signed char *__restrict__ M2, const int *__restrict__ nbase,
     const DD *__restrict__ pref, int cap1, const BuildParams *__restrict__ bp, int nl, int b,
     int d, int id, WNode *__restrict__ nodes, WNode32 *__restrict__ n32, double inv_th2) {
     const double H = bp->root[3];
-    const int e = range_end(key, ext, b, d, nl);
+    const int e = (d == 0) ? nl - 1 : cell_last(L, M1, M2, b, d, nl);
     WNode w;
     const double M = dd_range(pref, cap1, 0, b, e);
     w.cx = dd_range(pref, cap1, 1, b, e) / M;
@@ -1314,11 +1383,13 @@ __device__ __forceinline__ void emit_internal(
 
 __global__ __launch_bounds__(CSG_TPB) void k_emit(
     const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext,
-    const int *__restrict__ idx, const signed char *__restrict__ L, const int *__restrict__ nbase,
-    const DD *__restrict__ pref, int cap1, const double *__restrict__ xs,
-    const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ms,
-    const BuildParams *__restrict__ bp, WNode *__restrict__ nodes, WNode32 *__restrict__ n32,
-    double inv_th2, int *__restrict__ owner) {
+    const int *__restrict__ idx, const signed char *__restrict__ L,
+    const signed char *__restrict__ M1, const signed char *__restrict__ M2,
+    const int *__restrict__ nbase, const DD *__restrict__ pref, int cap1,
+    const double *__restrict__ xs, const double *__restrict__ ys, const double *__restrict__ zs,
+    const double *__restrict__ ms, const BuildParams *__restrict__ bp, WNode *__restrict__ nodes,
+    WNode32 *__restrict__ n32, double inv_th2) {
+    __shared__ int s_off[CSG_TPB], s_lo[CSG_TPB], s_id[CSG_TPB];
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
     if (bp->overflow) return;
@@ -1359,14 +1430,41 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
         }
         return;
     }
-    if (b >= nl) return;
-    const int lp = (b > 0) ? L[b - 1] : -1;
-    const int lc = L[b];
-    const int lo = min(lp, 63) + 1, hi = min(lc, 63);
-    int id = nbase[b];
-    for (int d = lo; d <= hi; ++d, ++id) owner[id] = b;   // internal: k_emit_internal
-    if (lp < 64) {
-        owner[id] = -1;
+    const bool live = b < nl;
+    const int lane = threadIdx.x & 63, wbase = threadIdx.x & ~63;
+    int lp = -1, lc = -1, lo = 0, id0 = 0, opened = 0;
+    if (live) {
+        lp = (b > 0) ? L[b - 1] : -1;
+        lc = L[b];
+        lo = min(lp, 63) + 1;
+        opened = max(0, min(lc, 63) - lo + 1);   // internal depths lo .. min(lc, 63)
+        id0 = nbase[b];
+    }
+    // the wave's internal nodes, in body order: exclusive offsets of the counts
+    int incl = opened;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    const int total = __shfl(incl, 63, 64);
+    s_off[threadIdx.x] = incl - opened;
+    s_lo[threadIdx.x] = lo;
+    s_id[threadIdx.x] = id0;
+    __syncthreads();
+    for (int j = lane; j < total; j += 64) {
+        // the last lane whose offset is <= j opens node j (it opens at least one)
+        int k0 = 0, k1 = 63;
+        while (k0 < k1) {
+            const int mid = (k0 + k1 + 1) >> 1;
+            if (s_off[wbase + mid] <= j) k0 = mid; else k1 = mid - 1;
+        }
+        const int r = j - s_off[wbase + k0];
+        emit_internal(L, M1, M2, nbase, pref, cap1, bp, nl, blockIdx.x * CSG_TPB + wbase + k0,
+                      s_lo[wbase + k0] + r, s_id[wbase + k0] + r, nodes, n32, inv_th2);
+    }
+    if (live && lp < 64) {
+        const int id = id0 + opened;
         WNode w;
         const int i = idx[b];
         int e = b;
@@ -1389,23 +1487,6 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
         w.skip = id + 1;
         w.body = i; w.depth = depth; w.first = b; w.last = e; w.pad = 0;
         store_nodes(nodes, n32, id, w, inv_th2);
-    }
-}
-
-// One thread per node id (grid-stride): internal nodes found through owner[].
-__global__ __launch_bounds__(CSG_TPB) void k_emit_internal(
-    const uint64_t *__restrict__ key, const uint64_t *__restrict__ ext,
-    const signed char *__restrict__ L, const int *__restrict__ nbase, const DD *__restrict__ pref,
-    int cap1, const BuildParams *__restrict__ bp, WNode *__restrict__ nodes,
-    WNode32 *__restrict__ n32, double inv_th2, const int *__restrict__ owner) {
-    const int nl = bp->n_live, nn = bp->n_nodes;
-    if (bp->overflow || nl == 0) return;
-    for (int id = blockIdx.x * CSG_TPB + threadIdx.x; id < nn; id += gridDim.x * CSG_TPB) {
-        const int b = owner[id];
-        if (b < 0) continue;
-        const int lp = (b > 0) ? L[b - 1] : -1;
-        const int d = min(lp, 63) + 1 + (id - nbase[b]);
-        emit_internal(key, ext, nbase, pref, cap1, bp, nl, b, d, id, nodes, n32, inv_th2);
     }
 }
 
