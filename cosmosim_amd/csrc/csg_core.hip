@@ -38,7 +38,6 @@ struct csg_ctx {
     unsigned rs_epoch = 0;             // epoch of the last sort pass
     double *xs = nullptr, *ys = nullptr, *zs = nullptr, *ms = nullptr;  // sorted copies
     int *depth_part = nullptr;
-    int *runs = nullptr;
     signed char *lcp = nullptr;
     signed char *lmin1 = nullptr, *lmin2 = nullptr;  // lcp minima per 64 / 4096 sorted bodies
     int *cnt = nullptr, *nbase = nullptr, *scan_part = nullptr;
@@ -174,7 +173,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->bbox_part); dfree(c->key); dfree(c->key2); dfree(c->ext); dfree(c->ext_tmp);
     dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
     dfree(c->depth_part);
-    dfree(c->runs); dfree(c->lcp); dfree(c->lmin1); dfree(c->lmin2); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
+    dfree(c->lcp); dfree(c->lmin1); dfree(c->lmin2); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->tpref); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
     dfree(c->wcost); dfree(c->worder); dfree(c->xyzm);
     c->wcost_n = 0;
@@ -216,7 +215,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->key2, C) && dalloc(c->ext, 3 * C) && dalloc(c->ext_tmp, 3 * C) &&
               dalloc(c->idx, C) && dalloc(c->idx2, C) && dalloc(c->rs_hist, (size_t)256 * nb) &&
               dalloc(c->rs_tot, 256) && dalloc(c->xs, C) && dalloc(c->ys, C) && dalloc(c->zs, C) &&
-              dalloc(c->ms, C) && dalloc(c->depth_part, (size_t)blocks_for((long long)C1, CSG_TPB)) && dalloc(c->runs, C) &&
+              dalloc(c->ms, C) && dalloc(c->depth_part, (size_t)blocks_for((long long)C1, CSG_TPB)) &&
               dalloc(c->lcp, C1 + 64) && dalloc(c->lmin1, lmin1_bytes(C)) &&
               dalloc(c->lmin2, lmin2_bytes(C)) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
@@ -628,9 +627,8 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
     CSG_LAUNCHED();
     if (n > 0) {
         const int nt = blocks_for(n, RS_TILE);
-        k_keys<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bp, c->key,
-                                                          c->xyzm);
-        k_rs_count<<<nt, CSG_TPB, 0, st>>>(c->key, n, c->rs_ctl);
+        k_keys<<<nt, RS_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bp, c->key, c->xyzm,
+                                      c->rs_ctl);
         if (c->rs_epoch > 0x7fffff00u) {   // epochs are 31 bits: start the words over
             clear_words(c->rs_status, (size_t)256 * blocks_for(c->cap, RS_TILE), st);
             c->rs_epoch = 0;
@@ -651,9 +649,8 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
         }
         CSG_LAUNCHED();
         // an even number of passes: the sorted pairs are back in key / idx
-        k_runs_mark<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->key, c->bp, c->runs);
-        k_runs_fix<<<256, CSG_TPB, 0, st>>>(c->key, c->idx, c->x, c->y, c->z, c->bp, c->runs,
-                                            c->ext, c->idx2, c->ext_tmp);
+        k_runs<<<256, CSG_TPB, 0, st>>>(c->key, c->idx, c->x, c->y, c->z, c->bp, c->ext,
+                                        c->idx2, c->ext_tmp);
         CSG_LAUNCHED();
     }
     const int n1 = n + 1;
@@ -1126,7 +1123,7 @@ int csg_debug_fail_build(csg_ctx *c, int nth) {
 #define CSG_NOALLOC_MAX_BODIES 16384
 enum {
     NA_BBOX, NA_KEY, NA_KEY2, NA_EXT, NA_EXT_TMP, NA_IDX, NA_IDX2, NA_XS, NA_YS, NA_ZS, NA_MS,
-    NA_DEPTH, NA_RUNS, NA_LCP, NA_LMIN1, NA_LMIN2, NA_CNT, NA_NBASE, NA_SCAN, NA_PREF,
+    NA_DEPTH, NA_LCP, NA_LMIN1, NA_LMIN2, NA_CNT, NA_NBASE, NA_SCAN, NA_PREF,
     NA_DDPART, NA_TPREF,
     NA_NODES, NA_NODES32, NA_ICOUNT, NA_WCOST, NA_WORDER, NA_XYZM, NA_RSCTL,
     NA_RSSTATUS, NA_BP, NA_ITOTAL, NA_REGIONS
@@ -1147,7 +1144,7 @@ constexpr NoallocLayout noalloc_layout(int cap) {
         C * sizeof(uint64_t), C * sizeof(uint64_t), 3 * C * sizeof(uint64_t),
         3 * C * sizeof(uint64_t), C * sizeof(int), C * sizeof(int),
         C * sizeof(double), C * sizeof(double), C * sizeof(double), C * sizeof(double),
-        ((C1 + CSG_TPB - 1) / CSG_TPB) * sizeof(int), C * sizeof(int), C1 + 64,
+        ((C1 + CSG_TPB - 1) / CSG_TPB) * sizeof(int), C1 + 64,
         lmin1_bytes(C), lmin2_bytes(C), C1 * sizeof(int), C1 * sizeof(int), nb * sizeof(int), 4 * C1 * sizeof(DD),
         4 * nb * sizeof(DD), 4 * (size_t)CSG_TPB * nb * sizeof(DD),
         nodes * sizeof(WNode), nodes * sizeof(WNode32), C * sizeof(int), waves * sizeof(int),
@@ -1233,7 +1230,7 @@ int cosmosim_ext_forces_noalloc(const double *x, const double *y, const double *
     c.idx = (int *)at(NA_IDX); c.idx2 = (int *)at(NA_IDX2);
     c.xs = (double *)at(NA_XS); c.ys = (double *)at(NA_YS); c.zs = (double *)at(NA_ZS);
     c.ms = (double *)at(NA_MS);
-    c.depth_part = (int *)at(NA_DEPTH); c.runs = (int *)at(NA_RUNS);
+    c.depth_part = (int *)at(NA_DEPTH);
     c.lcp = (signed char *)at(NA_LCP);
     c.lmin1 = (signed char *)at(NA_LMIN1); c.lmin2 = (signed char *)at(NA_LMIN2);
     c.cnt = (int *)at(NA_CNT); c.nbase = (int *)at(NA_NBASE);

@@ -339,7 +339,6 @@ __device__ __forceinline__ int block_excl_scan(int v, int *lds4, int &total) {
 #define RS_ITEMS 4                   // keys per thread in the sort passes
 #define RS_WAVES (RS_TPB / 64)
 #define RS_TILE (RS_TPB * RS_ITEMS)  // keys per tile
-#define RS_COUNT_ITEMS (RS_TILE / CSG_TPB)
 struct RsCtl {
     int ticket[CSG_SORT_PASSES];
     int hist[CSG_SORT_PASSES][256];
@@ -444,53 +443,44 @@ __global__ __launch_bounds__(CSG_TPB) void k_bbox_final(const double *__restrict
 // =================================================================== keys
 // Dead bodies get the all-ones key and sort behind every live body.
 // Also writes (x, y, z, m) interleaved, one 32-byte line per body, so the sorted
-// gather after the sort reads one sector per body instead of four.
-__global__ __launch_bounds__(CSG_TPB) void k_keys(const double *__restrict__ x,
-                                                  const double *__restrict__ y,
-                                                  const double *__restrict__ z,
-                                                  const double *__restrict__ m, int n,
-                                                  const BuildParams *__restrict__ bp,
-                                                  uint64_t *__restrict__ key,
-                                                  double4 *__restrict__ xyzm) {
-    const int i = blockIdx.x * CSG_TPB + threadIdx.x;
-    if (i >= n) return;
-    uint64_t k = ~0ull;
-    const double px = x[i], py = y[i], pz = z[i], pm = m[i];
-    if (live_mass(pm)) {
-        double cx = bp->root[0], cy = bp->root[1], cz = bp->root[2], h = bp->root[3];
-        k = path_word(px, py, pz, cx, cy, cz, h, CSG_KEY_LEVELS);
-    }
-    key[i] = k;
-    xyzm[i] = make_double4(px, py, pz, pm);
-}
-
-// Count of every digit value of every sort pass (the sort's global digit
-// offsets) from one read of the keys: RS_TILE keys per block, per-block counts
-// in LDS, then one global add per nonzero count.
-__global__ __launch_bounds__(CSG_TPB) void k_rs_count(const uint64_t *__restrict__ key, int n,
-                                                      RsCtl *__restrict__ ctl) {
+// gather after the sort reads one sector per body instead of four, and counts
+// the digit values of every sort pass (all passes' offsets from this one pass
+// over the bodies): tiles of RS_TILE bodies, per-tile counts in LDS added to the
+// sort's global counts.
+__global__ __launch_bounds__(RS_TPB) void k_keys(const double *__restrict__ x,
+                                                 const double *__restrict__ y,
+                                                 const double *__restrict__ z,
+                                                 const double *__restrict__ m, int n,
+                                                 const BuildParams *__restrict__ bp,
+                                                 uint64_t *__restrict__ key,
+                                                 double4 *__restrict__ xyzm,
+                                                 RsCtl *__restrict__ ctl) {
     __shared__ int h[CSG_SORT_PASSES][256];
-    for (int k = threadIdx.x; k < CSG_SORT_PASSES * 256; k += CSG_TPB) (&h[0][0])[k] = 0;
-    const int base = blockIdx.x * RS_TILE + threadIdx.x;
-    uint64_t kk[RS_COUNT_ITEMS];
+    for (int k = threadIdx.x; k < CSG_SORT_PASSES * 256; k += RS_TPB) (&h[0][0])[k] = 0;
+    __syncthreads();
+    const double r0 = bp->root[0], r1 = bp->root[1], r2 = bp->root[2], r3 = bp->root[3];
 #pragma unroll
-    for (int r = 0; r < RS_COUNT_ITEMS; ++r) {
-        const int i = base + r * CSG_TPB;
-        kk[r] = i < n ? key[i] : 0ull;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int i = blockIdx.x * RS_TILE + r * RS_TPB + threadIdx.x;
+        if (i >= n) break;
+        uint64_t k = ~0ull;
+        const double px = x[i], py = y[i], pz = z[i], pm = m[i];
+        if (live_mass(pm)) {
+            double cx = r0, cy = r1, cz = r2, hh = r3;
+            k = path_word(px, py, pz, cx, cy, cz, hh, CSG_KEY_LEVELS);
+        }
+        key[i] = k;
+        xyzm[i] = make_double4(px, py, pz, pm);
+#pragma unroll
+        for (int p = 0; p < CSG_SORT_PASSES; ++p) atomicAdd(&h[p][(int)((k >> (8 * p)) & 255)], 1);
     }
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RS_COUNT_ITEMS; ++r) {
-        if (base + r * CSG_TPB >= n) break;
-#pragma unroll
-        for (int p = 0; p < CSG_SORT_PASSES; ++p) atomicAdd(&h[p][(int)((kk[r] >> (8 * p)) & 255)], 1);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < CSG_SORT_PASSES * 256; k += CSG_TPB) {
+    for (int k = threadIdx.x; k < CSG_SORT_PASSES * 256; k += RS_TPB) {
         const int v = (&h[0][0])[k];
         if (v) atomicAdd(&(&ctl->hist[0][0])[k], v);
     }
 }
+
 
 // ============================================================ radix sort
 // Tile of CSG_TILE consecutive elements per block; wave w owns the sub-tile
@@ -814,79 +804,88 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_sweep(
 }
 
 // ======================================================= collision runs
-__global__ __launch_bounds__(CSG_TPB) void k_runs_mark(const uint64_t *__restrict__ key,
-                                                       BuildParams *bp, int *__restrict__ runs) {
-    const int b = blockIdx.x * CSG_TPB + threadIdx.x;
-    const int nl = bp->n_live;
-    if (b + 1 >= nl) return;
-    const uint64_t k = key[b];
-    if (k == key[b + 1] && (b == 0 || key[b - 1] != k)) {
-        const int r = atomicAdd(&bp->n_runs, 1);
-        runs[r] = b;
-    }
-}
-
-// One block per run (grid-stride): levels CSG_KEY_LEVELS..63 for each member,
-// then members are re-ranked by (those levels, body index); ties keep index order.
-__global__ __launch_bounds__(CSG_TPB) void k_runs_fix(
+// Groups of sorted bodies whose keys are equal (all CSG_KEY_LEVELS levels):
+// each member gets its levels CSG_KEY_LEVELS..63, then the members are re-ranked
+// by (those levels, body index); ties keep index order.  Block j scans slice j
+// of the sorted keys in chunks of CSG_TPB for the first members of runs and
+// fixes each run it finds with the whole block (a run may extend past the
+// slice; it belongs to the slice of its first member).
+__global__ __launch_bounds__(CSG_TPB) void k_runs(
     const uint64_t *__restrict__ key, int *__restrict__ idx, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const BuildParams *__restrict__ bp,
-    const int *__restrict__ runs, uint64_t *__restrict__ ext, int *__restrict__ tmp_idx,
-    uint64_t *__restrict__ tmp_ext) {
-    __shared__ int s_end;
-    const int nl = bp->n_live, nr = bp->n_runs;
+    const double *__restrict__ y, const double *__restrict__ z, BuildParams *__restrict__ bp,
+    uint64_t *__restrict__ ext, int *__restrict__ tmp_idx, uint64_t *__restrict__ tmp_ext) {
+    __shared__ int s_end, s_nstart;
+    __shared__ int s_start[CSG_TPB];
+    const int nl = bp->n_live;
     double root[4] = {bp->root[0], bp->root[1], bp->root[2], bp->root[3]};
-    for (int r = blockIdx.x; r < nr; r += gridDim.x) {
-        const int start = runs[r];
-        const uint64_t k0 = key[start];
-        if (threadIdx.x == 0) s_end = nl;
+    const int per = (nl + gridDim.x - 1) / gridDim.x;
+    const int lo = blockIdx.x * per, hi = min(nl, lo + per);
+    int found = 0;
+    for (int c0 = lo; c0 < hi; c0 += CSG_TPB) {
+        if (threadIdx.x == 0) s_nstart = 0;
         __syncthreads();
-        for (int c0 = start + 1;; c0 += CSG_TPB) {
-            const int j = c0 + threadIdx.x;
-            const bool stop = (j >= nl) || (j < nl && key[j] != k0);
-            if (stop) atomicMin(&s_end, j);
-            if (__syncthreads_or(stop)) break;
-        }
-        const int end = min(s_end, nl);
-        for (int p = start + threadIdx.x; p < end; p += CSG_TPB) {
-            const int i = idx[p];
-            path_ext(x[i], y[i], z[i], root, ext + 3 * (size_t)p);
+        const int b = c0 + threadIdx.x;
+        if (b < hi && b + 1 < nl) {
+            const uint64_t k = key[b];
+            if (k == key[b + 1] && (b == 0 || key[b - 1] != k)) s_start[atomicAdd(&s_nstart, 1)] = b;
         }
         __syncthreads();
-        int differ = 0;
-        for (int p = start + 1 + threadIdx.x; p < end; p += CSG_TPB) {
-            const uint64_t *e = ext + 3 * (size_t)p, *f = ext + 3 * (size_t)start;
-            differ |= (e[0] != f[0]) | (e[1] != f[1]) | (e[2] != f[2]);
-        }
-        if (__syncthreads_or(differ)) {
+        const int ns = s_nstart;
+        found += ns;
+        for (int q = 0; q < ns; ++q) {
+            const int start = s_start[q];
+            const uint64_t k0 = key[start];
+            if (threadIdx.x == 0) s_end = nl;
+            __syncthreads();
+            // the run's end: the first body after start with another key
+            for (int e0 = start + 1;; e0 += CSG_TPB) {
+                const int j = e0 + threadIdx.x;
+                const bool stop = (j >= nl) || (j < nl && key[j] != k0);
+                if (stop) atomicMin(&s_end, j);
+                if (__syncthreads_or(stop)) break;
+            }
+            const int end = min(s_end, nl);
             for (int p = start + threadIdx.x; p < end; p += CSG_TPB) {
-                const uint64_t *e = ext + 3 * (size_t)p;
-                const int ip = idx[p];
-                int rank = 0;
-                for (int q = start; q < end; ++q) {
-                    const uint64_t *f = ext + 3 * (size_t)q;
-                    bool less;
-                    if (f[0] != e[0]) less = f[0] < e[0];
-                    else if (f[1] != e[1]) less = f[1] < e[1];
-                    else if (f[2] != e[2]) less = f[2] < e[2];
-                    else less = idx[q] < ip;
-                    rank += less ? 1 : 0;
-                }
-                tmp_idx[start + rank] = ip;
-                tmp_ext[3 * (size_t)(start + rank) + 0] = e[0];
-                tmp_ext[3 * (size_t)(start + rank) + 1] = e[1];
-                tmp_ext[3 * (size_t)(start + rank) + 2] = e[2];
+                const int i = idx[p];
+                path_ext(x[i], y[i], z[i], root, ext + 3 * (size_t)p);
             }
             __syncthreads();
-            for (int p = start + threadIdx.x; p < end; p += CSG_TPB) {
-                idx[p] = tmp_idx[p];
-                ext[3 * (size_t)p + 0] = tmp_ext[3 * (size_t)p + 0];
-                ext[3 * (size_t)p + 1] = tmp_ext[3 * (size_t)p + 1];
-                ext[3 * (size_t)p + 2] = tmp_ext[3 * (size_t)p + 2];
+            int differ = 0;
+            for (int p = start + 1 + threadIdx.x; p < end; p += CSG_TPB) {
+                const uint64_t *e = ext + 3 * (size_t)p, *f = ext + 3 * (size_t)start;
+                differ |= (e[0] != f[0]) | (e[1] != f[1]) | (e[2] != f[2]);
             }
+            if (__syncthreads_or(differ)) {
+                for (int p = start + threadIdx.x; p < end; p += CSG_TPB) {
+                    const uint64_t *e = ext + 3 * (size_t)p;
+                    const int ip = idx[p];
+                    int rank = 0;
+                    for (int t = start; t < end; ++t) {
+                        const uint64_t *f = ext + 3 * (size_t)t;
+                        bool less;
+                        if (f[0] != e[0]) less = f[0] < e[0];
+                        else if (f[1] != e[1]) less = f[1] < e[1];
+                        else if (f[2] != e[2]) less = f[2] < e[2];
+                        else less = idx[t] < ip;
+                        rank += less ? 1 : 0;
+                    }
+                    tmp_idx[start + rank] = ip;
+                    tmp_ext[3 * (size_t)(start + rank) + 0] = e[0];
+                    tmp_ext[3 * (size_t)(start + rank) + 1] = e[1];
+                    tmp_ext[3 * (size_t)(start + rank) + 2] = e[2];
+                }
+                __syncthreads();
+                for (int p = start + threadIdx.x; p < end; p += CSG_TPB) {
+                    idx[p] = tmp_idx[p];
+                    ext[3 * (size_t)p + 0] = tmp_ext[3 * (size_t)p + 0];
+                    ext[3 * (size_t)p + 1] = tmp_ext[3 * (size_t)p + 1];
+                    ext[3 * (size_t)p + 2] = tmp_ext[3 * (size_t)p + 2];
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
+    if (threadIdx.x == 0 && found) atomicAdd(&bp->n_runs, found);
 }
 
 // ============================================================ lcp + count
