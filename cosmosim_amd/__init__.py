@@ -81,6 +81,7 @@ _SIGS = {
     "cosmosim_ext_export_tree": (_I, [_VP, _VP, _I, _VP]),
     "cosmosim_ext_handle_error": (_I, [_VP]),
     "cosmosim_ext_debug_fail_build": (_I, [_VP, _I]),
+    "cosmosim_ext_debug_orders": (_I, [_VP, _VP, _VP, _VP, _I, _VP]),
     "cosmosim_ext_last_error": (_I, []),
     "cosmosim_ext_clear_error": (None, []),
     "cosmosim_ext_error_string": (ctypes.c_char_p, [_I]),
@@ -130,6 +131,7 @@ _SIGS = {
     "csg_set_timing": (_I, [_VP, _I]),
     "csg_synchronize": (_I, [_VP]),
     "csg_debug_fail_build": (_I, [_VP, _I]),
+    "csg_debug_orders": (_I, [_VP, _VP, _VP, _VP, _I, _VP]),
     "csg_walk_taped": (_I, [_VP, _I, _D, _D, _D]),
     "csg_force_vjp": (_I, [_VP, _VP, _VP]),
     "csg_forces_frozen": (_I, [_VP, _VP, _VP]),
@@ -477,6 +479,19 @@ class Simulation(Handle):
     def debug_fail_build(self, nth):
         """Test hook: the nth tree build from now fails as an over-capacity tree."""
         _check(self.L.cosmosim_ext_debug_fail_build(self.h, int(nth)), "debug_fail_build")
+
+    def debug_orders(self):
+        """Test hook: (sorted_idx, wave_idx, keys) of the last tree, n_live entries each:
+        the bodies in key order, the FP32 walk's wavefront order (Peano-Hilbert order of
+        the cells) and the sorted 48-bit octree keys."""
+        si = np.zeros(max(self.n, 1), np.int32)
+        wi = np.zeros(max(self.n, 1), np.int32)
+        ks = np.zeros(max(self.n, 1), np.uint64)
+        nl = ctypes.c_int(0)
+        _check(self.L.cosmosim_ext_debug_orders(self.h, ptr(si), ptr(wi), ptr(ks), self.n,
+                                                ctypes.byref(nl)), "debug_orders")
+        k = nl.value
+        return si[:k].copy(), wi[:k].copy(), ks[:k].copy()
 
     # ------------------------------------------------------------ adjoint
     def force_vjp(self, w):

@@ -686,6 +686,13 @@ COSMOSIM_API int cosmosim_ext_debug_fail_build(SimHandle h, int nth)
     return h ? csg_debug_fail_build(((SimState *)h)->ctx, nth) : COSMOSIM_E_ARG;
 }
 
+COSMOSIM_API int cosmosim_ext_debug_orders(SimHandle h, int *sorted_idx, int *wave_idx,
+                                           unsigned long long *keys, int n, int *n_live)
+{
+    return h ? csg_debug_orders(((SimState *)h)->ctx, sorted_idx, wave_idx, keys, n, n_live)
+             : COSMOSIM_E_ARG;
+}
+
 COSMOSIM_API int cosmosim_ext_handle_error(SimHandle h)
 {
     return h ? ((SimState *)h)->error : COSMOSIM_E_ARG;

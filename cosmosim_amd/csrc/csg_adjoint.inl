@@ -829,6 +829,17 @@ int csg_set_tape(csg_ctx *c, int enabled) {
     return COSMOSIM_OK;
 }
 
+// A recording's taped walk: the tape, and the trajectory's forces as the untaped
+// walk computes them (the FP32 walk without a tape groups bodies into wavefronts
+// in Peano-Hilbert order, the taped one in sorted order; they agree only to FP32
+// rounding, and a recorded run must follow the same forces as the substeps, the
+// checkpoint recompute and the time reversal).
+static int walk_taped_traj(csg_ctx *c, int precision, double G, double softening, double theta) {
+    int s = walk_taped(c, precision, G, softening, theta);
+    if (s || precision != COSMOSIM_PREC_FP32 || !c->hilbert_waves) return s;
+    return launch_walk(c, precision, G, softening, theta);
+}
+
 int csg_walk_taped(csg_ctx *c, int precision, double G, double softening, double theta) {
     CSG_USE(c);
     if (!c->tree_valid) return COSMOSIM_E_STATE;
@@ -903,7 +914,7 @@ int csg_record(csg_ctx *c, int k, double dt, double G, double softening, double 
         c->frozen = new (std::nothrow) FrozenStep[k + 1];
         if (!c->frozen) return COSMOSIM_E_ALLOC;
         c->n_frozen = k + 1;
-        if ((s = launch_build(c)) || (s = walk_taped(c, precision, G, softening, theta)) ||
+        if ((s = launch_build(c)) || (s = walk_taped_traj(c, precision, G, softening, theta)) ||
             (s = frozen_store(c, 0)))
             return s;
     }
@@ -911,7 +922,7 @@ int csg_record(csg_ctx *c, int k, double dt, double G, double softening, double 
         if (step % every == 0 && (s = state_copy(c, c->ckpt + slot * (step / every), 1))) return s;
         if (keep_trees) {
             if ((s = launch_kick_drift(c, dt, 1)) || (s = launch_build(c)) ||
-                (s = walk_taped(c, precision, G, softening, theta)) || (s = launch_kick(c, dt)) ||
+                (s = walk_taped_traj(c, precision, G, softening, theta)) || (s = launch_kick(c, dt)) ||
                 (s = frozen_store(c, step + 1)))
                 return s;
         } else if ((s = csg_substeps(c, 1, dt, G, softening, theta, precision))) {
@@ -968,7 +979,15 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
         int r;
         k_adj_kick<<<g, CSG_TPB, 0, st>>>(c->m, n, dt, 0, c->xb, c->vb, c->ab);
         if ((r = launch_build(c))) return r;
-        if ((r = walk_taped(c, prec, G, eps, th))) return r;
+        // the tape is what this walk is for: its accelerations go to spare arrays and
+        // the state keeps the forward pass's, which the time reversal's un-kick uses
+        // (the taped walk groups bodies into wavefronts in sorted order, the
+        // forward's FP32 walk in Peano-Hilbert order: equal to FP32 rounding only)
+        double *const kx = c->ax, *const ky = c->ay, *const kz = c->az;
+        c->ax = c->gtmp; c->ay = c->gtmp + n; c->az = c->gtmp + 2 * (size_t)n;
+        r = walk_taped(c, prec, G, eps, th);
+        c->ax = kx; c->ay = ky; c->az = kz;
+        if (r) return r;
         if ((r = force_vjp_dev(c, c->ab, c->xb, 1))) return r;
         k_adj_kick<<<g, CSG_TPB, 0, st>>>(c->m, n, dt, 1, c->xb, c->vb, c->ab);
         CSG_LAUNCHED();

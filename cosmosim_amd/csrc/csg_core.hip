@@ -32,6 +32,7 @@ struct csg_ctx {
     double *bbox_part = nullptr;
     uint64_t *key = nullptr, *key2 = nullptr, *ext = nullptr, *ext_tmp = nullptr;
     int *idx = nullptr, *idx2 = nullptr;
+    int *widx = nullptr;   // FP32 wave-mask walk: the sorted bodies in Peano-Hilbert order
     int *rs_hist = nullptr, *rs_tot = nullptr;
     RsCtl *rs_ctl = nullptr;           // body sort: tickets and digit counts
     uint64_t *rs_status = nullptr;     // body sort: per-tile digit counts (look-back)
@@ -45,6 +46,7 @@ struct csg_ctx {
     DD *tpref = nullptr;   // build: each thread's exclusive prefix inside its tile
     WNode *nodes = nullptr;
     WNode32 *nodes32 = nullptr;
+    int *nlast = nullptr;   // build: each node's last sorted member (Peano-Hilbert pass)
     int *icount = nullptr;
     unsigned long long *itotal = nullptr;
     int tree_valid = 0;
@@ -96,6 +98,7 @@ struct csg_ctx {
     // 3.71 grouped; strict 5.42 vs 4.45)
     int walk_group = -1;
     int walk_group32 = 0;   // FP32 walk: 0 = wave-mask traversal, else lanes per group
+    int hilbert_waves = 1;  // untaped FP32 wave-mask walks take wavefronts in Hilbert order
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
     int nodes64_valid = 0;  // the FP64 node records belong to the current tree
     double4 *xyzm = nullptr;  // build scratch: (x, y, z, m) interleaved, body order
@@ -171,7 +174,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->vx); dfree(c->vy); dfree(c->vz);
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->m); dfree(c->aos);
     dfree(c->bbox_part); dfree(c->key); dfree(c->key2); dfree(c->ext); dfree(c->ext_tmp);
-    dfree(c->idx); dfree(c->idx2); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
+    dfree(c->idx); dfree(c->idx2); dfree(c->widx); dfree(c->nlast); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
     dfree(c->depth_part);
     dfree(c->lcp); dfree(c->lmin1); dfree(c->lmin2); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->tpref); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
@@ -213,13 +216,14 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->az, C) && dalloc(c->m, C) && dalloc(c->aos, C) &&
               dalloc(c->bbox_part, (size_t)CSG_BBOX_BLOCKS * 8) && dalloc(c->key, C) &&
               dalloc(c->key2, C) && dalloc(c->ext, 3 * C) && dalloc(c->ext_tmp, 3 * C) &&
-              dalloc(c->idx, C) && dalloc(c->idx2, C) && dalloc(c->rs_hist, (size_t)256 * nb) &&
+              dalloc(c->idx, C) && dalloc(c->idx2, C) && dalloc(c->widx, C) &&
+              dalloc(c->rs_hist, (size_t)256 * nb) &&
               dalloc(c->rs_tot, 256) && dalloc(c->xs, C) && dalloc(c->ys, C) && dalloc(c->zs, C) &&
               dalloc(c->ms, C) && dalloc(c->depth_part, (size_t)blocks_for((long long)C1, CSG_TPB)) &&
               dalloc(c->lcp, C1 + 64) && dalloc(c->lmin1, lmin1_bytes(C)) &&
               dalloc(c->lmin2, lmin2_bytes(C)) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
-              dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->tpref, (size_t)4 * CSG_TPB * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) &&
+              dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->tpref, (size_t)4 * CSG_TPB * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) && dalloc(c->nlast, 8 * C + 4096) &&
               dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, 2 * (size_t)(C / 64 + 1) + 4096) &&
               dalloc(c->xyzm, C) && dalloc(c->rs_ctl, 1) &&
               dalloc(c->rs_status, (size_t)256 * blocks_for((long long)C, RS_TILE));
@@ -232,6 +236,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
     clear_words(c->rs_status, (size_t)256 * blocks_for((long long)C, RS_TILE), c->stream);
     clear_words(c->idx, C, c->stream);
     clear_words(c->idx2, C, c->stream);
+    clear_words(c->widx, C, c->stream);
     c->rs_epoch = 0;
     if (hipGetLastError() != hipSuccess) {
         free_buffers(c);
@@ -355,6 +360,7 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     }
     if (const char *g = getenv("COSMOSIM_XCD_CHUNK")) c->xcd_chunk = atoi(g);
     if (const char *g = getenv("COSMOSIM_WAVE_ORDER")) c->wave_order = atoi(g) != 0;
+    if (const char *g = getenv("COSMOSIM_HILBERT_WAVES")) c->hilbert_waves = atoi(g) != 0;
     if (const char *g = getenv("COSMOSIM_WAVE_SPLIT")) {
         const int v = atoi(g);
         if (v >= 1 && v <= WO_MAX_SPLIT) c->wave_split = v;
@@ -655,8 +661,10 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
     }
     const int n1 = n + 1;
     const int nlb = blocks_for(n1, CSG_TPB);
+    // k_lcp also writes the sorted bodies' Peano-Hilbert keys into key2 (the sort's
+    // spare key buffer)
     k_lcp<<<nlb, CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt, c->depth_part,
-                                   c->lmin1);
+                                   c->lmin1, c->key2);
     const int nt = blocks_for(n1, CSG_TILE);
     k_iscan_reduce<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->bp, c->scan_part);
     k_iscan_top<<<1, CSG_TPB, 0, st>>>(c->scan_part, nt, c->depth_part, nlb, c->bp, c->lmin1,
@@ -664,12 +672,17 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
     k_iscan_down<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->scan_part, c->bp, c->nbase);
     k_gather_dd<<<nt, CSG_TPB, 0, st>>>(c->idx, c->xyzm, c->bp, c->xs, c->ys, c->zs, c->ms,
                                         c->dd_part, c->tpref);
+    // k_dd_down2 also clears cnt for k_emit's Peano-Hilbert offsets
     k_dd_down2<<<nt, CSG_TPB, 0, st>>>(c->xs, c->ys, c->zs, c->ms, c->bp, c->dd_part, c->tpref,
-                                       c->pref);
+                                       c->pref, c->cnt);
     const double inv_th2 = c->theta_hint > 0.0 ? 1.0 / (c->theta_hint * c->theta_hint) : HUGE_VAL;
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
         c->key, c->ext, c->idx, c->lcp, c->lmin1, c->lmin2, c->nbase, c->pref, c->cap + 1, c->xs,
-        c->ys, c->zs, c->ms, c->bp, with64 ? c->nodes : nullptr, c->nodes32, inv_th2);
+        c->ys, c->zs, c->ms, c->bp, with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->nlast);
+    k_hilbert<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(c->lcp, c->nbase, c->nlast,
+                                                                   c->key2, c->bp, c->cnt);
+    k_iscan_reduce<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->bp, c->scan_part);
+    k_hrank<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->scan_part, c->idx, c->bp, c->widx);
     CSG_LAUNCHED();
     c->tree_valid = 1;
     c->geom_valid = 0;
@@ -700,9 +713,9 @@ static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, dou
 }
 
 template <bool LC, bool TP, bool O32, bool TOT, bool SELF>
-static void launch_walk32w(csg_ctx *c, double G, double eps2, double th2, int grid, int nb8,
-                           TapeOut tape, WaveOrder wo, int tpb) {
-    k_walk32w<LC, TP, O32, TOT, SELF><<<grid, tpb, 0, c->stream>>>(c->nodes32, c->bp, c->idx, c->x,
+static void launch_walk32w(csg_ctx *c, const int *sidx, double G, double eps2, double th2,
+                           int grid, int nb8, TapeOut tape, WaveOrder wo, int tpb) {
+    k_walk32w<LC, TP, O32, TOT, SELF><<<grid, tpb, 0, c->stream>>>(c->nodes32, c->bp, sidx, c->x,
         c->y, c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk,
         tape, wo);
 }
@@ -718,9 +731,12 @@ static void launch_walk64w(csg_ctx *c, double G, double eps2, double th2, int gr
 // lane_counts: record per-body interaction counts (c->icount); walks inside the
 // substep loops skip them.  count_total: keep the interaction total (the FP32
 // substep loop keeps it only in its last walk; the statistics report that one).
+// hilbert: an untaped FP32 wave-mask walk takes its wavefronts' bodies in
+// Peano-Hilbert order (c->widx); false keeps sorted order, as the taped walk
+// does (the time-reversed adjoint recomputes its forces that way).
 static int launch_walk(csg_ctx *c, int precision, double G, double softening, double theta,
                        TapeOut tape = TapeOut{nullptr, nullptr, 0, nullptr},
-                       bool lane_counts = true, bool count_total = true) {
+                       bool lane_counts = true, bool count_total = true, bool hilbert = true) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const double eps2 = softening * softening, th2 = theta * theta;
@@ -765,27 +781,28 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         // wave-mask traversal (also the taped FP32 walk: the tape is per wavefront)
         const bool off32 = (size_t)c->node_cap * sizeof(WNode32) < ((size_t)1 << 32);
         const bool taping = tape.rec != nullptr;
+        const int *sidx = (hilbert && !taping && c->hilbert_waves) ? c->widx : c->idx;
         const int v = (lane_counts ? 4 : 0) | (taping ? 2 : 0) | (off32 ? 1 : 0);
         if (v < 2 && !count_total) {
             // softened: the own leaf's offset is exactly 0 and its factor
             // m / eps^3 is finite in FP32 (eps^2 >= 1e-12 keeps it below
             // m * 1e18), so it adds exactly 0 and needs no exclusion
             if (off32 && (float)eps2 >= 1e-12f)
-                launch_walk32w<false, false, true, false, false>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
+                launch_walk32w<false, false, true, false, false>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb);
             else if (off32)
-                launch_walk32w<false, false, true, false, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
+                launch_walk32w<false, false, true, false, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb);
             else
-                launch_walk32w<false, false, false, false, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
+                launch_walk32w<false, false, false, false, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb);
         } else {
             switch (v) {
-            case 0: launch_walk32w<false, false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-            case 1: launch_walk32w<false, false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-            case 2: launch_walk32w<false, true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-            case 3: launch_walk32w<false, true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-            case 4: launch_walk32w<true, false, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-            case 5: launch_walk32w<true, false, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-            case 6: launch_walk32w<true, true, false, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-            case 7: launch_walk32w<true, true, true, true, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 0: launch_walk32w<false, false, false, true, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 1: launch_walk32w<false, false, true, true, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 2: launch_walk32w<false, true, false, true, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 3: launch_walk32w<false, true, true, true, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 4: launch_walk32w<true, false, false, true, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 5: launch_walk32w<true, false, true, true, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 6: launch_walk32w<true, true, false, true, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+            case 7: launch_walk32w<true, true, true, true, true>(c, sidx, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
             }
         }
         CSG_LAUNCHED();
@@ -849,7 +866,7 @@ static int check_overflow(csg_ctx *c) {
     int s = read_params(c);
     if (s) return s;
     collect_timers(c);
-    const int pending = (c->hbp->overflow || c->hbp->sort_stall) ? 1 : 0;
+    const int pending = (c->hbp->overflow || c->hbp->sort_stall || c->hbp->order_bad) ? 1 : 0;
     const bool earlier = c->hbp->fail_count != c->fail_seen;
     c->fail_seen = c->hbp->fail_count + pending;
     if (pending || earlier) {
@@ -857,6 +874,10 @@ static int check_overflow(csg_ctx *c) {
         c->counts_valid = 0;
         if (c->hbp->sort_stall) {
             fprintf(stderr, "cosmosim: body sort stalled waiting for an earlier tile\n");
+            return COSMOSIM_E_HIP;
+        }
+        if (c->hbp->order_bad) {
+            fprintf(stderr, "cosmosim: Peano-Hilbert wave order is not a permutation\n");
             return COSMOSIM_E_HIP;
         }
         return COSMOSIM_E_CAPACITY;
@@ -1110,6 +1131,27 @@ int csg_debug_fail_build(csg_ctx *c, int nth) {
     return COSMOSIM_OK;
 }
 
+int csg_debug_orders(csg_ctx *c, int *sorted_idx, int *wave_idx, unsigned long long *keys, int n,
+                     int *n_live) {
+    CSG_USE(c);
+    if (!n_live || n < 0) return COSMOSIM_E_ARG;
+    if (!c->tree_valid) return COSMOSIM_E_STATE;
+    int s = read_params(c);
+    if (s) return s;
+    const int nl = c->hbp->n_live;
+    *n_live = nl;
+    if (n < nl) return COSMOSIM_E_CAPACITY;
+    if (nl == 0) return COSMOSIM_OK;
+    if (sorted_idx)
+        CSG_CHECK(hipMemcpyAsync(sorted_idx, c->idx, sizeof(int) * nl, hipMemcpyDeviceToHost, c->stream));
+    if (wave_idx)
+        CSG_CHECK(hipMemcpyAsync(wave_idx, c->widx, sizeof(int) * nl, hipMemcpyDeviceToHost, c->stream));
+    if (keys)
+        CSG_CHECK(hipMemcpyAsync(keys, c->key, sizeof(uint64_t) * nl, hipMemcpyDeviceToHost, c->stream));
+    CSG_CHECK(hipStreamSynchronize(c->stream));
+    return COSMOSIM_OK;
+}
+
 }  // extern "C"
 
 // ================================================ allocation-free force call
@@ -1122,10 +1164,11 @@ int csg_debug_fail_build(csg_ctx *c, int nth) {
 // copy until the process ends, and every call on that GPU reuses it.
 #define CSG_NOALLOC_MAX_BODIES 16384
 enum {
-    NA_BBOX, NA_KEY, NA_KEY2, NA_EXT, NA_EXT_TMP, NA_IDX, NA_IDX2, NA_XS, NA_YS, NA_ZS, NA_MS,
+    NA_BBOX, NA_KEY, NA_KEY2, NA_EXT, NA_EXT_TMP, NA_IDX, NA_IDX2, NA_WIDX, NA_XS, NA_YS, NA_ZS,
+    NA_MS,
     NA_DEPTH, NA_LCP, NA_LMIN1, NA_LMIN2, NA_CNT, NA_NBASE, NA_SCAN, NA_PREF,
     NA_DDPART, NA_TPREF,
-    NA_NODES, NA_NODES32, NA_ICOUNT, NA_WCOST, NA_WORDER, NA_XYZM, NA_RSCTL,
+    NA_NODES, NA_NODES32, NA_NLAST, NA_ICOUNT, NA_WCOST, NA_WORDER, NA_XYZM, NA_RSCTL,
     NA_RSSTATUS, NA_BP, NA_ITOTAL, NA_REGIONS
 };
 struct NoallocLayout {
@@ -1142,12 +1185,13 @@ constexpr NoallocLayout noalloc_layout(int cap) {
     const size_t bytes[NA_REGIONS] = {
         (size_t)CSG_BBOX_BLOCKS * 8 * sizeof(double),
         C * sizeof(uint64_t), C * sizeof(uint64_t), 3 * C * sizeof(uint64_t),
-        3 * C * sizeof(uint64_t), C * sizeof(int), C * sizeof(int),
+        3 * C * sizeof(uint64_t), C * sizeof(int), C * sizeof(int), C * sizeof(int),
         C * sizeof(double), C * sizeof(double), C * sizeof(double), C * sizeof(double),
         ((C1 + CSG_TPB - 1) / CSG_TPB) * sizeof(int), C1 + 64,
         lmin1_bytes(C), lmin2_bytes(C), C1 * sizeof(int), C1 * sizeof(int), nb * sizeof(int), 4 * C1 * sizeof(DD),
         4 * nb * sizeof(DD), 4 * (size_t)CSG_TPB * nb * sizeof(DD),
-        nodes * sizeof(WNode), nodes * sizeof(WNode32), C * sizeof(int), waves * sizeof(int),
+        nodes * sizeof(WNode), nodes * sizeof(WNode32), nodes * sizeof(int), C * sizeof(int),
+        waves * sizeof(int),
         (2 * waves + 4096) * sizeof(int), C * sizeof(double4), sizeof(RsCtl), 256 * ((C + RS_TILE - 1) / RS_TILE) * sizeof(uint64_t),
         sizeof(BuildParams), sizeof(unsigned long long)};
     size_t o = 0;
@@ -1189,7 +1233,8 @@ static int noalloc_scratch(int dev, unsigned char **out) {
 
 // status: 0, or the code a build that overflowed (or stalled) would return.
 __global__ void k_noalloc_status(const BuildParams *bp, int *status) {
-    *status = bp->sort_stall ? COSMOSIM_E_HIP : (bp->overflow ? COSMOSIM_E_CAPACITY : 0);
+    *status = (bp->sort_stall || bp->order_bad) ? COSMOSIM_E_HIP
+                                                : (bp->overflow ? COSMOSIM_E_CAPACITY : 0);
 }
 
 extern "C" {
@@ -1227,7 +1272,7 @@ int cosmosim_ext_forces_noalloc(const double *x, const double *y, const double *
     c.bbox_part = (double *)at(NA_BBOX);
     c.key = (uint64_t *)at(NA_KEY); c.key2 = (uint64_t *)at(NA_KEY2);
     c.ext = (uint64_t *)at(NA_EXT); c.ext_tmp = (uint64_t *)at(NA_EXT_TMP);
-    c.idx = (int *)at(NA_IDX); c.idx2 = (int *)at(NA_IDX2);
+    c.idx = (int *)at(NA_IDX); c.idx2 = (int *)at(NA_IDX2); c.widx = (int *)at(NA_WIDX);
     c.xs = (double *)at(NA_XS); c.ys = (double *)at(NA_YS); c.zs = (double *)at(NA_ZS);
     c.ms = (double *)at(NA_MS);
     c.depth_part = (int *)at(NA_DEPTH);
@@ -1236,7 +1281,8 @@ int cosmosim_ext_forces_noalloc(const double *x, const double *y, const double *
     c.cnt = (int *)at(NA_CNT); c.nbase = (int *)at(NA_NBASE);
     c.scan_part = (int *)at(NA_SCAN); c.pref = (DD *)at(NA_PREF); c.dd_part = (DD *)at(NA_DDPART);
     c.tpref = (DD *)at(NA_TPREF); c.nodes = (WNode *)at(NA_NODES);
-    c.nodes32 = (WNode32 *)at(NA_NODES32); c.icount = (int *)at(NA_ICOUNT);
+    c.nodes32 = (WNode32 *)at(NA_NODES32); c.nlast = (int *)at(NA_NLAST);
+    c.icount = (int *)at(NA_ICOUNT);
     c.wcost = (int *)at(NA_WCOST); c.worder = (int *)at(NA_WORDER);
     c.xyzm = (double4 *)at(NA_XYZM);
     c.rs_ctl = (RsCtl *)at(NA_RSCTL); c.rs_status = (uint64_t *)at(NA_RSSTATUS);
@@ -1247,6 +1293,7 @@ int cosmosim_ext_forces_noalloc(const double *x, const double *y, const double *
     clear_words(c.rs_status, 256 * (size_t)blocks_for(c.cap, RS_TILE), c.stream);
     clear_words(c.idx, (size_t)c.cap, c.stream);
     clear_words(c.idx2, (size_t)c.cap, c.stream);
+    clear_words(c.widx, (size_t)c.cap, c.stream);
     CSG_LAUNCHED();
     c.theta_hint = theta;
     if ((s = launch_build(&c, precision != COSMOSIM_PREC_FP32))) return s;

@@ -180,6 +180,10 @@ int cosmosim_ext_quasar_config(SimHandle h, CosmosimQuasarConfig *cfg, int set);
  * next) fails as a tree needing more than the node capacity would, so a call
  * that runs through it must return COSMOSIM_E_CAPACITY; 0 cancels. */
 int cosmosim_ext_debug_fail_build(SimHandle h, int nth);
+/* Test hook: the last tree's body orders (csg_debug_orders): sorted order, the
+ * FP32 walk's Peano-Hilbert wavefront order and the sorted keys, n_live each. */
+int cosmosim_ext_debug_orders(SimHandle h, int *sorted_idx, int *wave_idx,
+                              unsigned long long *keys, int n, int *n_live);
 /* Last error recorded on the handle by a void legacy call (cosmosim_step). */
 int cosmosim_ext_handle_error(SimHandle h);
 /* Last error recorded on the calling thread by octree_* / integrator_*. */

@@ -126,6 +126,13 @@ int  csg_synchronize(csg_ctx *ctx);
 /* Test hook: the nth build from now (1 = the next) runs with no node capacity,
  * so it fails as a tree that does not fit would; 0 cancels. */
 int  csg_debug_fail_build(csg_ctx *ctx, int nth);
+/* Test hook: the last tree's body orders, n_live entries each (any output may be
+ * NULL): the sorted order (body indices by octree key), the FP32 walk's
+ * wavefront order (the same bodies in Peano-Hilbert order of the cells) and the
+ * sorted 48-bit keys (16 levels of octant digits, the first level in bits 47..45).
+ * COSMOSIM_E_CAPACITY when n < n_live (*n_live is set first). */
+int  csg_debug_orders(csg_ctx *ctx, int *sorted_idx, int *wave_idx, unsigned long long *keys,
+                      int n, int *n_live);
 
 /* Adjoint (no counterpart in the reference).
  * csg_walk_taped: the walk, also recording its tape (the visited nodes and the

@@ -263,24 +263,27 @@ def test_runs_are_deterministic():
 
 
 def test_fp32_traversal_forms_agree(monkeypatch):
-    """The default FP32 walk (one traversal per wavefront on lane masks) and the
-    grouped walk with an explicit DFS position at 64 lanes visit the same nodes in
-    the same order with the same arithmetic: bit-identical accelerations, equal
-    counts.  Narrower groups fold their compensated sums at other steps: equal
-    counts, accelerations within FP32 rounding."""
+    """The FP32 wave-mask walk (one traversal per wavefront on lane masks) with
+    wavefronts of sorted bodies and the grouped walk with an explicit DFS position
+    at 64 lanes visit the same nodes in the same order with the same arithmetic:
+    bit-identical accelerations, equal counts; the dispatch order changes nothing.
+    Narrower groups, and the default Peano-Hilbert wavefronts, fold their
+    compensated sums at other steps: equal counts, accelerations within FP32
+    rounding."""
     b = ca.generate(ca.IC_MERGER, 50000, 7)
     b["mass"][::101] = 0.0
     res = {}
-    for gs in ["0", "64", "32", "8", "0-unordered"]:
+    for gs in ["0", "0-sorted", "64", "32", "8", "0-unordered"]:
         monkeypatch.setenv("COSMOSIM_WALK_GROUP32", gs.split("-")[0])
         monkeypatch.setenv("COSMOSIM_WAVE_ORDER", "0" if "unordered" in gs else "1")
+        monkeypatch.setenv("COSMOSIM_HILBERT_WAVES", "0" if "sorted" in gs else "1")
         sim = sim_for(b, 0.005, 0.6, precision="fp32")
         sim.advance(2)          # later walks dispatch longest-first from recorded costs
         res[gs] = forces_on(sim, 0.6, "fp32")
         sim.close()
-    assert same_bits(res["64"][0], res["0"][0])
+    assert same_bits(res["64"][0], res["0-sorted"][0])
     assert same_bits(res["0-unordered"][0], res["0"][0])
-    for gs in ["64", "32", "8"]:
+    for gs in ["0-sorted", "64", "32", "8"]:
         assert np.array_equal(res[gs][1], res["0"][1])
         assert rel_err(res[gs][0], res["0"][0]).max() <= 1e-5
     assert np.all(res["0"][0][b["mass"] <= 0] == 0.0)
