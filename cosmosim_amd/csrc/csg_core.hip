@@ -38,7 +38,6 @@ struct csg_ctx {
     uint64_t *rs_status = nullptr;     // body sort: per-tile digit counts (look-back)
     unsigned rs_epoch = 0;             // epoch of the last sort pass
     double *xs = nullptr, *ys = nullptr, *zs = nullptr, *ms = nullptr;  // sorted copies
-    int *depth_part = nullptr;
     signed char *lcp = nullptr;
     signed char *lmin1 = nullptr, *lmin2 = nullptr;  // lcp minima per 64 / 4096 sorted bodies
     int *cnt = nullptr, *nbase = nullptr, *scan_part = nullptr;
@@ -175,7 +174,6 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->m); dfree(c->aos);
     dfree(c->bbox_part); dfree(c->key); dfree(c->key2); dfree(c->ext); dfree(c->ext_tmp);
     dfree(c->idx); dfree(c->idx2); dfree(c->widx); dfree(c->nlast); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
-    dfree(c->depth_part);
     dfree(c->lcp); dfree(c->lmin1); dfree(c->lmin2); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->tpref); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
     dfree(c->wcost); dfree(c->worder); dfree(c->xyzm);
@@ -198,7 +196,7 @@ static void clear_words(T *p, size_t n, hipStream_t st) {
 }
 
 // Chunk minima of the common-prefix depths: k_lcp writes 4 per block of
-// CSG_TPB bodies (one per wavefront of 64), k_iscan_top one per 64 of those;
+// CSG_TPB bodies (one per wavefront of 64), k_iscan_down one per 64 of those;
 // padded so 16-byte reads past the last entry stay inside.
 static constexpr size_t lmin1_count(size_t cap) { return 4 * ((cap + 1 + CSG_TPB - 1) / CSG_TPB); }
 static constexpr size_t lmin1_bytes(size_t cap) { return (lmin1_count(cap) + 63) / 64 * 64 + 16; }
@@ -219,7 +217,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->idx, C) && dalloc(c->idx2, C) && dalloc(c->widx, C) &&
               dalloc(c->rs_hist, (size_t)256 * nb) &&
               dalloc(c->rs_tot, 256) && dalloc(c->xs, C) && dalloc(c->ys, C) && dalloc(c->zs, C) &&
-              dalloc(c->ms, C) && dalloc(c->depth_part, (size_t)blocks_for((long long)C1, CSG_TPB)) &&
+              dalloc(c->ms, C) &&
               dalloc(c->lcp, C1 + 64) && dalloc(c->lmin1, lmin1_bytes(C)) &&
               dalloc(c->lmin2, lmin2_bytes(C)) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
@@ -634,7 +632,7 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
     if (n > 0) {
         const int nt = blocks_for(n, RS_TILE);
         k_keys<<<nt, RS_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bp, c->key, c->xyzm,
-                                      c->rs_ctl);
+                                      c->rs_ctl, c->scan_part, blocks_for(n + 1, CSG_TILE));
         if (c->rs_epoch > 0x7fffff00u) {   // epochs are 31 bits: start the words over
             clear_words(c->rs_status, (size_t)256 * blocks_for(c->cap, RS_TILE), st);
             c->rs_epoch = 0;
@@ -663,13 +661,11 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
     const int nlb = blocks_for(n1, CSG_TPB);
     // k_lcp also writes the sorted bodies' Peano-Hilbert keys into key2 (the sort's
     // spare key buffer)
-    k_lcp<<<nlb, CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt, c->depth_part,
+    k_lcp<<<nlb, CSG_TPB, 0, st>>>(c->key, c->ext, c->bp, c->lcp, c->cnt, c->scan_part,
                                    c->lmin1, c->key2);
     const int nt = blocks_for(n1, CSG_TILE);
-    k_iscan_reduce<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->bp, c->scan_part);
-    k_iscan_top<<<1, CSG_TPB, 0, st>>>(c->scan_part, nt, c->depth_part, nlb, c->bp, c->lmin1,
-                                       4 * nlb, c->lmin2);
-    k_iscan_down<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->scan_part, c->bp, c->nbase);
+    k_iscan_down<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->scan_part, c->bp, c->nbase, c->lmin1,
+                                         4 * nlb, c->lmin2);
     k_gather_dd<<<nt, CSG_TPB, 0, st>>>(c->idx, c->xyzm, c->bp, c->xs, c->ys, c->zs, c->ms,
                                         c->dd_part, c->tpref);
     // k_dd_down2 also clears cnt for k_emit's Peano-Hilbert offsets
@@ -679,10 +675,13 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
     k_emit<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
         c->key, c->ext, c->idx, c->lcp, c->lmin1, c->lmin2, c->nbase, c->pref, c->cap + 1, c->xs,
         c->ys, c->zs, c->ms, c->bp, with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->nlast);
+    // the Peano-Hilbert wavefront order: the child-boundary steps go to the tile
+    // prefixes' buffer, which k_dd_down2 is done with
+    int *const hbnd = (int *)c->tpref;
     k_hilbert<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(c->lcp, c->nbase, c->nlast,
-                                                                   c->key2, c->bp, c->cnt);
-    k_iscan_reduce<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->bp, c->scan_part);
-    k_hrank<<<nt, CSG_TPB, 0, st>>>(c->cnt, c->scan_part, c->idx, c->bp, c->widx);
+                                                                   c->key2, c->bp, c->cnt, hbnd);
+    k_hsum<<<nt, CSG_TPB, 0, st>>>(c->cnt, hbnd, c->lcp, c->bp, c->scan_part);
+    k_hrank<<<nt, CSG_TPB, 0, st>>>(c->cnt, hbnd, c->lcp, c->scan_part, c->idx, c->bp, c->widx);
     CSG_LAUNCHED();
     c->tree_valid = 1;
     c->geom_valid = 0;
@@ -1082,6 +1081,10 @@ int csg_interaction_counts(csg_ctx *c, int *counts, int n) {
 int csg_get_stats(csg_ctx *c, csg_stats *o) {
     CSG_USE(c);
     if (!o) return COSMOSIM_E_ARG;
+    if (c->tree_valid && c->n > 0) {   // the deepest leaf is found on demand
+        k_max_depth<<<blocks_for(c->cap, CSG_TILE), CSG_TPB, 0, c->stream>>>(c->lcp, c->bp);
+        CSG_LAUNCHED();
+    }
     int s = read_params(c);
     if (s) return s;
     unsigned long long it = 0;
@@ -1166,7 +1169,7 @@ int csg_debug_orders(csg_ctx *c, int *sorted_idx, int *wave_idx, unsigned long l
 enum {
     NA_BBOX, NA_KEY, NA_KEY2, NA_EXT, NA_EXT_TMP, NA_IDX, NA_IDX2, NA_WIDX, NA_XS, NA_YS, NA_ZS,
     NA_MS,
-    NA_DEPTH, NA_LCP, NA_LMIN1, NA_LMIN2, NA_CNT, NA_NBASE, NA_SCAN, NA_PREF,
+    NA_LCP, NA_LMIN1, NA_LMIN2, NA_CNT, NA_NBASE, NA_SCAN, NA_PREF,
     NA_DDPART, NA_TPREF,
     NA_NODES, NA_NODES32, NA_NLAST, NA_ICOUNT, NA_WCOST, NA_WORDER, NA_XYZM, NA_RSCTL,
     NA_RSSTATUS, NA_BP, NA_ITOTAL, NA_REGIONS
@@ -1187,7 +1190,7 @@ constexpr NoallocLayout noalloc_layout(int cap) {
         C * sizeof(uint64_t), C * sizeof(uint64_t), 3 * C * sizeof(uint64_t),
         3 * C * sizeof(uint64_t), C * sizeof(int), C * sizeof(int), C * sizeof(int),
         C * sizeof(double), C * sizeof(double), C * sizeof(double), C * sizeof(double),
-        ((C1 + CSG_TPB - 1) / CSG_TPB) * sizeof(int), C1 + 64,
+        C1 + 64,
         lmin1_bytes(C), lmin2_bytes(C), C1 * sizeof(int), C1 * sizeof(int), nb * sizeof(int), 4 * C1 * sizeof(DD),
         4 * nb * sizeof(DD), 4 * (size_t)CSG_TPB * nb * sizeof(DD),
         nodes * sizeof(WNode), nodes * sizeof(WNode32), nodes * sizeof(int), C * sizeof(int),
@@ -1275,7 +1278,6 @@ int cosmosim_ext_forces_noalloc(const double *x, const double *y, const double *
     c.idx = (int *)at(NA_IDX); c.idx2 = (int *)at(NA_IDX2); c.widx = (int *)at(NA_WIDX);
     c.xs = (double *)at(NA_XS); c.ys = (double *)at(NA_YS); c.zs = (double *)at(NA_ZS);
     c.ms = (double *)at(NA_MS);
-    c.depth_part = (int *)at(NA_DEPTH);
     c.lcp = (signed char *)at(NA_LCP);
     c.lmin1 = (signed char *)at(NA_LMIN1); c.lmin2 = (signed char *)at(NA_LMIN2);
     c.cnt = (int *)at(NA_CNT); c.nbase = (int *)at(NA_NBASE);

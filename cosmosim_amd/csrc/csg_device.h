@@ -448,7 +448,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_bbox_final(const double *__restrict
 // gather after the sort reads one sector per body instead of four, and counts
 // the digit values of every sort pass (all passes' offsets from this one pass
 // over the bodies): tiles of RS_TILE bodies, per-tile counts in LDS added to the
-// sort's global counts.
+// sort's global counts.  Clears the npart tile counters of the node-count scan.
 __global__ __launch_bounds__(RS_TPB) void k_keys(const double *__restrict__ x,
                                                  const double *__restrict__ y,
                                                  const double *__restrict__ z,
@@ -456,8 +456,10 @@ __global__ __launch_bounds__(RS_TPB) void k_keys(const double *__restrict__ x,
                                                  const BuildParams *__restrict__ bp,
                                                  uint64_t *__restrict__ key,
                                                  double4 *__restrict__ xyzm,
-                                                 RsCtl *__restrict__ ctl) {
+                                                 RsCtl *__restrict__ ctl,
+                                                 int *__restrict__ part, int npart) {
     __shared__ int h[CSG_SORT_PASSES][256];
+    for (int g = blockIdx.x * RS_TPB + threadIdx.x; g < npart; g += gridDim.x * RS_TPB) part[g] = 0;
     for (int k = threadIdx.x; k < CSG_SORT_PASSES * 256; k += RS_TPB) (&h[0][0])[k] = 0;
     __syncthreads();
     const double r0 = bp->root[0], r1 = bp->root[1], r2 = bp->root[2], r3 = bp->root[3];
@@ -956,19 +958,19 @@ __device__ __forceinline__ uint64_t hilbert_key(uint64_t key, const unsigned cha
 // and hkey[b], the Peano-Hilbert key of sorted body b.
 __global__ __launch_bounds__(CSG_TPB) void k_lcp(const uint64_t *__restrict__ key,
                                                  const uint64_t *__restrict__ ext,
-                                                 const BuildParams *__restrict__ bp,
+                                                 BuildParams *__restrict__ bp,
                                                  signed char *__restrict__ L,
                                                  int *__restrict__ cnt,
-                                                 int *__restrict__ depth_part,
+                                                 int *__restrict__ part,
                                                  signed char *__restrict__ M1,
                                                  uint64_t *__restrict__ hkey) {
-    __shared__ int wmax[4];
+    __shared__ int wsum[4];
     __shared__ unsigned char tab[24 * 8];
     for (int k = threadIdx.x; k < 24 * 8; k += CSG_TPB) tab[k] = c_hilbert[k];
     __syncthreads();
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     const int nl = bp->n_live;
-    int leafdepth = 0, lmin = 127;
+    int lmin = 127, c = 0;
     if (b < nl) {
         hkey[b] = hilbert_key(key[b], tab);
         const int lc = (b + 1 < nl) ? lcp_sorted(key, ext, b, b + 1) : -1;
@@ -976,21 +978,43 @@ __global__ __launch_bounds__(CSG_TPB) void k_lcp(const uint64_t *__restrict__ ke
         L[b] = (signed char)lc;
         lmin = lc;
         const int opened = max(0, min(lc, 63) - min(lp, 63));
-        cnt[b] = opened + (lp < 64 ? 1 : 0);
-        leafdepth = min(max(lp, lc) + 1, 64);
+        c = opened + (lp < 64 ? 1 : 0);
+        cnt[b] = c;
     } else if (b == nl) {
         cnt[b] = 0;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        leafdepth = max(leafdepth, __shfl_xor(leafdepth, o, 64));
         lmin = min(lmin, __shfl_xor(lmin, o, 64));
+        c += __shfl_xor(c, o, 64);
     }
-    if ((threadIdx.x & 63) == 0) M1[b >> 6] = (signed char)lmin;
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = leafdepth;
+    if ((threadIdx.x & 63) == 0) {
+        M1[b >> 6] = (signed char)lmin;
+        wsum[threadIdx.x >> 6] = c;
+    }
     __syncthreads();
+    // the block's node count into its scan tile's counter (cleared by k_keys)
     if (threadIdx.x == 0)
-        depth_part[blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        atomicAdd(&part[blockIdx.x / (CSG_TILE / CSG_TPB)], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+}
+
+// Deepest leaf of the last tree, for the statistics only (bp->max_depth, cleared
+// by k_bbox_final): a sorted body's leaf sits one level below the deeper of its
+// common prefixes with its neighbours (64 at most).  CSG_TILE bodies per block.
+__global__ __launch_bounds__(CSG_TPB) void k_max_depth(const signed char *__restrict__ L,
+                                                       BuildParams *__restrict__ bp) {
+    __shared__ int wmax[4];
+    const int nl = bp->n_live;
+    int d = 0;
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        const int b = blockIdx.x * CSG_TILE + r * CSG_TPB + threadIdx.x;
+        if (b < nl) d = max(d, min(max(b > 0 ? (int)L[b - 1] : -1, (int)L[b]) + 1, 64));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) d = max(d, __shfl_xor(d, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(&bp->max_depth, max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])));
 }
 
 // Sorted copies of position and mass (coalesced reads for the scans and emit).
@@ -1017,7 +1041,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_gather_sorted4(
 }
 
 // ============================================================ int scan
-// Exclusive scan of cnt[0..n_live] into base[]; n_nodes = base[n_live].
+// Per-tile sums of in[0..n_live], CSG_TILE entries per block, into part.
 __global__ __launch_bounds__(CSG_TPB) void k_iscan_reduce(const int *__restrict__ in,
                                                           const BuildParams *__restrict__ bp,
                                                           int *__restrict__ part) {
@@ -1032,62 +1056,35 @@ __global__ __launch_bounds__(CSG_TPB) void k_iscan_reduce(const int *__restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-// Also M2[s] = min of M1 over the 64 chunks of superchunk s (4096 bodies), for
-// the nm1 chunk minima k_lcp wrote.
-__global__ __launch_bounds__(CSG_TPB) void k_iscan_top(int *__restrict__ part, int nb,
-                                                       const int *__restrict__ depth_part,
-                                                       int ndp, BuildParams *bp,
-                                                       const signed char *__restrict__ M1,
-                                                       int nm1, signed char *__restrict__ M2) {
-    __shared__ int lds4[4];
-    for (int sc = threadIdx.x; sc < (nm1 + 63) / 64; sc += CSG_TPB) {
-        int m = 127;
-        const int c0 = sc * 64, c1 = min(c0 + 64, nm1);
-        if (c1 - c0 == 64) {
-            const int4 *q = (const int4 *)(M1 + c0);
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                const int4 v = q[w];
-                const int vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (int j = 0; j < 16; ++j) m = min(m, (int)(signed char)(vv[j >> 2] >> (8 * (j & 3))));
-            }
-        } else {
-            for (int c = c0; c < c1; ++c) m = min(m, (int)M1[c]);
-        }
-        M2[sc] = (signed char)m;
-    }
-    int carry = 0;
-    for (int c0 = 0; c0 < nb; c0 += CSG_TPB) {
-        const int i = c0 + threadIdx.x;
-        const int v = i < nb ? part[i] : 0;
-        int total;
-        const int ex = block_excl_scan(v, lds4, total);
-        if (i < nb) part[i] = carry + ex;
-        carry += total;
-    }
-    // deepest leaf of the tree
-    int d = 0;
-    for (int k = threadIdx.x; k < ndp; k += CSG_TPB) d = max(d, depth_part[k]);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) d = max(d, __shfl_xor(d, o, 64));
-    __shared__ int wmax[4];
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d;
-    __syncthreads();
-    if (threadIdx.x == 0) bp->max_depth = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-}
-
+// Exclusive scan of in[0..n_live] (the node counts) into out, one tile of CSG_TILE
+// per block: the tile sums k_lcp accumulated in part (the tiles before this one,
+// summed here) and a scan inside the tile; the entry past the last body is the
+// node count.  Block t also writes M2[t], the minimum of the 64 chunk minima
+// M1[64 t, 64 t + 64) (the superchunk summary cell_last searches).
 __global__ __launch_bounds__(CSG_TPB) void k_iscan_down(const int *__restrict__ in,
                                                         const int *__restrict__ part,
-                                                        BuildParams *bp, int *__restrict__ out) {
+                                                        BuildParams *bp, int *__restrict__ out,
+                                                        const signed char *__restrict__ M1,
+                                                        int nm1, signed char *__restrict__ M2) {
     __shared__ int lds4[4];
+    if (threadIdx.x < 64 && (int)blockIdx.x * 64 < nm1) {
+        const int i = blockIdx.x * 64 + threadIdx.x;
+        int m = i < nm1 ? (int)M1[i] : 127;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+        if (threadIdx.x == 0) M2[blockIdx.x] = (signed char)m;
+    }
     const int nl = bp->n_live, len = nl + 1;
+    int c = 0;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += CSG_TPB) c += part[k];
+    int carry;
+    (void)block_excl_scan(c, lds4, carry);
     const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
     int v[CSG_ITEMS], s = 0;
 #pragma unroll
     for (int r = 0; r < CSG_ITEMS; ++r) { v[r] = (base + r < len) ? in[base + r] : 0; s += v[r]; }
     int total;
-    int run = block_excl_scan(s, lds4, total) + part[blockIdx.x];
+    int run = block_excl_scan(s, lds4, total) + carry;
 #pragma unroll
     for (int r = 0; r < CSG_ITEMS; ++r) {
         const int i = base + r;
@@ -1579,15 +1576,20 @@ __global__ __launch_bounds__(CSG_TPB) void k_emit(
 // recorded (the first child is the next node in pre-order, a later one the first
 // node its first member opens), their Hilbert digits from the members' Hilbert
 // keys.  Each child's Hilbert offset within the cell minus its sorted offset is
-// added over its members as a difference array: at each child boundary the
-// change from the previous child's value (one atomic per boundary).  A sorted
-// body's Hilbert rank is its position plus the prefix sum of hdiff up to it.
+// added over its members as a difference array: the change from one child's
+// value to the next at each boundary between two children, stored in hbnd (a
+// position k is such a boundary of exactly one cell, the one of depth L[k-1]),
+// the first child's value at the cell's first member and minus the last child's
+// past its last member, added into hdiff (cells share those positions).  A
+// sorted body's Hilbert rank is its position plus the prefix sum up to it
+// (hilbert_step).
 __global__ __launch_bounds__(CSG_TPB) void k_hilbert(const signed char *__restrict__ L,
                                                      const int *__restrict__ nbase,
                                                      const int *__restrict__ nlast,
                                                      const uint64_t *__restrict__ hkey,
                                                      const BuildParams *__restrict__ bp,
-                                                     int *__restrict__ hdiff) {
+                                                     int *__restrict__ hdiff,
+                                                     int *__restrict__ hbnd) {
     __shared__ int s_off[CSG_TPB], s_lo[CSG_TPB], s_id[CSG_TPB];
     constexpr int KL = CSG_KEY_LEVELS;
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
@@ -1627,18 +1629,53 @@ __global__ __launch_bounds__(CSG_TPB) void k_hilbert(const signed char *__restri
                     if (h[i] < h[j]) off += f[i + 1] - f[i];
                 cur = off - (f[j] - s);
             }
-            if (cur != prev) atomicAdd(&hdiff[f[j]], cur - prev);   // f[j] <= e + 1 <= n_live
+            // f[j] <= e + 1 <= n_live
+            if (j > 0 && in) hbnd[f[j]] = cur - prev;
+            else if (cur != prev) atomicAdd(&hdiff[f[j]], cur - prev);
             prev = cur;
             if (!in) break;
         }
     });
 }
 
+// The rank step at sorted position k < n_live: hdiff[k], plus hbnd[k] when k is a
+// boundary between two children of a cell of depth < CSG_KEY_LEVELS.
+__device__ __forceinline__ int hilbert_step(const int *__restrict__ hdiff,
+                                            const int *__restrict__ hbnd,
+                                            const signed char *__restrict__ L, int k) {
+    int v = hdiff[k];
+    if (k > 0) {
+        const int d = L[k - 1];
+        if (d >= 0 && d < CSG_KEY_LEVELS) v += hbnd[k];
+    }
+    return v;
+}
+
+// Per-tile sums of the rank steps of positions < n_live, CSG_TILE per block.
+__global__ __launch_bounds__(CSG_TPB) void k_hsum(const int *__restrict__ hdiff,
+                                                  const int *__restrict__ hbnd,
+                                                  const signed char *__restrict__ L,
+                                                  const BuildParams *__restrict__ bp,
+                                                  int *__restrict__ part) {
+    __shared__ int lds4[4];
+    const int nl = bp->n_live;
+    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r)
+        if (base + r < nl) s += hilbert_step(hdiff, hbnd, L, base + r);
+    int total;
+    (void)block_excl_scan(s, lds4, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
 // Peano-Hilbert rank of every sorted body b < n_live: b plus the prefix sum of
-// hdiff up to b (the tile sums hpart of k_iscan_reduce before this tile, then a
-// scan inside it); the walk order gets widx[rank] = idx[b].  A rank outside [0, n_live) (a broken
+// its rank steps (the tile sums hpart of k_hsum before this tile, then a scan
+// inside it); the walk order gets widx[rank] = idx[b].  A rank outside [0, n_live) (a broken
 // difference array) is not stored and flags order_bad, which fails the build.
 __global__ __launch_bounds__(CSG_TPB) void k_hrank(const int *__restrict__ hdiff,
+                                                   const int *__restrict__ hbnd,
+                                                   const signed char *__restrict__ L,
                                                    const int *__restrict__ hpart,
                                                    const int *__restrict__ idx,
                                                    BuildParams *__restrict__ bp,
@@ -1655,7 +1692,10 @@ __global__ __launch_bounds__(CSG_TPB) void k_hrank(const int *__restrict__ hdiff
     const int base = tile0 + threadIdx.x * CSG_ITEMS;
     int v[CSG_ITEMS], sum = 0;
 #pragma unroll
-    for (int r = 0; r < CSG_ITEMS; ++r) { v[r] = (base + r < nl) ? hdiff[base + r] : 0; sum += v[r]; }
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        v[r] = (base + r < nl) ? hilbert_step(hdiff, hbnd, L, base + r) : 0;
+        sum += v[r];
+    }
     int total;
     int run = carry + block_excl_scan(sum, lds4, total);
 #pragma unroll
