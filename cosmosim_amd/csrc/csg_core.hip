@@ -695,16 +695,19 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
 extern "C++" {
 // FP64 walks: dispatched through `wo` (grid = blocks for the ordered slots) or
 // the XCD-chunked mapping (grid = nb8); the grouped FP32 forms ignore the order.
+// sidx64: the FP64 walks' body order (each lane sums its own terms in FP64, so the
+// grouping into wavefronts changes nothing in the results; Peano-Hilbert order
+// shortens the traversals); the grouped FP32 walk keeps sorted order.
 template <int GS>
-static void launch_walk_gs(csg_ctx *c, int precision, double G, double eps2, double th2, int grid,
-                           int nb8, TapeOut tape, WaveOrder wo, int tpb) {
+static void launch_walk_gs(csg_ctx *c, int precision, const int *sidx64, double G, double eps2,
+                           double th2, int grid, int nb8, TapeOut tape, WaveOrder wo, int tpb) {
     hipStream_t st = c->stream;
     if (precision == COSMOSIM_PREC_FP64_STRICT)
-        k_walk<WALK_FP64_STRICT, GS><<<grid, tpb, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y,
+        k_walk<WALK_FP64_STRICT, GS><<<grid, tpb, 0, st>>>(c->nodes, c->bp, sidx64, c->x, c->y,
             c->z, c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape,
             wo);
     else if (precision == COSMOSIM_PREC_FP64)
-        k_walk<WALK_FP64, GS><<<grid, tpb, 0, st>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
+        k_walk<WALK_FP64, GS><<<grid, tpb, 0, st>>>(c->nodes, c->bp, sidx64, c->x, c->y, c->z,
             c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape, wo);
     else
         k_walk32<GS><<<nb8, CSG_TPB, 0, st>>>(c->nodes32, c->bp, c->idx, c->x, c->y, c->z, c->ax, c->ay,
@@ -720,9 +723,9 @@ static void launch_walk32w(csg_ctx *c, const int *sidx, double G, double eps2, d
 }
 
 template <int PREC, bool TP>
-static void launch_walk64w(csg_ctx *c, double G, double eps2, double th2, int grid, int nb8,
-                           TapeOut tape, WaveOrder wo, int tpb) {
-    k_walk64w<PREC, TP><<<grid, tpb, 0, c->stream>>>(c->nodes, c->bp, c->idx, c->x, c->y, c->z,
+static void launch_walk64w(csg_ctx *c, const int *sidx, double G, double eps2, double th2,
+                           int grid, int nb8, TapeOut tape, WaveOrder wo, int tpb) {
+    k_walk64w<PREC, TP><<<grid, tpb, 0, c->stream>>>(c->nodes, c->bp, sidx, c->x, c->y, c->z,
         c->ax, c->ay, c->az, c->icount, c->itotal, G, eps2, th2, nb8, c->xcd_chunk, tape, wo);
 }
 }  // extern "C++"
@@ -809,6 +812,9 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         return COSMOSIM_OK;
     }
     c->counts_valid = 1;
+    // the untaped FP64 walks take their bodies in Peano-Hilbert order (results do not
+    // depend on the grouping); a taped walk keeps sorted order, like the FP32 one
+    const int *sidx64 = (hilbert && !tape.rec && c->hilbert_waves) ? c->widx : c->idx;
     // FP64: the wave-mask walk (also for tapes, which are per wavefront) when the
     // records span < 4 GiB, else the grouped walk
     const int g64 = c->walk_group >= 0 ? c->walk_group
@@ -817,11 +823,11 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         (size_t)c->node_cap * sizeof(WNode) < ((size_t)1 << 32)) {
         const bool taping = tape.rec != nullptr;
         if (precision == COSMOSIM_PREC_FP64_STRICT) {
-            if (taping) launch_walk64w<WALK_FP64_STRICT, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
-            else launch_walk64w<WALK_FP64_STRICT, false>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
+            if (taping) launch_walk64w<WALK_FP64_STRICT, true>(c, c->idx, G, eps2, th2, grid, nb8, tape, wo, tpb);
+            else launch_walk64w<WALK_FP64_STRICT, false>(c, sidx64, G, eps2, th2, grid, nb8, tape, wo, tpb);
         } else {
-            if (taping) launch_walk64w<WALK_FP64, true>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
-            else launch_walk64w<WALK_FP64, false>(c, G, eps2, th2, grid, nb8, tape, wo, tpb);
+            if (taping) launch_walk64w<WALK_FP64, true>(c, c->idx, G, eps2, th2, grid, nb8, tape, wo, tpb);
+            else launch_walk64w<WALK_FP64, false>(c, sidx64, G, eps2, th2, grid, nb8, tape, wo, tpb);
         }
         CSG_LAUNCHED();
         return COSMOSIM_OK;
@@ -829,10 +835,10 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     // a taped walk runs with whole-wave traversal (the tape is per wavefront)
     const int gs = tape.rec ? 64 : (precision == COSMOSIM_PREC_FP32 ? c->walk_group32 : g64);
     switch (gs) {
-    case 8: launch_walk_gs<8>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-    case 16: launch_walk_gs<16>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-    case 32: launch_walk_gs<32>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
-    default: launch_walk_gs<64>(c, precision, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+    case 8: launch_walk_gs<8>(c, precision, sidx64, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+    case 16: launch_walk_gs<16>(c, precision, sidx64, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+    case 32: launch_walk_gs<32>(c, precision, sidx64, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
+    default: launch_walk_gs<64>(c, precision, sidx64, G, eps2, th2, grid, nb8, tape, wo, tpb); break;
     }
     CSG_LAUNCHED();
     return COSMOSIM_OK;

@@ -12,7 +12,8 @@ depth-64 cell and tiny counts:
     (independent of the table);
   * the FP32 forces with Hilbert wavefronts equal those with sorted-order
     wavefronts to FP32 rounding, and the per-body interaction counts are equal:
-    the grouping changes only which bodies share a wavefront.
+    the grouping changes only which bodies share a wavefront;
+  * the FP64 walks' forces do not depend on the grouping at all (bit-identical).
 """
 import os
 
@@ -144,3 +145,21 @@ def test_hilbert_wavefronts_change_only_the_grouping(case):
     assert np.all(a1[~live] == 0.0)
     s1.close()
     s0.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64_strict"])
+def test_fp64_walks_do_not_depend_on_the_grouping(precision):
+    """Each lane of an FP64 walk sums its own terms in FP64 (a lane that does not
+    interact adds an exact zero), so Peano-Hilbert and sorted-order wavefronts give
+    bit-identical accelerations and equal counts."""
+    b = cluster(2)
+    b2 = CASES["merger_100k"]()
+    for bb in (b, b2):
+        s1, s0 = _sim(bb, True), _sim(bb, False)
+        for s in (s1, s0):
+            s.set_params(precision=precision)
+            s.compute_forces()
+        assert np.array_equal(s1.accelerations().view(np.uint64), s0.accelerations().view(np.uint64))
+        assert np.array_equal(s1.interaction_counts(), s0.interaction_counts())
+        s1.close()
+        s0.close()
