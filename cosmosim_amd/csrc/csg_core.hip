@@ -98,6 +98,10 @@ struct csg_ctx {
     int walk_group = -1;
     int walk_group32 = 0;   // FP32 walk: 0 = wave-mask traversal, else lanes per group
     int hilbert_waves = 1;  // untaped FP32 wave-mask walks take wavefronts in Hilbert order
+    // the kick-drift before the next walk already wrote its dispatch order (for these
+    // waves per block, slots per XCD and waves), zeroed the dead bodies'
+    // accelerations and the interaction total
+    int order_ready = 0, order_gw = 0, order_capg = 0, order_nw = 0;
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
     int nodes64_valid = 0;  // the FP64 node records belong to the current tree
     double4 *xyzm = nullptr;  // build scratch: (x, y, z, m) interleaved, body order
@@ -590,12 +594,44 @@ static int bbox_blocks(int n) {
                  : 1;
 }
 
+// Whether the next walk at `precision` (taped or not) over nw waves dispatches
+// in longest-first order from the previous walk's step counts, and then its waves
+// per block and slots per XCD range.
+static bool walk_order_params(const csg_ctx *c, int precision, bool taped, int nw, int *gw,
+                              int *capg) {
+    const bool wave_mask32 = precision == COSMOSIM_PREC_FP32 && (c->walk_group32 == 0 || taped);
+    const bool grouped32 = precision == COSMOSIM_PREC_FP32 && !wave_mask32;
+    if (grouped32 || !c->wave_order || c->wcost_n != nw) return false;
+    // blocks of gw waves (one by default: the finest balance; larger blocks, whose
+    // waves share a CU's scalar cache, measured slower).  The FP64 walks' kernels are
+    // bounded at CSG_TPB threads.
+    *gw = wave_mask32 ? c->walk_block_waves : min(c->walk_block_waves, CSG_TPB / 64);
+    *capg = WO_CAP(blocks_for(nw, *gw));
+    return true;
+}
+
 // bbox (may be null): also leave the drifted positions' bounding-box partials
 // in c->bbox_part and their block count in *bbox, for the build that follows.
-static int launch_kick_drift(csg_ctx *c, double dt, int kicks, int *bbox = nullptr) {
+// next_prec >= 0 (with bbox): the precision of the untaped walk that follows the
+// build; when that walk dispatches in longest-first order, the order is computed
+// in the same launch (k_kick_drift_order) and the walk skips its own.
+static int launch_kick_drift(csg_ctx *c, double dt, int kicks, int *bbox = nullptr,
+                             int next_prec = -1) {
     if (bbox) *bbox = 0;
     if (c->n == 0) return COSMOSIM_OK;
-    if (bbox) {
+    const int nw = blocks_for(c->n, 64);
+    int gw = 1, capg = 0;
+    if (bbox && next_prec >= 0 && walk_order_params(c, next_prec, false, nw, &gw, &capg)) {
+        const int nkd = min(blocks_for(c->n, 1024), CSG_BBOX_BLOCKS);
+        *bbox = nkd;
+        k_kick_drift_order<<<WO_XCDS + nkd, 1024, 0, c->stream>>>(c->x, c->y, c->z, c->vx,
+            c->vy, c->vz, c->ax, c->ay, c->az, c->m, c->n, dt, kicks, c->bbox_part, c->icount,
+            c->itotal, c->wcost, nw, gw, c->wave_split, capg, c->worder);
+        c->order_ready = 1;
+        c->order_gw = gw;
+        c->order_capg = capg;
+        c->order_nw = nw;
+    } else if (bbox) {
         *bbox = bbox_blocks(c->n);
         k_kick_drift_bbox<<<*bbox, CSG_TPB, 0, c->stream>>>(c->x, c->y, c->z, c->vx, c->vy, c->vz,
             c->ax, c->ay, c->az, c->m, c->n, dt, kicks, c->bbox_part);
@@ -748,12 +784,15 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         const int s = launch_build(c, c->nodes64_valid != 0);
         if (s) return s;
     }
+    const bool pre = c->order_ready != 0;   // the kick-drift before did the order and zeroing
+    c->order_ready = 0;
     if (n == 0) {
         k_set_u64<<<1, 1, 0, st>>>(c->itotal, 0ull);
         return COSMOSIM_OK;
     }
-    k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az, c->icount,
-                                                            c->itotal);
+    if (!pre)
+        k_zero_dead<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->m, n, c->ax, c->ay, c->az,
+                                                                c->icount, c->itotal);
     int nb8 = blocks_for(n, CSG_TPB);
     const int gran = c->xcd_chunk > 0 ? 8 * c->xcd_chunk : 8;
     nb8 = (nb8 + gran - 1) / gran * gran;
@@ -766,13 +805,11 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     const int nw = blocks_for(n, 64);
     WaveOrder wo{nullptr, nw, grouped32 ? nullptr : c->wcost, 0};
     int grid = nb8, tpb = CSG_TPB;
-    if (!grouped32 && c->wave_order && c->wcost_n == nw) {
-        // blocks of gw waves (one by default: the finest balance; larger blocks,
-        // whose waves share a CU's scalar cache, measured slower).  The FP64
-        // walks' kernels are bounded at CSG_TPB threads.
-        const int gw = wave_mask32 ? c->walk_block_waves : min(c->walk_block_waves, CSG_TPB / 64);
-        const int capg = WO_CAP(blocks_for(nw, gw));
-        k_wave_order<<<WO_XCDS, 1024, 0, st>>>(c->wcost, nw, gw, c->wave_split, capg, c->worder);
+    int gw = 1, capg = 0;
+    if (walk_order_params(c, precision, tape.rec != nullptr, nw, &gw, &capg)) {
+        if (!(pre && c->order_gw == gw && c->order_capg == capg && c->order_nw == nw))
+            k_wave_order<<<WO_XCDS, 1024, 0, st>>>(c->wcost, nw, gw, c->wave_split, capg,
+                                                   c->worder);
         wo.order = c->worder;
         wo.slots = WO_XCDS * capg * gw;
         grid = WO_XCDS * capg;
@@ -921,7 +958,7 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
     if (k == 0) return COSMOSIM_OK;
     int s, bbox = 0;
     timer_start(c);
-    if ((s = launch_kick_drift(c, dt, 1, &bbox))) return s;
+    if ((s = launch_kick_drift(c, dt, 1, &bbox, precision))) return s;
     timer_stop(c, PH_KDK);
     for (int sub = 0; sub < k; ++sub) {
         timer_start(c);
@@ -934,7 +971,7 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
         timer_start(c);
         // end-of-substep half kick fused with the next substep's kick + drift
         // (which also bounds the drifted positions for the next build)
-        s = (sub + 1 < k) ? launch_kick_drift(c, dt, 2, &bbox) : launch_kick(c, dt);
+        s = (sub + 1 < k) ? launch_kick_drift(c, dt, 2, &bbox, precision) : launch_kick(c, dt);
         if (s) return s;
         timer_stop(c, PH_KDK);
     }

@@ -2008,14 +2008,14 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
 #define WO_REG 16
 #define WO_MAX_SPLIT 4
 #define WO_CAP(ng) ((((5 * (ng)) / 32 + 8) + 3) & ~3)
-__global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cost, int nw, int gw,
-                                                     int split, int capg, int *__restrict__ order) {
+__device__ __forceinline__ void wave_order_block(int r, const int *__restrict__ cost, int nw,
+                                                 int gw, int split, int capg,
+                                                 int *__restrict__ order) {
     __shared__ int hist[4096];
     __shared__ long long wsum[16];
     __shared__ int bnd[WO_XCDS * WO_MAX_SPLIT + 1];
     __shared__ int part[16];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    const int r = blockIdx.x;
     const int ng = (nw + gw - 1) / gw;
     const int R = WO_XCDS * split;   // ranges: XCD x takes ranges x split .. x split + split - 1
     // step count of group g (stale or garbage counts are clamped: any value
@@ -2137,6 +2137,80 @@ __global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cos
         __syncthreads();
     }
     for (int j = (xhi - xlo) + t; j < capg; j += 1024) place(j, -1);
+}
+
+__global__ __launch_bounds__(1024) void k_wave_order(const int *__restrict__ cost, int nw, int gw,
+                                                     int split, int capg, int *__restrict__ order) {
+    wave_order_block(blockIdx.x, cost, nw, gw, split, capg, order);
+}
+
+// Between two walks of a substep run, in one launch of 1024-thread blocks: blocks
+// 0 .. WO_XCDS-1 order the next walk's dispatch from the walk just finished
+// (k_wave_order's work); the other blocks do k_kick_drift_bbox's update of the
+// live bodies with one bounding-box partial each (part[8 (block - WO_XCDS) ..]),
+// and k_zero_dead's zero accelerations of the dead bodies and zero interaction
+// total for the next walk.
+__global__ __launch_bounds__(1024, 8) void k_kick_drift_order(
+    double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+    double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    const double *__restrict__ m, int n, double dt, int kicks, double *__restrict__ part,
+    int *__restrict__ icount, unsigned long long *__restrict__ itotal,
+    const int *__restrict__ cost, int nw, int gw, int split, int capg, int *__restrict__ order) {
+    if (blockIdx.x < WO_XCDS) {
+        wave_order_block(blockIdx.x, cost, nw, gw, split, capg, order);
+        return;
+    }
+    const int bid = blockIdx.x - WO_XCDS, nkd = gridDim.x - WO_XCDS;
+    if (bid == 0 && threadIdx.x == 0 && itotal) *itotal = 0ull;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double cnt = 0.0;
+    for (int i = bid * 1024 + threadIdx.x; i < n; i += nkd * 1024) {
+        if (!live_mass(m[i])) {
+            ax[i] = 0.0; ay[i] = 0.0; az[i] = 0.0;
+            if (icount) icount[i] = 0;
+            continue;
+        }
+        double u = vx[i], v = vy[i], w = vz[i];
+        const double hx = 0.5 * ax[i], hy = 0.5 * ay[i], hz = 0.5 * az[i];
+        for (int k = 0; k < kicks; ++k) {
+            u = u + hx * dt;
+            v = v + hy * dt;
+            w = w + hz * dt;
+        }
+        vx[i] = u; vy[i] = v; vz[i] = w;
+        const double p[3] = {x[i] + u * dt, y[i] + v * dt, z[i] + w * dt};
+        x[i] = p[0]; y[i] = p[1]; z[i] = p[2];
+        cnt += 1.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (p[a] < lo[a]) lo[a] = p[a];
+            if (p[a] > hi[a]) hi[a] = p[a];
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fmin(lo[a], __shfl_xor(lo[a], o, 64));
+            hi[a] = fmax(hi[a], __shfl_xor(hi[a], o, 64));
+        }
+        cnt += __shfl_xor(cnt, o, 64);
+    }
+    __shared__ double sw[16][7];
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        for (int a = 0; a < 3; ++a) { sw[wid][a] = lo[a]; sw[wid][3 + a] = hi[a]; }
+        sw[wid][6] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        const int a = threadIdx.x;
+        double v = sw[0][a];
+        for (int k = 1; k < 16; ++k)
+            v = a < 3 ? fmin(v, sw[k][a]) : (a < 6 ? fmax(v, sw[k][a]) : v + sw[k][a]);
+        part[bid * 8 + a] = v;
+    }
 }
 
 // Lane select on a wave mask held in SGPRs: m's bit for this lane ? t : f.
