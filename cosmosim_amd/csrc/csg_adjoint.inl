@@ -594,7 +594,10 @@ static int adj_reserve_records(csg_ctx *c, size_t R) {
 }
 
 // Walk that records the tape; grows the per-wave tape when a wave overflows it.
-static int walk_taped(csg_ctx *c, int precision, double G, double softening, double theta) {
+// counts: also the per-body interaction counts (the public call keeps them; the
+// adjoint's own walks do not read them).
+static int walk_taped(csg_ctx *c, int precision, double G, double softening, double theta,
+                      bool counts = true) {
     // the replay of the tape reads the FP64 node records
     int s0 = ensure_nodes64(c, COSMOSIM_PREC_FP64);
     if (s0) return s0;
@@ -606,7 +609,7 @@ static int walk_taped(csg_ctx *c, int precision, double G, double softening, dou
         k_set_i32<<<1, 1, 0, c->stream>>>(c->tape_ovf, 0);
         fill_i32(c->tcount, (size_t)nw, 0, c->stream);
         TapeOut t{c->tape, c->tcount, c->tape_cap, c->tape_ovf};
-        if ((s = launch_walk(c, precision, G, softening, theta, t))) return s;
+        if ((s = launch_walk(c, precision, G, softening, theta, t, counts))) return s;
         int ovf = 0;
         CSG_CHECK(hipMemcpyAsync(&ovf, c->tape_ovf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         CSG_CHECK(hipStreamSynchronize(c->stream));
@@ -835,7 +838,7 @@ int csg_set_tape(csg_ctx *c, int enabled) {
 // rounding, and a recorded run must follow the same forces as the substeps, the
 // checkpoint recompute and the time reversal).
 static int walk_taped_traj(csg_ctx *c, int precision, double G, double softening, double theta) {
-    int s = walk_taped(c, precision, G, softening, theta);
+    int s = walk_taped(c, precision, G, softening, theta, false);
     if (s || precision != COSMOSIM_PREC_FP32 || !c->hilbert_waves) return s;
     return launch_walk(c, precision, G, softening, theta);
 }
@@ -925,7 +928,7 @@ int csg_record(csg_ctx *c, int k, double dt, double G, double softening, double 
                 (s = walk_taped_traj(c, precision, G, softening, theta)) || (s = launch_kick(c, dt)) ||
                 (s = frozen_store(c, step + 1)))
                 return s;
-        } else if ((s = csg_substeps(c, 1, dt, G, softening, theta, precision))) {
+        } else if ((s = substeps(c, 1, dt, G, softening, theta, precision, false))) {
             return s;
         }
     }
@@ -985,7 +988,7 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
         // forward's FP32 walk in Peano-Hilbert order: equal to FP32 rounding only)
         double *const kx = c->ax, *const ky = c->ay, *const kz = c->az;
         c->ax = c->gtmp; c->ay = c->gtmp + n; c->az = c->gtmp + 2 * (size_t)n;
-        r = walk_taped(c, prec, G, eps, th);
+        r = walk_taped(c, prec, G, eps, th, false);
         c->ax = kx; c->ay = ky; c->az = kz;
         if (r) return r;
         if ((r = force_vjp_dev(c, c->ab, c->xb, 1))) return r;
@@ -1000,7 +1003,7 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
             const int s0 = seg * every, s1 = (s0 + every < k) ? s0 + every : k;
             if ((s = state_copy(c, c->ckpt + slot * seg, 0))) return s;
             for (int step = s0; step < s1; ++step) {
-                if ((s = csg_substeps(c, 1, dt, G, eps, th, prec))) return s;
+                if ((s = substeps(c, 1, dt, G, eps, th, prec, false))) return s;
                 double *dstp = c->seg + 3 * (size_t)n * (step - s0);
                 CSG_CHECK(hipMemcpyAsync(dstp, c->x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
                 CSG_CHECK(hipMemcpyAsync(dstp + n, c->y, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
@@ -1041,7 +1044,7 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
     }
     // a_0 = A(x_0): x0bar += J^T abar_0 at the initial positions
     if ((s = launch_build(c))) return s;
-    if ((s = walk_taped(c, prec, G, eps, th))) return s;
+    if ((s = walk_taped(c, prec, G, eps, th, false))) return s;
     if ((s = force_vjp_dev(c, c->ab, c->xb, 1))) return s;
     if ((s = download3(c, x0bar, c->xb))) return s;
     if ((s = download3(c, v0bar, c->vb))) return s;

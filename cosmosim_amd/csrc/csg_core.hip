@@ -950,9 +950,19 @@ int csg_walk(csg_ctx *c, int precision, double G, double softening, double theta
     return COSMOSIM_OK;
 }
 
+// total: keep the interaction total of the last walk (the statistics report it);
+// the adjoint's internal one-substep calls skip it.
+static int substeps(csg_ctx *c, int k, double dt, double G, double softening, double theta,
+                    int precision, bool total = true);
+
 int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, double theta,
                  int precision) {
     CSG_USE(c);
+    return substeps(c, k, dt, G, softening, theta, precision);
+}
+
+static int substeps(csg_ctx *c, int k, double dt, double G, double softening, double theta,
+                    int precision, bool total) {
     c->theta_hint = theta;
     if (precision < 0 || precision > 2 || k < 0) return COSMOSIM_E_ARG;
     if (k == 0) return COSMOSIM_OK;
@@ -965,7 +975,8 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
         if ((s = launch_build(c, precision != COSMOSIM_PREC_FP32, bbox))) return s;
         timer_stop(c, PH_BUILD);
         timer_start(c);
-        if ((s = launch_walk(c, precision, G, softening, theta, TapeOut{}, false, sub + 1 == k)))
+        if ((s = launch_walk(c, precision, G, softening, theta, TapeOut{}, false,
+                             total && sub + 1 == k)))
             return s;
         timer_stop(c, PH_WALK);
         timer_start(c);
