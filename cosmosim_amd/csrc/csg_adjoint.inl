@@ -839,7 +839,9 @@ int csg_set_tape(csg_ctx *c, int enabled) {
 // checkpoint recompute and the time reversal).
 static int walk_taped_traj(csg_ctx *c, int precision, double G, double softening, double theta) {
     int s = walk_taped(c, precision, G, softening, theta, false);
-    if (s || precision != COSMOSIM_PREC_FP32 || !c->hilbert_waves) return s;
+    if (s || precision != COSMOSIM_PREC_FP32 || !c->widx_valid ||
+        blocks_for(c->n, 64) <= c->resident_waves)
+        return s;
     return launch_walk(c, precision, G, softening, theta);
 }
 
@@ -917,14 +919,16 @@ int csg_record(csg_ctx *c, int k, double dt, double G, double softening, double 
         c->frozen = new (std::nothrow) FrozenStep[k + 1];
         if (!c->frozen) return COSMOSIM_E_ALLOC;
         c->n_frozen = k + 1;
-        if ((s = launch_build(c)) || (s = walk_taped_traj(c, precision, G, softening, theta)) ||
+        if ((s = launch_build(c, true, 0, hilbert_order_for(c, precision))) ||
+            (s = walk_taped_traj(c, precision, G, softening, theta)) ||
             (s = frozen_store(c, 0)))
             return s;
     }
     for (int step = 0; step < k; ++step) {
         if (step % every == 0 && (s = state_copy(c, c->ckpt + slot * (step / every), 1))) return s;
         if (keep_trees) {
-            if ((s = launch_kick_drift(c, dt, 1)) || (s = launch_build(c)) ||
+            if ((s = launch_kick_drift(c, dt, 1)) ||
+                (s = launch_build(c, true, 0, hilbert_order_for(c, precision))) ||
                 (s = walk_taped_traj(c, precision, G, softening, theta)) || (s = launch_kick(c, dt)) ||
                 (s = frozen_store(c, step + 1)))
                 return s;
@@ -981,7 +985,7 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
     auto reverse_substep = [&]() -> int {
         int r;
         k_adj_kick<<<g, CSG_TPB, 0, st>>>(c->m, n, dt, 0, c->xb, c->vb, c->ab);
-        if ((r = launch_build(c))) return r;
+        if ((r = launch_build(c, true, 0, 0))) return r;   // the taped walk: sorted order
         // the tape is what this walk is for: its accelerations go to spare arrays and
         // the state keeps the forward pass's, which the time reversal's un-kick uses
         // (the taped walk groups bodies into wavefronts in sorted order, the
@@ -1025,7 +1029,7 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
             // reconstruct the state of substep `step`
             k_unstep<<<g, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay,
                                             c->az, c->m, n, dt, 0);
-            if ((s = launch_build(c))) return s;
+            if ((s = launch_build(c, true, 0, hilbert_order_for(c, prec)))) return s;
             if ((s = launch_walk(c, prec, G, eps, th))) return s;
             k_unstep<<<g, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay,
                                             c->az, c->m, n, dt, 1);
@@ -1043,7 +1047,7 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
         memcpy(&xs_dev, &bits, sizeof(double));
     }
     // a_0 = A(x_0): x0bar += J^T abar_0 at the initial positions
-    if ((s = launch_build(c))) return s;
+    if ((s = launch_build(c, true, 0, 0))) return s;
     if ((s = walk_taped(c, prec, G, eps, th, false))) return s;
     if ((s = force_vjp_dev(c, c->ab, c->xb, 1))) return s;
     if ((s = download3(c, x0bar, c->xb))) return s;

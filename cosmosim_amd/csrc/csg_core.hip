@@ -97,7 +97,17 @@ struct csg_ctx {
     // 3.71 grouped; strict 5.42 vs 4.45)
     int walk_group = -1;
     int walk_group32 = 0;   // FP32 walk: 0 = wave-mask traversal, else lanes per group
-    int hilbert_waves = 1;  // untaped FP32 wave-mask walks take wavefronts in Hilbert order
+    // Untaped walks take their bodies in Peano-Hilbert order (hilbert_waves) when it
+    // pays: the grouped FP64 walks from hilbert_min bodies (below, the three launches
+    // of the order cost more than the shorter traversals save), the wave-mask walks
+    // only with more wavefronts than the GPU holds at once (resident_waves): when
+    // every wavefront is resident the longest one sets the walk's time, and Hilbert
+    // order can make the longest one longer (a group of bodies spread through a
+    // sparse region); beyond that the total of the traversals counts, which it cuts.
+    int hilbert_waves = 1;
+    int hilbert_min = 16384;
+    int resident_waves = 8192;   // compute units x 4 SIMDs x 8 wavefronts
+    int widx_valid = 0;          // widx holds the current tree's Peano-Hilbert order
     // the kick-drift before the next walk already wrote its dispatch order (for these
     // waves per block, slots per XCD and waves), zeroed the dead bodies'
     // accelerations and the interaction total
@@ -363,6 +373,7 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     if (const char *g = getenv("COSMOSIM_XCD_CHUNK")) c->xcd_chunk = atoi(g);
     if (const char *g = getenv("COSMOSIM_WAVE_ORDER")) c->wave_order = atoi(g) != 0;
     if (const char *g = getenv("COSMOSIM_HILBERT_WAVES")) c->hilbert_waves = atoi(g) != 0;
+    if (const char *g = getenv("COSMOSIM_HILBERT_MIN_BODIES")) c->hilbert_min = atoi(g);
     if (const char *g = getenv("COSMOSIM_WAVE_SPLIT")) {
         const int v = atoi(g);
         if (v >= 1 && v <= WO_MAX_SPLIT) c->wave_split = v;
@@ -390,6 +401,12 @@ int csg_create(int device, int capacity, csg_ctx **out) {
         return COSMOSIM_E_ALLOC;
     }
     memset(c->hbp, 0, sizeof(BuildParams));
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+        c->resident_waves = 32 * cus;
+    (void)hipGetLastError();
+    if (const char *g = getenv("COSMOSIM_HILBERT_MIN_WAVES")) c->resident_waves = atoi(g);
     // the build counters start at zero (k_bbox_final counts earlier failures)
     clear_words((int *)c->bp, sizeof(BuildParams) / sizeof(int), c->stream);
     int s = alloc_buffers(c, capacity);
@@ -579,13 +596,26 @@ int csg_set_state(csg_ctx *c, const double *pos, const double *vel, const double
     return COSMOSIM_OK;
 }
 
-static int launch_build(csg_ctx *c, bool with64 = true, int bbox = 0);
+static int launch_build(csg_ctx *c, bool with64 = true, int bbox = 0, int order = -1);
+
+// Whether the untaped walk at `precision` over the current bodies takes the
+// Peano-Hilbert wavefront order (so the build before it computes the order): the
+// grouped FP64 walks from hilbert_min bodies, the wave-mask walks only with more
+// wavefronts than the GPU holds at once; the grouped FP32 walk keeps sorted order.
+static int hilbert_order_for(const csg_ctx *c, int precision) {
+    if (!c->hilbert_waves || c->n < c->hilbert_min) return 0;
+    if (precision == COSMOSIM_PREC_FP32 && c->walk_group32 != 0) return 0;
+    const int g64 = c->walk_group >= 0 ? c->walk_group
+                                       : (precision == COSMOSIM_PREC_FP64_STRICT ? 8 : 0);
+    if (precision != COSMOSIM_PREC_FP32 && g64 != 0) return 1;
+    return blocks_for(c->n, 64) > c->resident_waves ? 1 : 0;
+}
 
 // The FP64 node records of the current tree, re-emitted by a full build when the
 // last build wrote only the FP32 records (same positions, so the same tree).
 static int ensure_nodes64(csg_ctx *c, int precision) {
     if (precision == COSMOSIM_PREC_FP32 || c->nodes64_valid) return COSMOSIM_OK;
-    return launch_build(c, true);
+    return launch_build(c, true, 0, hilbert_order_for(c, precision));
 }
 
 // Blocks of the bounding-box partials (k_bbox_partial's grid).
@@ -656,7 +686,10 @@ static int launch_kick(csg_ctx *c, double dt) {
 // tree export read them; the FP32 walk reads only the 40-byte records).
 // bbox: the block count of bounding-box partials of the current positions that
 // the preceding kick-drift left in c->bbox_part (0: compute them here).
-static int launch_build(csg_ctx *c, bool with64, int bbox) {
+// order: 1 / 0 compute the Peano-Hilbert wavefront order or not (hilbert_order_for
+// of the walk that follows); -1 (a build with no particular walk in view): from
+// hilbert_min bodies.
+static int launch_build(csg_ctx *c, bool with64, int bbox, int order) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const int nbb = bbox > 0 ? bbox : bbox_blocks(n);
@@ -713,11 +746,15 @@ static int launch_build(csg_ctx *c, bool with64, int bbox) {
         c->ys, c->zs, c->ms, c->bp, with64 ? c->nodes : nullptr, c->nodes32, inv_th2, c->nlast);
     // the Peano-Hilbert wavefront order: the child-boundary steps go to the tile
     // prefixes' buffer, which k_dd_down2 is done with
-    int *const hbnd = (int *)c->tpref;
-    k_hilbert<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(c->lcp, c->nbase, c->nlast,
-                                                                   c->key2, c->bp, c->cnt, hbnd);
-    k_hsum<<<nt, CSG_TPB, 0, st>>>(c->cnt, hbnd, c->lcp, c->bp, c->scan_part);
-    k_hrank<<<nt, CSG_TPB, 0, st>>>(c->cnt, hbnd, c->lcp, c->scan_part, c->idx, c->bp, c->widx);
+    c->widx_valid = order >= 0 ? order : (c->hilbert_waves && n >= c->hilbert_min);
+    if (c->widx_valid) {
+        int *const hbnd = (int *)c->tpref;
+        k_hilbert<<<at_least1(blocks_for(n, CSG_TPB)), CSG_TPB, 0, st>>>(
+            c->lcp, c->nbase, c->nlast, c->key2, c->bp, c->cnt, hbnd);
+        k_hsum<<<nt, CSG_TPB, 0, st>>>(c->cnt, hbnd, c->lcp, c->bp, c->scan_part);
+        k_hrank<<<nt, CSG_TPB, 0, st>>>(c->cnt, hbnd, c->lcp, c->scan_part, c->idx, c->bp,
+                                        c->widx);
+    }
     CSG_LAUNCHED();
     c->tree_valid = 1;
     c->geom_valid = 0;
@@ -781,7 +818,7 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     c->theta_hint = theta;
     if (precision == COSMOSIM_PREC_FP32 && c->tree_valid && c->th32 != theta) {
         // the FP32 records carry the opening angle: emit them again for this one
-        const int s = launch_build(c, c->nodes64_valid != 0);
+        const int s = launch_build(c, c->nodes64_valid != 0, 0, hilbert_order_for(c, precision));
         if (s) return s;
     }
     const bool pre = c->order_ready != 0;   // the kick-drift before did the order and zeroing
@@ -820,7 +857,8 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         // wave-mask traversal (also the taped FP32 walk: the tape is per wavefront)
         const bool off32 = (size_t)c->node_cap * sizeof(WNode32) < ((size_t)1 << 32);
         const bool taping = tape.rec != nullptr;
-        const int *sidx = (hilbert && !taping && c->hilbert_waves) ? c->widx : c->idx;
+        const int *sidx =
+            (hilbert && !taping && c->widx_valid && nw > c->resident_waves) ? c->widx : c->idx;
         const int v = (lane_counts ? 4 : 0) | (taping ? 2 : 0) | (off32 ? 1 : 0);
         if (v < 2 && !count_total) {
             // softened: the own leaf's offset is exactly 0 and its factor
@@ -851,7 +889,8 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     c->counts_valid = 1;
     // the untaped FP64 walks take their bodies in Peano-Hilbert order (results do not
     // depend on the grouping); a taped walk keeps sorted order, like the FP32 one
-    const int *sidx64 = (hilbert && !tape.rec && c->hilbert_waves) ? c->widx : c->idx;
+    const int *sidx64 = (hilbert && !tape.rec && c->widx_valid) ? c->widx : c->idx;
+    const int *sidx64w = nw > c->resident_waves ? sidx64 : c->idx;   // the wave-mask walk
     // FP64: the wave-mask walk (also for tapes, which are per wavefront) when the
     // records span < 4 GiB, else the grouped walk
     const int g64 = c->walk_group >= 0 ? c->walk_group
@@ -861,10 +900,10 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         const bool taping = tape.rec != nullptr;
         if (precision == COSMOSIM_PREC_FP64_STRICT) {
             if (taping) launch_walk64w<WALK_FP64_STRICT, true>(c, c->idx, G, eps2, th2, grid, nb8, tape, wo, tpb);
-            else launch_walk64w<WALK_FP64_STRICT, false>(c, sidx64, G, eps2, th2, grid, nb8, tape, wo, tpb);
+            else launch_walk64w<WALK_FP64_STRICT, false>(c, sidx64w, G, eps2, th2, grid, nb8, tape, wo, tpb);
         } else {
             if (taping) launch_walk64w<WALK_FP64, true>(c, c->idx, G, eps2, th2, grid, nb8, tape, wo, tpb);
-            else launch_walk64w<WALK_FP64, false>(c, sidx64, G, eps2, th2, grid, nb8, tape, wo, tpb);
+            else launch_walk64w<WALK_FP64, false>(c, sidx64w, G, eps2, th2, grid, nb8, tape, wo, tpb);
         }
         CSG_LAUNCHED();
         return COSMOSIM_OK;
@@ -972,7 +1011,9 @@ static int substeps(csg_ctx *c, int k, double dt, double G, double softening, do
     timer_stop(c, PH_KDK);
     for (int sub = 0; sub < k; ++sub) {
         timer_start(c);
-        if ((s = launch_build(c, precision != COSMOSIM_PREC_FP32, bbox))) return s;
+        if ((s = launch_build(c, precision != COSMOSIM_PREC_FP32, bbox,
+                              hilbert_order_for(c, precision))))
+            return s;
         timer_stop(c, PH_BUILD);
         timer_start(c);
         if ((s = launch_walk(c, precision, G, softening, theta, TapeOut{}, false,
@@ -1202,7 +1243,8 @@ int csg_debug_orders(csg_ctx *c, int *sorted_idx, int *wave_idx, unsigned long l
     if (sorted_idx)
         CSG_CHECK(hipMemcpyAsync(sorted_idx, c->idx, sizeof(int) * nl, hipMemcpyDeviceToHost, c->stream));
     if (wave_idx)
-        CSG_CHECK(hipMemcpyAsync(wave_idx, c->widx, sizeof(int) * nl, hipMemcpyDeviceToHost, c->stream));
+        CSG_CHECK(hipMemcpyAsync(wave_idx, c->widx_valid ? c->widx : c->idx, sizeof(int) * nl,
+                                 hipMemcpyDeviceToHost, c->stream));
     if (keys)
         CSG_CHECK(hipMemcpyAsync(keys, c->key, sizeof(uint64_t) * nl, hipMemcpyDeviceToHost, c->stream));
     CSG_CHECK(hipStreamSynchronize(c->stream));
@@ -1352,7 +1394,8 @@ int cosmosim_ext_forces_noalloc(const double *x, const double *y, const double *
     clear_words(c.widx, (size_t)c.cap, c.stream);
     CSG_LAUNCHED();
     c.theta_hint = theta;
-    if ((s = launch_build(&c, precision != COSMOSIM_PREC_FP32))) return s;
+    if ((s = launch_build(&c, precision != COSMOSIM_PREC_FP32, 0, hilbert_order_for(&c, precision))))
+        return s;
     if ((s = launch_walk(&c, precision, G, softening, theta))) return s;
     k_noalloc_status<<<1, 1, 0, c.stream>>>(c.bp, status);
     CSG_LAUNCHED();

@@ -388,7 +388,7 @@ int csg_substeps_sph(csg_ctx *c, int k, double dt, double G, double softening, d
     timer_stop(c, PH_KDK);
     for (int sub = 0; sub < k; ++sub) {
         timer_start(c);
-        if ((s = launch_build(c, true))) return s;
+        if ((s = launch_build(c, true, 0, hilbert_order_for(c, precision)))) return s;
         timer_stop(c, PH_BUILD);
         timer_start(c);
         if ((s = launch_walk(c, precision, G, softening, theta, TapeOut{}, false, sub + 1 == k)))

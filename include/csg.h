@@ -127,8 +127,12 @@ int  csg_synchronize(csg_ctx *ctx);
  * so it fails as a tree that does not fit would; 0 cancels. */
 int  csg_debug_fail_build(csg_ctx *ctx, int nth);
 /* Test hook: the last tree's body orders, n_live entries each (any output may be
- * NULL): the sorted order (body indices by octree key), the FP32 walk's
- * wavefront order (the same bodies in Peano-Hilbert order of the cells) and the
+ * NULL): the sorted order (body indices by octree key), the wavefront order the
+ * last build computed for the untaped walks (the same bodies in Peano-Hilbert order
+ * of the cells; the sorted order when the build computed none: fewer than 16384
+ * bodies, or an FP32 substep build with no more wavefronts than the GPU holds at
+ * once; the environment variables COSMOSIM_HILBERT_MIN_BODIES and
+ * COSMOSIM_HILBERT_MIN_WAVES at context creation change both thresholds) and the
  * sorted 48-bit keys (16 levels of octant digits, the first level in bits 47..45).
  * COSMOSIM_E_CAPACITY when n < n_live (*n_live is set first). */
 int  csg_debug_orders(csg_ctx *ctx, int *sorted_idx, int *wave_idx, unsigned long long *keys,

@@ -13,7 +13,12 @@ depth-64 cell and tiny counts:
   * the FP32 forces with Hilbert wavefronts equal those with sorted-order
     wavefronts to FP32 rounding, and the per-body interaction counts are equal:
     the grouping changes only which bodies share a wavefront;
-  * the FP64 walks' forces do not depend on the grouping at all (bit-identical).
+  * the FP64 walks' forces do not depend on the grouping at all (bit-identical);
+  * trees of fewer than 16384 bodies keep sorted-order wavefronts.
+
+The library uses the order in the wave-mask walks only when there are more
+wavefronts than the GPU holds at once; the tests lower that threshold (and the body
+threshold) through the environment so that every input exercises the ordered walks.
 """
 import os
 
@@ -101,16 +106,24 @@ CASES = {
 }
 
 
-def _sim(b, hilbert=True):
-    old = os.environ.get("COSMOSIM_HILBERT_WAVES")
-    os.environ["COSMOSIM_HILBERT_WAVES"] = "1" if hilbert else "0"
+def _sim(b, hilbert=True, min_bodies="1", min_waves="0"):
+    """A handle whose builds compute the Peano-Hilbert order from min_bodies bodies and
+    whose untaped walks use it (hilbert), the wave-mask walks from min_waves wavefronts
+    (the defaults here: every input; the library's own defaults are 16384 bodies and
+    the GPU's resident wavefronts)."""
+    env = {"COSMOSIM_HILBERT_WAVES": "1" if hilbert else "0",
+           "COSMOSIM_HILBERT_MIN_BODIES": min_bodies,
+           "COSMOSIM_HILBERT_MIN_WAVES": min_waves}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         return ca.Simulation(b, softening=0.01, theta=0.5, dt=1e-3, precision="fp32")
     finally:
-        if old is None:
-            del os.environ["COSMOSIM_HILBERT_WAVES"]
-        else:
-            os.environ["COSMOSIM_HILBERT_WAVES"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -163,3 +176,25 @@ def test_fp64_walks_do_not_depend_on_the_grouping(precision):
         assert np.array_equal(s1.interaction_counts(), s0.interaction_counts())
         s1.close()
         s0.close()
+
+
+def test_small_trees_keep_sorted_wavefronts():
+    """With the library's thresholds: a build of fewer than 16384 bodies skips the
+    order (the walks take sorted-order wavefronts); from 16384 bodies a build for the
+    FP64 walks computes it."""
+    for n, hil in ((4096, False), (16384, True)):
+        b = uniform(n, 7)
+        old = {k: os.environ.pop(k, None) for k in
+               ("COSMOSIM_HILBERT_WAVES", "COSMOSIM_HILBERT_MIN_BODIES", "COSMOSIM_HILBERT_MIN_WAVES")}
+        try:
+            sim = ca.Simulation(b, softening=0.01, theta=0.5, dt=1e-3, precision="fp64")
+        finally:
+            os.environ.update({k: v for k, v in old.items() if v is not None})
+        sim.compute_forces()
+        si, wi, keys = sim.debug_orders()
+        if hil:
+            assert np.array_equal(wi, si[np.argsort(hilbert_of_keys(keys), kind="stable")])
+            assert not np.array_equal(wi, si)
+        else:
+            assert np.array_equal(wi, si)
+        sim.close()
