@@ -168,7 +168,7 @@ def _adjoint_problem(ca, n, seed, steps, eps, th, dt, prec):
     return b, target
 
 
-def adjoint_runs(ca, big_n, big_steps, everies=(10, 25, 50)):
+def adjoint_runs(ca, big_n, big_steps, everies=(10, 25, 50), configs=(3, 4)):
     """Adjoint throughput.  Config 3: 8,192-body merger (seed 4), 50 substeps,
     dt 1e-3, eps 0.02, theta 0.5, FP64 walk, checkpoints every 10.  Config 4:
     big_n-body merger (seed 5), FP32 walk, theta 0.7, eps 0.003, dt 5e-4,
@@ -176,10 +176,12 @@ def adjoint_runs(ca, big_n, big_steps, everies=(10, 25, 50)):
     (segments recomputed forward) for each K, and the time-reversed leapfrog.
     Loss: mean squared distance to the target snapshot."""
     out = {}
-    for name, n, seed, steps, eps, th, dt, prec, ks in [
-            ("config3_8192_fp64_theta0.5", 8192, 4, 50, 0.02, 0.5, 1e-3, "fp64", (10,)),
-            (f"config4_{big_n}_fp32_theta0.7", big_n, 5, big_steps, 0.003, 0.7, 5e-4, "fp32",
+    for cfg, name, n, seed, steps, eps, th, dt, prec, ks in [
+            (3, "config3_8192_fp64_theta0.5", 8192, 4, 50, 0.02, 0.5, 1e-3, "fp64", (10,)),
+            (4, f"config4_{big_n}_fp32_theta0.7", big_n, 5, big_steps, 0.003, 0.7, 5e-4, "fp32",
              everies)]:
+        if cfg not in configs:
+            continue
         b, target = _adjoint_problem(ca, n, seed, steps, eps, th, dt, prec)
         r = {"bodies": n, "substeps": steps}
         grads = {}
@@ -402,6 +404,11 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-bodies", type=int, default=4_000_000)
     ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-everies", default="10,25,50",
+                    help="checkpoint intervals K of the large adjoint configuration")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the large adjoint configuration (for profiling): prints "
+                         "its figures as one JSON object")
     ap.add_argument("--no-neighbors", action="store_true")
     ap.add_argument("--no-per-step", action="store_true",
                     help="skip the per-substep subset accuracy runs")
@@ -422,6 +429,11 @@ def main():
         copy_calibration()
         return
     import cosmosim_amd as ca
+    everies = tuple(int(k) for k in args.adjoint_everies.split(","))
+    if args.adjoint_only:
+        adj = adjoint_runs(ca, args.adjoint_bodies, args.adjoint_steps, everies, configs=(4,))
+        print(json.dumps({"adjoint": _rounded(adj)}))
+        return
     dev_name, cus = ca.device_info(0)
     bodies = ca.generate(ca.IC_MERGER, N_BODIES, SEED)
     rng = np.random.default_rng(12345)
@@ -516,10 +528,10 @@ def main():
     acc = results["theta0.5_accuracy_step0"]
     line["accuracy_theta0.5"] = {k: float(f"{v:.3e}") for k, v in acc.items()}
     if not args.no_adjoint:
-        adj = adjoint_runs(ca, args.adjoint_bodies, args.adjoint_steps)
+        adj = adjoint_runs(ca, args.adjoint_bodies, args.adjoint_steps, everies)
         line["adjoint"] = _rounded(adj)
         big = adj[f"config4_{args.adjoint_bodies}_fp32_theta0.7"]
-        line["adjoint_substeps_per_s"] = round(big["K10"]["adjoint_substeps_per_s"], 3)
+        line["adjoint_substeps_per_s"] = round(big[f"K{everies[0]}"]["adjoint_substeps_per_s"], 3)
     if not args.no_neighbors:
         line["neighbors"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
                              for k, v in neighbor_runs(ca, bodies).items()}

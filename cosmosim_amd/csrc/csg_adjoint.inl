@@ -61,8 +61,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_tape_scan(const int *__restrict__ c
     if (threadIdx.x == 0) *total = carry;
 }
 
-// Step 1: per-lane -T w and per-record lane sums (one wave per 64 sorted bodies,
-// one wave per workgroup).  The wave index is uniform, so tape records and node
+// Step 1: per-lane -T w and per-record lane sums (one wave per tape wavefront,
+// one wave per workgroup).  Lane l of tape wavefront w holds sorted body
+// 64 w + l, or wpos[64 w + l] for a tape recorded in Peano-Hilbert order.  The wave index is uniform, so tape records and node
 // records come by scalar loads, four records per round (their latencies
 // overlap).  The lane sums of a round go through LDS, two records (six sums) at
 // a time: every lane stores its six values, lane 8 s + p adds the values of
@@ -76,16 +77,16 @@ __global__ __launch_bounds__(ADJ_WAVE_TPB) void k_adj_walk(
     const int *__restrict__ tcount, const int *__restrict__ toff, int cap,
     const BuildParams *__restrict__ bp, const double *__restrict__ xs,
     const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ws,
-    double G, double eps2, double *__restrict__ xbar_s, uint64_t *__restrict__ rkey,
-    int *__restrict__ rval, double *__restrict__ rsum) {
+    const int *__restrict__ wpos, double G, double eps2, double *__restrict__ xbar_s,
+    uint64_t *__restrict__ rkey, int *__restrict__ rval, double *__restrict__ rsum) {
     __shared__ double sums[6][72];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
     const int nl = bp->n_live;
     const int b0 = wave * 64;
     if (b0 >= nl) return;   // the whole workgroup (one wave) leaves together
-    const int b = b0 + lane;
-    const bool act = b < nl;
+    const bool act = b0 + lane < nl;
+    const int b = act ? (wpos ? wpos[b0 + lane] : b0 + lane) : 0;   // the lane's sorted body
     const double x = act ? xs[b] : 0.0, y = act ? ys[b] : 0.0, z = act ? zs[b] : 0.0;
     const double wx = act ? ws[3 * (size_t)b] : 0.0, wy = act ? ws[3 * (size_t)b + 1] : 0.0,
                  wz = act ? ws[3 * (size_t)b + 2] : 0.0;
@@ -410,16 +411,17 @@ __global__ __launch_bounds__(CSG_TPB) void k_refresh_nodes(const WNode *__restri
 __global__ __launch_bounds__(CSG_TPB) void k_tape_forces(
     const WNode *__restrict__ nodes, const TapeRec *__restrict__ tape,
     const int *__restrict__ tcount, int cap, const BuildParams *__restrict__ bp,
-    const int *__restrict__ idx, const double *__restrict__ xs, const double *__restrict__ ys,
-    const double *__restrict__ zs, double G, double eps2, double *__restrict__ ax,
-    double *__restrict__ ay, double *__restrict__ az) {
+    const int *__restrict__ idx, const int *__restrict__ wpos, const double *__restrict__ xs,
+    const double *__restrict__ ys, const double *__restrict__ zs, double G, double eps2,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az) {
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * CSG_TPB + threadIdx.x) >> 6;
     const int nl = bp->n_live;
     const int b0 = wave * 64;
     if (b0 >= nl) return;
-    const int b = b0 + lane;
-    const bool act = b < nl;
+    const bool act = b0 + lane < nl;
+    // the lane's sorted body (through wpos for a tape in Peano-Hilbert order)
+    const int b = act ? (wpos ? wpos[b0 + lane] : b0 + lane) : 0;
     const double x = act ? xs[b] : 0.0, y = act ? ys[b] : 0.0, z = act ? zs[b] : 0.0;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const int cnt = tcount[wave];
@@ -649,7 +651,8 @@ static int force_vjp_dev(csg_ctx *c, const double *w_dev, double *xbar_dev, int 
     if ((s = adj_reserve_records(c, (size_t)R + 1))) return s;
     k_adj_walk<<<nw, ADJ_WAVE_TPB, 0, st>>>(
         c->nodes, c->tape, c->tcount, c->toff, c->tape_cap, c->bp, c->xs, c->ys, c->zs, c->ws,
-        c->tape_G, c->tape_eps2, c->xbar_s, c->rkey, c->rval, c->rsum);
+        c->tape_wmap ? c->hpos : nullptr, c->tape_G, c->tape_eps2, c->xbar_s, c->rkey, c->rval,
+        c->rsum);
     CSG_LAUNCHED();
     // stable radix sort of the records by node id: the transposed lists
     int bits = 1;
@@ -715,8 +718,9 @@ static int forces_frozen_dev(csg_ctx *c) {
                                                                 c->nodes_alt);
     const int nw = blocks_for(nl, 64);
     k_tape_forces<<<blocks_for((long long)nw * 64, CSG_TPB), CSG_TPB, 0, st>>>(
-        c->nodes_alt, c->tape, c->tcount, c->tape_cap, c->bp, c->idx, c->xs, c->ys, c->zs,
-        c->tape_G, c->tape_eps2, c->ax, c->ay, c->az);
+        c->nodes_alt, c->tape, c->tcount, c->tape_cap, c->bp, c->idx,
+        c->tape_wmap ? c->hpos : nullptr, c->xs, c->ys, c->zs, c->tape_G, c->tape_eps2, c->ax,
+        c->ay, c->az);
     CSG_LAUNCHED();
     return COSMOSIM_OK;
 }
@@ -749,9 +753,11 @@ static int state_copy(csg_ctx *c, double *dst, int to_slot) {
 }
 
 // Frozen structures of one force evaluation of a recorded run: sorted order,
-// build parameters, node records and tape.
+// build parameters, node records, tape and (for a tape in Peano-Hilbert order)
+// its wavefronts' sorted bodies.
 struct FrozenStep {
     int *idx = nullptr;
+    int *wpos = nullptr;
     BuildParams *bp = nullptr;
     WNode *nodes = nullptr;
     int *tcount = nullptr;
@@ -763,7 +769,7 @@ static void frozen_free(csg_ctx *c) {
     if (!c->frozen) return;
     for (int e = 0; e < c->n_frozen; ++e) {
         FrozenStep &f = c->frozen[e];
-        dfree(f.idx); dfree(f.bp); dfree(f.nodes); dfree(f.tcount); dfree(f.tape);
+        dfree(f.idx); dfree(f.wpos); dfree(f.bp); dfree(f.nodes); dfree(f.tcount); dfree(f.tape);
     }
     delete[] c->frozen;
     c->frozen = nullptr;
@@ -778,13 +784,16 @@ static int frozen_store(csg_ctx *c, int e) {
     const int n = c->n > 0 ? c->n : 1, nn = c->hbp->n_nodes;
     const int nw = at_least1(blocks_for(c->n, 64));
     if (!dalloc(f.idx, (size_t)n) || !dalloc(f.bp, 1) || !dalloc(f.nodes, (size_t)nn) ||
-        !dalloc(f.tcount, (size_t)nw) || !dalloc(f.tape, (size_t)nw * c->tape_cap)) {
+        !dalloc(f.tcount, (size_t)nw) || !dalloc(f.tape, (size_t)nw * c->tape_cap) ||
+        (c->tape_wmap && !dalloc(f.wpos, (size_t)n))) {
         (void)hipGetLastError();
         return COSMOSIM_E_ALLOC;
     }
     f.cap = c->tape_cap;
     hipStream_t st = c->stream;
     CSG_CHECK(hipMemcpyAsync(f.idx, c->idx, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+    if (f.wpos)
+        CSG_CHECK(hipMemcpyAsync(f.wpos, c->hpos, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
     CSG_CHECK(hipMemcpyAsync(f.bp, c->bp, sizeof(BuildParams), hipMemcpyDeviceToDevice, st));
     CSG_CHECK(hipMemcpyAsync(f.nodes, c->nodes, sizeof(WNode) * nn, hipMemcpyDeviceToDevice, st));
     CSG_CHECK(hipMemcpyAsync(f.tcount, c->tcount, sizeof(int) * nw, hipMemcpyDeviceToDevice, st));
@@ -817,8 +826,8 @@ static int frozen_forces(csg_ctx *c, int e) {
                                                                 c->nodes_alt);
     const int nw = blocks_for(nl, 64);
     k_tape_forces<<<blocks_for((long long)nw * 64, CSG_TPB), CSG_TPB, 0, st>>>(
-        c->nodes_alt, f.tape, f.tcount, f.cap, f.bp, f.idx, c->xs, c->ys, c->zs, c->rec_G,
-        c->rec_eps * c->rec_eps, c->ax, c->ay, c->az);
+        c->nodes_alt, f.tape, f.tcount, f.cap, f.bp, f.idx, f.wpos, c->xs, c->ys, c->zs,
+        c->rec_G, c->rec_eps * c->rec_eps, c->ax, c->ay, c->az);
     CSG_LAUNCHED();
     c->tree_valid = 0;  // the sorted copies now hold replay positions
     return COSMOSIM_OK;
@@ -832,17 +841,11 @@ int csg_set_tape(csg_ctx *c, int enabled) {
     return COSMOSIM_OK;
 }
 
-// A recording's taped walk: the tape, and the trajectory's forces as the untaped
-// walk computes them (the FP32 walk without a tape groups bodies into wavefronts
-// in Peano-Hilbert order, the taped one in sorted order; they agree only to FP32
-// rounding, and a recorded run must follow the same forces as the substeps, the
-// checkpoint recompute and the time reversal).
+// A recording's taped walk.  Its forces are the trajectory's: a taped walk
+// groups bodies into wavefronts as the untaped walk of the substeps does, so the
+// two give bit-identical accelerations.
 static int walk_taped_traj(csg_ctx *c, int precision, double G, double softening, double theta) {
-    int s = walk_taped(c, precision, G, softening, theta, false);
-    if (s || precision != COSMOSIM_PREC_FP32 || !c->widx_valid ||
-        blocks_for(c->n, 64) <= c->resident_waves)
-        return s;
-    return launch_walk(c, precision, G, softening, theta);
+    return walk_taped(c, precision, G, softening, theta, false);
 }
 
 int csg_walk_taped(csg_ctx *c, int precision, double G, double softening, double theta) {
@@ -981,20 +984,22 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
         }
         c->seg_slots = every;
     }
-    // one reverse substep with the positions x_{s+1} in the state arrays
-    auto reverse_substep = [&]() -> int {
+    // one reverse substep with the positions x_{s+1} in the state arrays; have_tape:
+    // the tree and tape of x_{s+1} are already there (the time reversal's walk at
+    // the same positions recorded them)
+    auto reverse_substep = [&](bool have_tape) -> int {
         int r;
         k_adj_kick<<<g, CSG_TPB, 0, st>>>(c->m, n, dt, 0, c->xb, c->vb, c->ab);
-        if ((r = launch_build(c, true, 0, 0))) return r;   // the taped walk: sorted order
-        // the tape is what this walk is for: its accelerations go to spare arrays and
-        // the state keeps the forward pass's, which the time reversal's un-kick uses
-        // (the taped walk groups bodies into wavefronts in sorted order, the
-        // forward's FP32 walk in Peano-Hilbert order: equal to FP32 rounding only)
-        double *const kx = c->ax, *const ky = c->ay, *const kz = c->az;
-        c->ax = c->gtmp; c->ay = c->gtmp + n; c->az = c->gtmp + 2 * (size_t)n;
-        r = walk_taped(c, prec, G, eps, th, false);
-        c->ax = kx; c->ay = ky; c->az = kz;
-        if (r) return r;
+        if (!have_tape) {
+            if ((r = launch_build(c, true, 0, hilbert_order_for(c, prec)))) return r;
+            // the tape is what this walk is for: its accelerations (the forward
+            // pass's, bit for bit) go to spare arrays and the state keeps its own
+            double *const kx = c->ax, *const ky = c->ay, *const kz = c->az;
+            c->ax = c->gtmp; c->ay = c->gtmp + n; c->az = c->gtmp + 2 * (size_t)n;
+            r = walk_taped(c, prec, G, eps, th, false);
+            c->ax = kx; c->ay = ky; c->az = kz;
+            if (r) return r;
+        }
         if ((r = force_vjp_dev(c, c->ab, c->xb, 1))) return r;
         k_adj_kick<<<g, CSG_TPB, 0, st>>>(c->m, n, dt, 1, c->xb, c->vb, c->ab);
         CSG_LAUNCHED();
@@ -1018,19 +1023,21 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
                 CSG_CHECK(hipMemcpyAsync(c->x, srcp, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
                 CSG_CHECK(hipMemcpyAsync(c->y, srcp + n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
                 CSG_CHECK(hipMemcpyAsync(c->z, srcp + 2 * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-                if ((s = reverse_substep())) return s;
+                if ((s = reverse_substep(false))) return s;
             }
         }
         if ((s = state_copy(c, c->ckpt, 0))) return s;   // x_0
     } else {
         if ((s = state_copy(c, c->ckpt + slot * (nck - 1), 0))) return s;  // final state
         for (int step = k - 1; step >= 0; --step) {
-            if ((s = reverse_substep())) return s;
-            // reconstruct the state of substep `step`
+            if ((s = reverse_substep(step < k - 1))) return s;
+            // reconstruct the state of substep `step`: its forces come from a taped
+            // walk (the forward's accelerations, bit for bit), whose tree and tape
+            // the next reverse substep (or the final VJP at x_0) uses
             k_unstep<<<g, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay,
                                             c->az, c->m, n, dt, 0);
             if ((s = launch_build(c, true, 0, hilbert_order_for(c, prec)))) return s;
-            if ((s = launch_walk(c, prec, G, eps, th))) return s;
+            if ((s = walk_taped(c, prec, G, eps, th, false))) return s;
             k_unstep<<<g, CSG_TPB, 0, st>>>(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay,
                                             c->az, c->m, n, dt, 1);
             CSG_LAUNCHED();
@@ -1046,9 +1053,16 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
         CSG_CHECK(hipStreamSynchronize(st));
         memcpy(&xs_dev, &bits, sizeof(double));
     }
-    // a_0 = A(x_0): x0bar += J^T abar_0 at the initial positions
-    if ((s = launch_build(c, true, 0, 0))) return s;
-    if ((s = walk_taped(c, prec, G, eps, th, false))) return s;
+    // a_0 = A(x_0): x0bar += J^T abar_0 at the initial positions (the time
+    // reversal's last walk taped x_0 already)
+    if (mode == 0 || k == 0) {
+        if ((s = launch_build(c, true, 0, hilbert_order_for(c, prec)))) return s;
+        double *const kx = c->ax, *const ky = c->ay, *const kz = c->az;
+        c->ax = c->gtmp; c->ay = c->gtmp + n; c->az = c->gtmp + 2 * (size_t)n;
+        s = walk_taped(c, prec, G, eps, th, false);
+        c->ax = kx; c->ay = ky; c->az = kz;
+        if (s) return s;
+    }
     if ((s = force_vjp_dev(c, c->ab, c->xb, 1))) return s;
     if ((s = download3(c, x0bar, c->xb))) return s;
     if ((s = download3(c, v0bar, c->vb))) return s;

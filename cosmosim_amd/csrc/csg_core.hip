@@ -33,6 +33,7 @@ struct csg_ctx {
     uint64_t *key = nullptr, *key2 = nullptr, *ext = nullptr, *ext_tmp = nullptr;
     int *idx = nullptr, *idx2 = nullptr;
     int *widx = nullptr;   // FP32 wave-mask walk: the sorted bodies in Peano-Hilbert order
+    int *hpos = nullptr;   // ... and their sorted positions (hpos[r]: widx[r] = idx[hpos[r]])
     int *rs_hist = nullptr, *rs_tot = nullptr;
     RsCtl *rs_ctl = nullptr;           // body sort: tickets and digit counts
     uint64_t *rs_status = nullptr;     // body sort: per-tile digit counts (look-back)
@@ -70,6 +71,7 @@ struct csg_ctx {
     int timed_walks = 0;
     // adjoint: walk tape, transposed records, per-body adjoint buffers, checkpoints
     int tape_on = 0, tape_cap = 0, tape_waves = 0, tape_valid = 0;
+    int tape_wmap = 0;   // the tape's wavefronts hold the bodies of hpos, not sorted runs
     double tape_G = 1.0, tape_eps2 = 0.0;
     TapeRec *tape = nullptr;
     int *tcount = nullptr, *toff = nullptr, *tape_total = nullptr, *tape_ovf = nullptr;
@@ -187,7 +189,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->vx); dfree(c->vy); dfree(c->vz);
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->m); dfree(c->aos);
     dfree(c->bbox_part); dfree(c->key); dfree(c->key2); dfree(c->ext); dfree(c->ext_tmp);
-    dfree(c->idx); dfree(c->idx2); dfree(c->widx); dfree(c->nlast); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
+    dfree(c->idx); dfree(c->idx2); dfree(c->widx); dfree(c->hpos); dfree(c->nlast); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
     dfree(c->lcp); dfree(c->lmin1); dfree(c->lmin2); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->tpref); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
     dfree(c->wcost); dfree(c->worder); dfree(c->xyzm);
@@ -229,7 +231,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->bbox_part, (size_t)CSG_BBOX_BLOCKS * 8) && dalloc(c->key, C) &&
               dalloc(c->key2, C) && dalloc(c->ext, 3 * C) && dalloc(c->ext_tmp, 3 * C) &&
               dalloc(c->idx, C) && dalloc(c->idx2, C) && dalloc(c->widx, C) &&
-              dalloc(c->rs_hist, (size_t)256 * nb) &&
+              dalloc(c->hpos, C) && dalloc(c->rs_hist, (size_t)256 * nb) &&
               dalloc(c->rs_tot, 256) && dalloc(c->xs, C) && dalloc(c->ys, C) && dalloc(c->zs, C) &&
               dalloc(c->ms, C) &&
               dalloc(c->lcp, C1 + 64) && dalloc(c->lmin1, lmin1_bytes(C)) &&
@@ -753,7 +755,7 @@ static int launch_build(csg_ctx *c, bool with64, int bbox, int order) {
             c->lcp, c->nbase, c->nlast, c->key2, c->bp, c->cnt, hbnd);
         k_hsum<<<nt, CSG_TPB, 0, st>>>(c->cnt, hbnd, c->lcp, c->bp, c->scan_part);
         k_hrank<<<nt, CSG_TPB, 0, st>>>(c->cnt, hbnd, c->lcp, c->scan_part, c->idx, c->bp,
-                                        c->widx);
+                                        c->widx, c->hpos);
     }
     CSG_LAUNCHED();
     c->tree_valid = 1;
@@ -806,9 +808,12 @@ static void launch_walk64w(csg_ctx *c, const int *sidx, double G, double eps2, d
 // lane_counts: record per-body interaction counts (c->icount); walks inside the
 // substep loops skip them.  count_total: keep the interaction total (the FP32
 // substep loop keeps it only in its last walk; the statistics report that one).
-// hilbert: an untaped FP32 wave-mask walk takes its wavefronts' bodies in
-// Peano-Hilbert order (c->widx); false keeps sorted order, as the taped walk
-// does (the time-reversed adjoint recomputes its forces that way).
+// hilbert: the FP32 wave-mask walk takes its wavefronts' bodies in Peano-Hilbert
+// order (c->widx) when the tree has one and there are more wavefronts than the GPU
+// holds at once; false keeps sorted order.  A taped walk follows the same rule,
+// so its forces are bit-identical to the untaped walk's (the recording, the
+// checkpoint recompute and the time reversal follow one trajectory), and its tape
+// then maps wavefront lanes to sorted bodies through c->hpos (c->tape_wmap).
 static int launch_walk(csg_ctx *c, int precision, double G, double softening, double theta,
                        TapeOut tape = TapeOut{nullptr, nullptr, 0, nullptr},
                        bool lane_counts = true, bool count_total = true, bool hilbert = true) {
@@ -857,8 +862,9 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         // wave-mask traversal (also the taped FP32 walk: the tape is per wavefront)
         const bool off32 = (size_t)c->node_cap * sizeof(WNode32) < ((size_t)1 << 32);
         const bool taping = tape.rec != nullptr;
-        const int *sidx =
-            (hilbert && !taping && c->widx_valid && nw > c->resident_waves) ? c->widx : c->idx;
+        const bool hil = hilbert && c->widx_valid && nw > c->resident_waves;
+        const int *sidx = hil ? c->widx : c->idx;
+        if (taping) c->tape_wmap = hil ? 1 : 0;
         const int v = (lane_counts ? 4 : 0) | (taping ? 2 : 0) | (off32 ? 1 : 0);
         if (v < 2 && !count_total) {
             // softened: the own leaf's offset is exactly 0 and its factor
@@ -887,8 +893,9 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         return COSMOSIM_OK;
     }
     c->counts_valid = 1;
+    if (tape.rec) c->tape_wmap = 0;
     // the untaped FP64 walks take their bodies in Peano-Hilbert order (results do not
-    // depend on the grouping); a taped walk keeps sorted order, like the FP32 one
+    // depend on the grouping); a taped walk keeps sorted order
     const int *sidx64 = (hilbert && !tape.rec && c->widx_valid) ? c->widx : c->idx;
     const int *sidx64w = nw > c->resident_waves ? sidx64 : c->idx;   // the wave-mask walk
     // FP64: the wave-mask walk (also for tapes, which are per wavefront) when the

@@ -1671,7 +1671,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_hsum(const int *__restrict__ hdiff,
 
 // Peano-Hilbert rank of every sorted body b < n_live: b plus the prefix sum of
 // its rank steps (the tile sums hpart of k_hsum before this tile, then a scan
-// inside it); the walk order gets widx[rank] = idx[b].  A rank outside [0, n_live) (a broken
+// inside it); the walk order gets widx[rank] = idx[b] and, when hpos is given, the
+// sorted position hpos[rank] = b (how a tape recorded in this order maps its
+// wavefronts' lanes back to sorted bodies).  A rank outside [0, n_live) (a broken
 // difference array) is not stored and flags order_bad, which fails the build.
 __global__ __launch_bounds__(CSG_TPB) void k_hrank(const int *__restrict__ hdiff,
                                                    const int *__restrict__ hbnd,
@@ -1679,7 +1681,8 @@ __global__ __launch_bounds__(CSG_TPB) void k_hrank(const int *__restrict__ hdiff
                                                    const int *__restrict__ hpart,
                                                    const int *__restrict__ idx,
                                                    BuildParams *__restrict__ bp,
-                                                   int *__restrict__ widx) {
+                                                   int *__restrict__ widx,
+                                                   int *__restrict__ hpos) {
     __shared__ int lds4[4];
     const int nl = bp->n_live;
     if (bp->overflow) return;
@@ -1704,8 +1707,12 @@ __global__ __launch_bounds__(CSG_TPB) void k_hrank(const int *__restrict__ hdiff
         if (b < nl) {
             run += v[r];
             const int rank = b + run;
-            if ((unsigned)rank < (unsigned)nl) widx[rank] = idx[b];
-            else bp->order_bad = 1;
+            if ((unsigned)rank < (unsigned)nl) {
+                widx[rank] = idx[b];
+                if (hpos) hpos[rank] = b;
+            } else {
+                bp->order_bad = 1;
+            }
         }
     }
 }
