@@ -32,6 +32,24 @@ __device__ __forceinline__ void tidal_apply(double dx, double dy, double dz, dou
     gz = s * (wz - k * dz);
 }
 
+// T w / G for one interaction of the tape replay: m r^-3 (w - 3 (d.w) d / r^2),
+// r^2 = |d|^2 + eps^2.  1/r is the hardware estimate refined by one Newton step
+// (relative error ~1e-14, as in the FP64 walks); G multiplies the sums.
+__device__ __forceinline__ void tidal_apply_m(double dx, double dy, double dz, double wx,
+                                              double wy, double wz, double m, double eps2,
+                                              double &gx, double &gy, double &gz) {
+    const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double hr = 0.5 * r2;
+    const double ri = y * fma(-hr * y, y, 1.5);
+    const double ri2 = ri * ri;
+    const double s = m * (ri2 * ri);
+    const double k = (3.0 * fma(dx, wx, fma(dy, wy, dz * wz))) * ri2;
+    gx = s * fma(-k, dx, wx);
+    gy = s * fma(-k, dy, wy);
+    gz = s * fma(-k, dz, wz);
+}
+
 // Weights in sorted order: ws[b] = w[idx[b]].
 __global__ __launch_bounds__(CSG_TPB) void k_adj_gather_w(const int *__restrict__ idx,
                                                           const double *__restrict__ w,
@@ -45,26 +63,53 @@ __global__ __launch_bounds__(CSG_TPB) void k_adj_gather_w(const int *__restrict_
     ws[3 * (size_t)b + 2] = w[3 * i + 2];
 }
 
-// Exclusive scan of the per-wave tape counts (one block); total -> *total.
-__global__ __launch_bounds__(CSG_TPB) void k_tape_scan(const int *__restrict__ count, int nw,
-                                                       int *__restrict__ off, int *__restrict__ total) {
+// Exclusive scan of the per-wave tape counts in two launches: k_tape_sums sums
+// each tile of CSG_TILE counts; k_tape_scan adds the sums of the tiles before its
+// own and scans its tile.  The last tile writes the total to *total.
+__global__ __launch_bounds__(CSG_TPB) void k_tape_sums(const int *__restrict__ count, int nw,
+                                                       int *__restrict__ part) {
     __shared__ int lds4[4];
-    int carry = 0;
-    for (int c0 = 0; c0 < nw; c0 += CSG_TPB) {
-        const int i = c0 + threadIdx.x;
-        const int v = i < nw ? count[i] : 0;
-        int t;
-        const int ex = block_excl_scan(v, lds4, t);
-        if (i < nw) off[i] = carry + ex;
-        carry += t;
+    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r)
+        if (base + r < nw) s += count[base + r];
+    int total;
+    (void)block_excl_scan(s, lds4, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(CSG_TPB) void k_tape_scan(const int *__restrict__ count, int nw,
+                                                       const int *__restrict__ part,
+                                                       int *__restrict__ off,
+                                                       int *__restrict__ total) {
+    __shared__ int lds4[4];
+    int c = 0;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += CSG_TPB) c += part[k];
+    int carry;
+    (void)block_excl_scan(c, lds4, carry);
+    const int base = blockIdx.x * CSG_TILE + threadIdx.x * CSG_ITEMS;
+    int v[CSG_ITEMS], sum = 0;
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        v[r] = base + r < nw ? count[base + r] : 0;
+        sum += v[r];
     }
-    if (threadIdx.x == 0) *total = carry;
+    int tile;
+    int run = carry + block_excl_scan(sum, lds4, tile);
+#pragma unroll
+    for (int r = 0; r < CSG_ITEMS; ++r) {
+        if (base + r < nw) off[base + r] = run;
+        run += v[r];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = carry + tile;
 }
 
 // Step 1: per-lane -T w and per-record lane sums (one wave per tape wavefront,
 // one wave per workgroup).  Lane l of tape wavefront w holds sorted body
-// 64 w + l, or wpos[64 w + l] for a tape recorded in Peano-Hilbert order.  The wave index is uniform, so tape records and node
-// records come by scalar loads, four records per round (their latencies
+// 64 w + l, or wpos[64 w + l] for a tape recorded in Peano-Hilbert order.  The
+// wave index is uniform, so tape records and node records come by scalar
+// loads, four records per round (their latencies
 // overlap).  The lane sums of a round go through LDS, two records (six sums) at
 // a time: every lane stores its six values, lane 8 s + p adds the values of
 // lanes p, p + 8, ..., p + 56 for sum s in that order, and the eight partials of
@@ -78,7 +123,7 @@ __global__ __launch_bounds__(ADJ_WAVE_TPB) void k_adj_walk(
     const BuildParams *__restrict__ bp, const double *__restrict__ xs,
     const double *__restrict__ ys, const double *__restrict__ zs, const double *__restrict__ ws,
     const int *__restrict__ wpos, double G, double eps2, double *__restrict__ xbar_s,
-    uint64_t *__restrict__ rkey, int *__restrict__ rval, double *__restrict__ rsum) {
+    uint32_t *__restrict__ rkey, int *__restrict__ rval, double *__restrict__ rsum) {
     __shared__ double sums[6][72];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
@@ -109,8 +154,8 @@ __global__ __launch_bounds__(ADJ_WAVE_TPB) void k_adj_walk(
         for (int k = 0; k < 4; ++k) {
             double gx = 0.0, gy = 0.0, gz = 0.0;
             if (r0 + k < cnt && ((rr[k].imask >> lane) & 1ull))
-                tidal_apply(cm[k][0] - x, cm[k][1] - y, cm[k][2] - z, wx, wy, wz, G * cm[k][3],
-                            eps2, gx, gy, gz);
+                tidal_apply_m(cm[k][0] - x, cm[k][1] - y, cm[k][2] - z, wx, wy, wz, cm[k][3],
+                              eps2, gx, gy, gz);
             ax -= gx; ay -= gy; az -= gz;
             g[k][0] = gx; g[k][1] = gy; g[k][2] = gz;
         }
@@ -136,9 +181,9 @@ __global__ __launch_bounds__(ADJ_WAVE_TPB) void k_adj_walk(
                 const int rec = r0 + 2 * h + kk;
                 if (rec < cnt) {
                     const size_t q = (size_t)off + rec;
-                    rsum[3 * q + c] = acc;
+                    rsum[3 * q + c] = G * acc;
                     if (c == 0) {
-                        rkey[q] = (uint64_t)rr[2 * h + kk].node;
+                        rkey[q] = (uint32_t)rr[2 * h + kk].node;
                         rval[q] = (int)q;
                     }
                 }
@@ -146,9 +191,9 @@ __global__ __launch_bounds__(ADJ_WAVE_TPB) void k_adj_walk(
         }
     }
     if (act) {
-        xbar_s[3 * (size_t)b] = ax;
-        xbar_s[3 * (size_t)b + 1] = ay;
-        xbar_s[3 * (size_t)b + 2] = az;
+        xbar_s[3 * (size_t)b] = G * ax;
+        xbar_s[3 * (size_t)b + 1] = G * ay;
+        xbar_s[3 * (size_t)b + 2] = G * az;
     }
 }
 
@@ -179,14 +224,14 @@ static void fill_i32(int *p, size_t n, int v, hipStream_t st) {
 // records t, t + 256, ... and combine in a fixed tree.  Either way the summation
 // order depends only on the records, so the result is reproducible.
 #define CSG_SEG_SHORT 32
-__global__ __launch_bounds__(CSG_TPB) void k_seg_bounds(const uint64_t *__restrict__ key,
+__global__ __launch_bounds__(CSG_TPB) void k_seg_bounds(const uint32_t *__restrict__ key,
                                                         const int *__restrict__ total,
                                                         int *__restrict__ first,
                                                         int *__restrict__ end) {
     const int R = *total;
     const int r = blockIdx.x * CSG_TPB + threadIdx.x;
     if (r >= R) return;
-    const uint64_t k = key[r];
+    const uint32_t k = key[r];
     if (r == 0 || key[r - 1] != k) first[k] = r;
     if (r == R - 1 || key[r + 1] != k) end[k] = r + 1;
 }
@@ -375,10 +420,16 @@ __global__ __launch_bounds__(CSG_TPB) void k_adj_finish(const int *__restrict__ 
     const int b = blockIdx.x * CSG_TPB + threadIdx.x;
     if (b >= bp->n_live) return;
     const size_t i = (size_t)idx[b];
+    const double m = ms[b];
+    double v[3], o[3] = {0.0, 0.0, 0.0};
+    // every load first (the caller-order ones are scattered), then the stores
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const double v = xbar_s[3 * (size_t)b + c] + ms[b] * U[3 * (size_t)b + c];
-        xbar[3 * i + c] = accumulate ? xbar[3 * i + c] + v : v;
+        v[c] = xbar_s[3 * (size_t)b + c] + m * U[3 * (size_t)b + c];
+        if (accumulate) o[c] = xbar[3 * i + c];
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xbar[3 * i + c] = accumulate ? o[c] + v[c] : v[c];
 }
 
 // Frozen-tree forces: the node records get centres of mass and masses from the
@@ -644,7 +695,9 @@ static int force_vjp_dev(csg_ctx *c, const double *w_dev, double *xbar_dev, int 
     if (!accumulate) fill_f64(xbar_dev, 3 * (size_t)n, 0.0, st);
     const int nw = blocks_for(nl, 64);
     k_adj_gather_w<<<blocks_for(nl, CSG_TPB), CSG_TPB, 0, st>>>(c->idx, w_dev, c->bp, c->ws);
-    k_tape_scan<<<1, CSG_TPB, 0, st>>>(c->tcount, nw, c->toff, c->tape_total);
+    const int ntt = blocks_for(nw, CSG_TILE);   // nw >= 1 here
+    k_tape_sums<<<ntt, CSG_TPB, 0, st>>>(c->tcount, nw, c->scan_part);
+    k_tape_scan<<<ntt, CSG_TPB, 0, st>>>(c->tcount, nw, c->scan_part, c->toff, c->tape_total);
     int R = 0;
     CSG_CHECK(hipMemcpyAsync(&R, c->tape_total, sizeof(int), hipMemcpyDeviceToHost, st));
     CSG_CHECK(hipStreamSynchronize(st));
@@ -658,7 +711,7 @@ static int force_vjp_dev(csg_ctx *c, const double *w_dev, double *xbar_dev, int 
     int bits = 1;
     while ((1ll << bits) < (long long)nn) ++bits;
     const int passes = (bits + 7) / 8;
-    uint64_t *kin = c->rkey, *kout = c->rkey2;
+    uint32_t *kin = c->rkey, *kout = c->rkey2;
     int *vin = c->rval, *vout = c->rval2;
     if (R > 0) {
         const int nb = blocks_for(R, CSG_TILE);
@@ -667,7 +720,7 @@ static int force_vjp_dev(csg_ctx *c, const double *w_dev, double *xbar_dev, int 
             k_rs_rowscan<<<256, CSG_TPB, 0, st>>>(c->rhist, nb, c->rtot);
             k_rs_scatter<<<nb, CSG_TPB, 0, st>>>(kin, vin, kout, vout, R, 8 * pass, c->rhist,
                                                  c->rtot, nb);
-            uint64_t *tk = kin; kin = kout; kout = tk;
+            uint32_t *tk = kin; kin = kout; kout = tk;
             int *tv = vin; vin = vout; vout = tv;
         }
     }
@@ -1012,7 +1065,11 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
             const int s0 = seg * every, s1 = (s0 + every < k) ? s0 + every : k;
             if ((s = state_copy(c, c->ckpt + slot * seg, 0))) return s;
             for (int step = s0; step < s1; ++step) {
-                if ((s = substeps(c, 1, dt, G, eps, th, prec, false))) return s;
+                // the segment's last positions need no forces here: the reverse
+                // substep builds and tapes them
+                if (step == s1 - 1) s = launch_kick_drift(c, dt, 1);
+                else s = substeps(c, 1, dt, G, eps, th, prec, false);
+                if (s) return s;
                 double *dstp = c->seg + 3 * (size_t)n * (step - s0);
                 CSG_CHECK(hipMemcpyAsync(dstp, c->x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
                 CSG_CHECK(hipMemcpyAsync(dstp + n, c->y, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
@@ -1074,7 +1131,7 @@ int csg_backward(csg_ctx *c, const double *gx, const double *gv, int mode, doubl
     if (stats) {
         stats[0] = xs_dev;
         size_t bytes = (size_t)c->tape_waves * c->tape_cap * sizeof(TapeRec) +
-                       c->rec_cap * (2 * sizeof(uint64_t) + 2 * sizeof(int) + 3 * sizeof(double)) +
+                       c->rec_cap * (2 * sizeof(uint32_t) + 2 * sizeof(int) + 3 * sizeof(double)) +
                        (size_t)c->ckpt_slots * slot * sizeof(double) +
                        (size_t)c->seg_slots * 3 * n * sizeof(double) +
                        (size_t)c->node_cap * (sizeof(WNode) + 3 * sizeof(double)) +

@@ -76,7 +76,7 @@ struct csg_ctx {
     TapeRec *tape = nullptr;
     int *tcount = nullptr, *toff = nullptr, *tape_total = nullptr, *tape_ovf = nullptr;
     size_t rec_cap = 0;
-    uint64_t *rkey = nullptr, *rkey2 = nullptr;
+    uint32_t *rkey = nullptr, *rkey2 = nullptr;   // adjoint records' node ids
     int *rval = nullptr, *rval2 = nullptr, *rhist = nullptr, *rtot = nullptr;
     double *rsum = nullptr, *ws = nullptr, *xbar_s = nullptr, *Dv = nullptr, *Uv = nullptr;
     double *cbar = nullptr, *xb = nullptr, *vb = nullptr, *ab = nullptr, *gtmp = nullptr;

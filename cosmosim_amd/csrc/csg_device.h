@@ -489,8 +489,10 @@ __global__ __launch_bounds__(RS_TPB) void k_keys(const double *__restrict__ x,
 // ============================================================ radix sort
 // Tile of CSG_TILE consecutive elements per block; wave w owns the sub-tile
 // [w*512, w*512+512) and walks it in 8 rounds of 64, so (wave, round, lane) is
-// the element order and the per-wave counters make the scatter stable.
-__global__ __launch_bounds__(CSG_TPB) void k_rs_hist(const uint64_t *__restrict__ key, int n,
+// the element order and the per-wave counters make the scatter stable.  Keys
+// are 64-bit (neighbour queries) or 32-bit (the adjoint's node ids).
+template <typename K>
+__global__ __launch_bounds__(CSG_TPB) void k_rs_hist(const K *__restrict__ key, int n,
                                                      int shift, int *__restrict__ hist, int nb) {
     __shared__ int h[256];
     h[threadIdx.x] = 0;
@@ -522,8 +524,9 @@ __global__ __launch_bounds__(CSG_TPB) void k_rs_rowscan(int *__restrict__ hist, 
     if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
+template <typename K>
 __global__ __launch_bounds__(CSG_TPB) void k_rs_scatter(
-    const uint64_t *__restrict__ kin, const int *__restrict__ vin, uint64_t *__restrict__ kout,
+    const K *__restrict__ kin, const int *__restrict__ vin, K *__restrict__ kout,
     int *__restrict__ vout, int n, int shift, const int *__restrict__ hist,
     const int *__restrict__ tot, int nb) {
     __shared__ int wcnt[4][256];
@@ -538,13 +541,13 @@ __global__ __launch_bounds__(CSG_TPB) void k_rs_scatter(
     __syncthreads();
     const int base = blockIdx.x * CSG_TILE + w * (64 * CSG_ITEMS);
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint64_t kk[CSG_ITEMS];
+    K kk[CSG_ITEMS];
     int vv[CSG_ITEMS], dg[CSG_ITEMS], rk[CSG_ITEMS];
 #pragma unroll
     for (int r = 0; r < CSG_ITEMS; ++r) {
         const int i = base + r * 64 + lane;
         const bool valid = i < n;
-        kk[r] = valid ? kin[i] : 0ull;
+        kk[r] = valid ? kin[i] : (K)0;
         vv[r] = valid ? vin[i] : 0;
         const int d = (int)((kk[r] >> shift) & 255);
         dg[r] = d;
@@ -575,7 +578,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_rs_scatter(
     __syncthreads();
     // stage the tile in digit-sorted order, then write each digit run with
     // consecutive lanes (coalesced)
-    __shared__ uint64_t skey[CSG_TILE];
+    __shared__ K skey[CSG_TILE];
     __shared__ int sval[CSG_TILE];
 #pragma unroll
     for (int r = 0; r < CSG_ITEMS; ++r) {
@@ -590,7 +593,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_rs_scatter(
     __syncthreads();
     const int cnt = min(CSG_TILE, n - blockIdx.x * CSG_TILE);
     for (int j = threadIdx.x; j < cnt; j += CSG_TPB) {
-        const uint64_t k = skey[j];
+        const K k = skey[j];
         const int d = (int)((k >> shift) & 255);
         const int pos = binbase[d] + hist[d * nb + blockIdx.x] + (j - tstart[d]);
         kout[pos] = k;
