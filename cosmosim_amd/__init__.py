@@ -13,12 +13,12 @@ import os
 import numpy as np
 
 from ._abi import (BODY_DTYPE, NODE_DTYPE, CosmosimConfig, ExtParams, ExtStats, QuasarConfig,
-                   PRECISIONS, PREC_FP64_STRICT, PREC_FP64, PREC_FP32, IC_DISK, IC_MERGER,
+                   PRECISIONS, PREC_FP64_STRICT, PREC_FP64, PREC_FP32, PREC_FP32_LIST, IC_DISK, IC_MERGER,
                    make_bodies, positions, velocities, accelerations, ptr)
 
 __all__ = [
     "BODY_DTYPE", "NODE_DTYPE", "CosmosimConfig", "ExtParams", "ExtStats", "QuasarConfig",
-    "PRECISIONS", "PREC_FP64_STRICT", "PREC_FP64", "PREC_FP32", "IC_DISK", "IC_MERGER", "make_bodies",
+    "PRECISIONS", "PREC_FP64_STRICT", "PREC_FP64", "PREC_FP32", "PREC_FP32_LIST", "IC_DISK", "IC_MERGER", "make_bodies",
     "positions", "velocities", "accelerations", "lib", "library_path", "CosmosimError",
     "octree_build", "octree_compute_forces", "octree_find_neighbors", "integrator_step",
     "integrator_init_accelerations", "generate", "default_orbit", "orbit_coefficients",

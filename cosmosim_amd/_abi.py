@@ -43,7 +43,9 @@ BODY_STAR, BODY_GAS, BODY_SMBH, BODY_JET, BODY_DUST, BODY_LOBE = range(6)
 PREC_FP64_STRICT = 0
 PREC_FP64 = 1
 PREC_FP32 = 2
-PRECISIONS = {"fp64_strict": PREC_FP64_STRICT, "fp64": PREC_FP64, "fp32": PREC_FP32}
+PREC_FP32_LIST = 3
+PRECISIONS = {"fp64_strict": PREC_FP64_STRICT, "fp64": PREC_FP64, "fp32": PREC_FP32,
+              "fp32_list": PREC_FP32_LIST}
 
 IC_DISK = 0
 IC_MERGER = 1
@@ -110,6 +112,8 @@ class ExtStats(ctypes.Structure):
         ("ms_neighbors", ctypes.c_double),
         ("neighbor_queries", ctypes.c_longlong),
         ("ms_sph", ctypes.c_double),
+        ("list_fallbacks", ctypes.c_int),
+        ("reserved", ctypes.c_int),
     ]
 
 

@@ -323,7 +323,7 @@ COSMOSIM_API const char *cosmosim_ext_error_string(int code)
 
 static int valid_params(const CosmosimExtParams *p)
 {
-    return p && p->precision >= COSMOSIM_PREC_FP64_STRICT && p->precision <= COSMOSIM_PREC_FP32 &&
+    return p && p->precision >= COSMOSIM_PREC_FP64_STRICT && p->precision <= COSMOSIM_PREC_FP32_LIST &&
            p->theta >= 0.0 && p->softening >= 0.0 && p->device >= 0;
 }
 
@@ -511,6 +511,7 @@ COSMOSIM_API int cosmosim_ext_get_stats(SimHandle h, CosmosimExtStats *out)
     out->ms_neighbors = cs.ms_neighbors;
     out->neighbor_queries = cs.neighbor_queries;
     out->ms_sph = cs.ms_sph;
+    out->list_fallbacks = cs.list_fallbacks;
     return COSMOSIM_OK;
 }
 

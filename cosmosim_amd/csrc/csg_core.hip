@@ -142,6 +142,14 @@ struct csg_ctx {
     int walk_block_waves = 1;  // waves per block of the ordered walks
     int wave_split = 2;        // sub-ranges per XCD range of the walk's dispatch order
                                // (1M merger: 1.74 / 1.68 / 1.72 / 1.71 ms walk for 1-4)
+    int list_max = 4096;       // list entries after which a list walk's wave falls back
+    int list_block_waves = 2;  // list walk: independent wavefronts per workgroup (1, 2, 4;
+                               // 1M merger at theta 0.5: 1.35 / 1.29 / 1.31 ms walk)
+    int *wl_fall = nullptr;    // list walks: waves that fell back (device counter)
+    int *wmode = nullptr;      // list walks: per wave, walks left on the union traversal
+    int list_sticky = 8;       // walks a wave that fell back stays on the union traversal
+                               // (0: every walk tries the list; results then depend only
+                               // on the state, not on the walks before)
 };
 
 #define CSG_CHECK(expr)                                                                      \
@@ -192,7 +200,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->idx); dfree(c->idx2); dfree(c->widx); dfree(c->hpos); dfree(c->nlast); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
     dfree(c->lcp); dfree(c->lmin1); dfree(c->lmin2); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->tpref); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
-    dfree(c->wcost); dfree(c->worder); dfree(c->xyzm);
+    dfree(c->wcost); dfree(c->worder); dfree(c->xyzm); dfree(c->wmode);
     c->wcost_n = 0;
     c->cap = 0;
     c->node_cap = 0;
@@ -238,7 +246,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->lmin2, lmin2_bytes(C)) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
               dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->tpref, (size_t)4 * CSG_TPB * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) && dalloc(c->nlast, 8 * C + 4096) &&
-              dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->worder, 2 * (size_t)(C / 64 + 1) + 4096) &&
+              dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->wmode, C / 64 + 1) && dalloc(c->worder, 2 * (size_t)(C / 64 + 1) + 4096) &&
               dalloc(c->xyzm, C) && dalloc(c->rs_ctl, 1) &&
               dalloc(c->rs_status, (size_t)256 * blocks_for((long long)C, RS_TILE));
     if (!ok) {
@@ -251,6 +259,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
     clear_words(c->idx, C, c->stream);
     clear_words(c->idx2, C, c->stream);
     clear_words(c->widx, C, c->stream);
+    clear_words(c->wmode, C / 64 + 1, c->stream);
     c->rs_epoch = 0;
     if (hipGetLastError() != hipSuccess) {
         free_buffers(c);
@@ -384,6 +393,12 @@ int csg_create(int device, int capacity, csg_ctx **out) {
         const int v = atoi(g);
         if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) c->walk_block_waves = v;
     }
+    if (const char *g = getenv("COSMOSIM_LIST_MAX")) c->list_max = atoi(g);
+    if (const char *g = getenv("COSMOSIM_LIST_STICKY")) c->list_sticky = atoi(g) > 0 ? atoi(g) : 0;
+    if (const char *g = getenv("COSMOSIM_LIST_BLOCK_WAVES")) {
+        const int v = atoi(g);
+        if (v == 1 || v == 2 || v == 4) c->list_block_waves = v;
+    }
     if (const char *g = getenv("COSMOSIM_WALK_GROUP32")) {
         const int v = atoi(g);
         if (v == 0 || v == 8 || v == 16 || v == 32 || v == 64) c->walk_group32 = v;
@@ -397,6 +412,7 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     if (hipMalloc((void **)&c->bp, sizeof(BuildParams)) != hipSuccess ||
         hipHostMalloc((void **)&c->hbp, sizeof(BuildParams), hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&c->itotal, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void **)&c->wl_fall, sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&c->err_dev, sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
         csg_destroy(c);
@@ -411,6 +427,7 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     if (const char *g = getenv("COSMOSIM_HILBERT_MIN_WAVES")) c->resident_waves = atoi(g);
     // the build counters start at zero (k_bbox_final counts earlier failures)
     clear_words((int *)c->bp, sizeof(BuildParams) / sizeof(int), c->stream);
+    clear_words(c->wl_fall, 1, c->stream);
     int s = alloc_buffers(c, capacity);
     if (s != COSMOSIM_OK) {
         csg_destroy(c);
@@ -434,6 +451,7 @@ void csg_destroy(csg_ctx *c) {
     dfree(c->bp);
     dfree(c->itotal);
     dfree(c->err_dev);
+    dfree(c->wl_fall);
     dfree(c->pool_dev);
     if (c->diag) (void)hipFree(c->diag);
     if (c->hbp) (void)hipHostFree(c->hbp);
@@ -491,6 +509,7 @@ int csg_set_bodies(csg_ctx *c, const Body *bodies, int n) {
     c->tree_valid = 0;
     c->counts_valid = 0;
     c->wcost_n = 0;
+    clear_words(c->wmode, (size_t)(c->cap / 64 + 1), c->stream);   // new bodies: no history
     if (n == 0) return COSMOSIM_OK;
     CSG_CHECK(hipMemcpyAsync(c->aos, bodies, sizeof(Body) * (size_t)n, hipMemcpyHostToDevice,
                              c->stream));
@@ -594,8 +613,20 @@ int csg_set_state(csg_ctx *c, const double *pos, const double *vel, const double
         CSG_CHECK(hipMemcpyAsync(c->m, mass, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
         CSG_CHECK(hipStreamSynchronize(c->stream));
     }
-    if (pos || mass) c->tree_valid = 0;
+    if (pos || mass) {
+        c->tree_valid = 0;
+        clear_words(c->wmode, (size_t)(c->cap / 64 + 1), c->stream);   // no list-walk history
+    }
     return COSMOSIM_OK;
+}
+
+// Walk arithmetic codes the core accepts, and those whose walks read the FP32
+// node records (the FP64 records are then emitted only on demand).
+static inline bool prec_valid(int p) {
+    return p >= COSMOSIM_PREC_FP64_STRICT && p <= COSMOSIM_PREC_FP32_LIST;
+}
+static inline bool prec_rec32(int p) {
+    return p == COSMOSIM_PREC_FP32 || p == COSMOSIM_PREC_FP32_LIST;
 }
 
 static int launch_build(csg_ctx *c, bool with64 = true, int bbox = 0, int order = -1);
@@ -605,6 +636,8 @@ static int launch_build(csg_ctx *c, bool with64 = true, int bbox = 0, int order 
 // grouped FP64 walks from hilbert_min bodies, the wave-mask walks only with more
 // wavefronts than the GPU holds at once; the grouped FP32 walk keeps sorted order.
 static int hilbert_order_for(const csg_ctx *c, int precision) {
+    // the list walk's groups are most compact in Peano-Hilbert order at every size
+    if (precision == COSMOSIM_PREC_FP32_LIST) return c->hilbert_waves;
     if (!c->hilbert_waves || c->n < c->hilbert_min) return 0;
     if (precision == COSMOSIM_PREC_FP32 && c->walk_group32 != 0) return 0;
     const int g64 = c->walk_group >= 0 ? c->walk_group
@@ -616,7 +649,7 @@ static int hilbert_order_for(const csg_ctx *c, int precision) {
 // The FP64 node records of the current tree, re-emitted by a full build when the
 // last build wrote only the FP32 records (same positions, so the same tree).
 static int ensure_nodes64(csg_ctx *c, int precision) {
-    if (precision == COSMOSIM_PREC_FP32 || c->nodes64_valid) return COSMOSIM_OK;
+    if (prec_rec32(precision) || c->nodes64_valid) return COSMOSIM_OK;
     return launch_build(c, true, 0, hilbert_order_for(c, precision));
 }
 
@@ -636,8 +669,10 @@ static bool walk_order_params(const csg_ctx *c, int precision, bool taped, int n
     if (grouped32 || !c->wave_order || c->wcost_n != nw) return false;
     // blocks of gw waves (one by default: the finest balance; larger blocks, whose
     // waves share a CU's scalar cache, measured slower).  The FP64 walks' kernels are
-    // bounded at CSG_TPB threads.
-    *gw = wave_mask32 ? c->walk_block_waves : min(c->walk_block_waves, CSG_TPB / 64);
+    // bounded at CSG_TPB threads; the list walk's blocks hold list_block_waves
+    // independent wavefronts.
+    if (precision == COSMOSIM_PREC_FP32_LIST) *gw = c->list_block_waves;
+    else *gw = wave_mask32 ? c->walk_block_waves : min(c->walk_block_waves, CSG_TPB / 64);
     *capg = WO_CAP(blocks_for(nw, *gw));
     return true;
 }
@@ -797,6 +832,25 @@ static void launch_walk32w(csg_ctx *c, const int *sidx, double G, double eps2, d
         tape, wo);
 }
 
+// v: bit 1 = counts (per-body when lane_counts, and the total), bit 0 = own leaf
+// excluded by index
+template <int WPB>
+static void launch_walk32l(csg_ctx *c, int v, const int *sidx, double G, double eps2, int grid,
+                           bool lane_counts, WaveOrder wo) {
+    int *ic = lane_counts ? c->icount : nullptr;
+#define CSG_WL(C, S)                                                                         \
+    k_walk32l<C, S, WPB><<<grid, 64 * WPB, 0, c->stream>>>(c->nodes32, c->bp, sidx, c->x, c->y, \
+        c->z, c->ax, c->ay, c->az, ic, c->itotal, c->wl_fall, c->list_sticky ? c->wmode : nullptr, \
+        c->list_sticky, G, eps2, c->list_max, wo)
+    switch (v) {
+    case 3: CSG_WL(true, true); break;
+    case 2: CSG_WL(true, false); break;
+    case 1: CSG_WL(false, true); break;
+    default: CSG_WL(false, false); break;
+    }
+#undef CSG_WL
+}
+
 template <int PREC, bool TP>
 static void launch_walk64w(csg_ctx *c, const int *sidx, double G, double eps2, double th2,
                            int grid, int nb8, TapeOut tape, WaveOrder wo, int tpb) {
@@ -821,7 +875,7 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     hipStream_t st = c->stream;
     const double eps2 = softening * softening, th2 = theta * theta;
     c->theta_hint = theta;
-    if (precision == COSMOSIM_PREC_FP32 && c->tree_valid && c->th32 != theta) {
+    if (prec_rec32(precision) && c->tree_valid && c->th32 != theta) {
         // the FP32 records carry the opening angle: emit them again for this one
         const int s = launch_build(c, c->nodes64_valid != 0, 0, hilbert_order_for(c, precision));
         if (s) return s;
@@ -838,7 +892,8 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     int nb8 = blocks_for(n, CSG_TPB);
     const int gran = c->xcd_chunk > 0 ? 8 * c->xcd_chunk : 8;
     nb8 = (nb8 + gran - 1) / gran * gran;
-    if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
+    if (!prec_valid(precision) || (tape.rec && precision == COSMOSIM_PREC_FP32_LIST))
+        return COSMOSIM_E_ARG;
     c->counts_valid = 0;
     const bool wave_mask32 = precision == COSMOSIM_PREC_FP32 && (c->walk_group32 == 0 || tape.rec);
     const bool grouped32 = precision == COSMOSIM_PREC_FP32 && !wave_mask32;
@@ -858,6 +913,23 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         tpb = 64 * gw;
     }
     if (!grouped32) c->wcost_n = nw;
+    if (precision == COSMOSIM_PREC_FP32_LIST) {
+        // wave-shared interaction lists over groups of 64 bodies consecutive in
+        // Peano-Hilbert order, gw independent wavefronts per workgroup
+        const int *sidx_l = (hilbert && c->widx_valid) ? c->widx : c->idx;
+        const int wpb = wo.order ? gw : c->list_block_waves;
+        if (!wo.order) grid = blocks_for(nw, wpb);
+        const bool selfx = !((float)eps2 >= 1e-12f);
+        const int v = ((lane_counts || count_total) ? 2 : 0) | (selfx ? 1 : 0);
+        switch (wpb) {
+        case 2: launch_walk32l<2>(c, v, sidx_l, G, eps2, grid, lane_counts, wo); break;
+        case 4: launch_walk32l<4>(c, v, sidx_l, G, eps2, grid, lane_counts, wo); break;
+        default: launch_walk32l<1>(c, v, sidx_l, G, eps2, grid, lane_counts, wo); break;
+        }
+        CSG_LAUNCHED();
+        c->counts_valid = lane_counts ? 1 : 0;
+        return COSMOSIM_OK;
+    }
     if (wave_mask32) {
         // wave-mask traversal (also the taped FP32 walk: the tape is per wavefront)
         const bool off32 = (size_t)c->node_cap * sizeof(WNode32) < ((size_t)1 << 32);
@@ -985,7 +1057,7 @@ int csg_build(csg_ctx *c) {
 int csg_walk(csg_ctx *c, int precision, double G, double softening, double theta) {
     CSG_USE(c);
     if (!c->tree_valid) return COSMOSIM_E_STATE;
-    if (precision < 0 || precision > 2) return COSMOSIM_E_ARG;
+    if (!prec_valid(precision)) return COSMOSIM_E_ARG;
     int s0 = ensure_nodes64(c, precision);
     if (s0) return s0;
     timer_start(c);
@@ -1010,7 +1082,7 @@ int csg_substeps(csg_ctx *c, int k, double dt, double G, double softening, doubl
 static int substeps(csg_ctx *c, int k, double dt, double G, double softening, double theta,
                     int precision, bool total) {
     c->theta_hint = theta;
-    if (precision < 0 || precision > 2 || k < 0) return COSMOSIM_E_ARG;
+    if (!prec_valid(precision) || k < 0) return COSMOSIM_E_ARG;
     if (k == 0) return COSMOSIM_OK;
     int s, bbox = 0;
     timer_start(c);
@@ -1018,7 +1090,7 @@ static int substeps(csg_ctx *c, int k, double dt, double G, double softening, do
     timer_stop(c, PH_KDK);
     for (int sub = 0; sub < k; ++sub) {
         timer_start(c);
-        if ((s = launch_build(c, precision != COSMOSIM_PREC_FP32, bbox,
+        if ((s = launch_build(c, !prec_rec32(precision), bbox,
                               hilbert_order_for(c, precision))))
             return s;
         timer_stop(c, PH_BUILD);
@@ -1190,7 +1262,9 @@ int csg_get_stats(csg_ctx *c, csg_stats *o) {
     int s = read_params(c);
     if (s) return s;
     unsigned long long it = 0;
+    int fall = 0;
     CSG_CHECK(hipMemcpyAsync(&it, c->itotal, sizeof(it), hipMemcpyDeviceToHost, c->stream));
+    CSG_CHECK(hipMemcpyAsync(&fall, c->wl_fall, sizeof(fall), hipMemcpyDeviceToHost, c->stream));
     CSG_CHECK(hipStreamSynchronize(c->stream));
     collect_timers(c);
     memset(o, 0, sizeof(*o));
@@ -1209,6 +1283,7 @@ int csg_get_stats(csg_ctx *c, csg_stats *o) {
     o->ms_neighbors = c->ms_nb;
     o->ms_sph = c->ms_sph;
     o->neighbor_queries = c->nb_queries;
+    o->list_fallbacks = fall;
     return COSMOSIM_OK;
 }
 
@@ -1226,6 +1301,7 @@ int csg_set_timing(csg_ctx *c, int enabled) {
     c->ms_build = c->ms_walk = c->ms_kdk = c->ms_nb = c->ms_sph = 0.0;
     c->timed_walks = 0;
     c->nb_queries = 0;
+    clear_words(c->wl_fall, 1, c->stream);
     return COSMOSIM_OK;
 }
 

@@ -2263,38 +2263,21 @@ __device__ __forceinline__ int lane_sel_i32(unsigned long long m, int t, int f) 
 // w = order[s] (longest first, see k_wave_order).  Either way the results do not
 // depend on the dispatch order.  With WaveOrder::cost set, every wave stores its
 // node-step count there.
+// The union traversal of one wavefront (k_walk32w's loop): lanes with act hold
+// body me at (x, y, z); on return (sx - kx, sy - ky, sz - kz) is each lane's
+// compensated sum without G, cnt its interaction count (LANE_COUNTS), wcnt the
+// wave's (TOTAL), nrec its tape records (TAPING) and steps the wave's node-step
+// count to within the last round.
 template <bool LANE_COUNTS, bool TAPING, bool OFF32, bool TOTAL, bool SELF>
-__global__ __launch_bounds__(1024) void k_walk32w(
-    const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
-    const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
-    const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
-    double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
-    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape, WaveOrder wo) {
-    const int n_live = bp->n_live;
-    const int nnodes = bp->n_nodes;
-    if (bp->overflow || bp->order_bad) return;   // a failed build: no walk
-    const int lane = threadIdx.x & 63;
-    const int wv = walk_wave(nb8, chunk, wo);
-    if (wv < 0) return;
-    const int b0 = wv * 64;
-    if (b0 >= n_live) {
-        if (wo.cost && lane == 0 && wv < wo.n) wo.cost[wv] = 0;
-        return;
-    }
-    const int b = b0 + lane;
-    const bool act = b < n_live;
-    int nrec = 0;
-    const int me = act ? sidx[b] : -1;
-    const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
-    const float eps2f = (float)eps2;  // theta is folded into the records' s2
-    (void)theta2;
+__device__ __forceinline__ void walk32_union(
+    const WNode32 *__restrict__ nodes, int nnodes, int lane, int b0, bool act, int me, double x,
+    double y, double z, float eps2f, const TapeOut &tape, int &nrec, int &cnt,
+    unsigned long long &wcnt, float &sx, float &sy, float &sz, float &kx, float &ky, float &kz,
+    int &steps) {
     const char *const base = (const char *)nodes;
     int p = act ? 0 : 0x7fffffff;
-    int cnt = 0;
-    unsigned long long wcnt = 0;
     unsigned wc32 = 0;
     float bx = 0.f, by = 0.f, bz = 0.f;               // block sums
-    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     int ii = 0, quads = 0, folds = 0;
     // one node step of the wave (ii -> next node in the wave's depth-first order)
     auto step = [&]() {
@@ -2351,9 +2334,42 @@ __global__ __launch_bounds__(1024) void k_walk32w(
         }
     }
     wcnt += wc32;
-    // node steps of this wave, to within the last round (the dispatch order's key)
-    if (wo.cost && lane == 0) wo.cost[wv] = folds * CSG_FOLD_EVERY + quads * 8;
+    steps = folds * CSG_FOLD_EVERY + quads * 8;
     kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
+}
+
+template <bool LANE_COUNTS, bool TAPING, bool OFF32, bool TOTAL, bool SELF>
+__global__ __launch_bounds__(1024) void k_walk32w(
+    const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
+    const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
+    double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
+    double G, double eps2, double theta2, int nb8, int chunk, TapeOut tape, WaveOrder wo) {
+    const int n_live = bp->n_live;
+    const int nnodes = bp->n_nodes;
+    if (bp->overflow || bp->order_bad) return;   // a failed build: no walk
+    const int lane = threadIdx.x & 63;
+    const int wv = walk_wave(nb8, chunk, wo);
+    if (wv < 0) return;
+    const int b0 = wv * 64;
+    if (b0 >= n_live) {
+        if (wo.cost && lane == 0 && wv < wo.n) wo.cost[wv] = 0;
+        return;
+    }
+    const int b = b0 + lane;
+    const bool act = b < n_live;
+    int nrec = 0;
+    const int me = act ? sidx[b] : -1;
+    const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
+    const float eps2f = (float)eps2;  // theta is folded into the records' s2
+    (void)theta2;
+    int cnt = 0, steps = 0;
+    unsigned long long wcnt = 0;
+    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+    walk32_union<LANE_COUNTS, TAPING, OFF32, TOTAL, SELF>(nodes, nnodes, lane, b0, act, me, x, y,
+        z, eps2f, tape, nrec, cnt, wcnt, sx, sy, sz, kx, ky, kz, steps);
+    // node steps of this wave, to within the last round (the dispatch order's key)
+    if (wo.cost && lane == 0) wo.cost[wv] = steps;
     if (act) {
         ax[me] = G * ((double)sx - (double)kx);
         ay[me] = G * ((double)sy - (double)ky);
@@ -2362,6 +2378,235 @@ __global__ __launch_bounds__(1024) void k_walk32w(
     }
     if constexpr (TAPING) tape_close(tape, b0 >> 6, lane, nrec);
     if constexpr (TOTAL) {
+        if (lane == 0) atomicAdd(itotal, wcnt);
+    }
+}
+
+// FP32 walk on wave-shared interaction lists.  A wavefront holds 64 bodies
+// consecutive in the walk order and treats them as one group: a node is
+// accepted for the whole group when the opening criterion holds at the group's
+// bounding box (s2 < dmin^2, dmin the distance from the node's centre of mass
+// to the box), which implies the reference criterion (2h)^2/d^2 < theta^2 of
+// src/octree.c:203-205 for every body of the group; leaves are always taken.
+// The accepted nodes form the group's interaction list, and every lane sums
+// the whole list for its body, so each list entry keeps all 64 lanes busy with
+// one interaction.  Offsets are group-relative: the group's box centre g is
+// formed in FP64, every entry holds fp32(C - g) and every lane fp32(x - g), both
+// subtractions in FP64, and a lane's offset is their FP32 difference.
+//
+// Traversal: a stack in LDS of sibling chains (first node, end of the parent's
+// subtree).  Each round pops up to 64 chains, one per lane; a lane tests its
+// chain's first node, appends it to the list when accepted, and pushes the rest
+// of the chain (the node's skip, while below the end) and, for an opened node,
+// its children's chain (node + 1 up to its skip).  The buffered list (WL_LCAP
+// entries) is summed by all lanes whenever the next round could overflow it,
+// with a compensated fold every CSG_FOLD_EVERY entries.
+//
+// A group whose traversal would overrun the stack, or whose list grows past
+// lmax entries (a wavefront whose bodies are spread over a wide box), falls
+// back to the union traversal with each body's own criterion (walk32_union).
+// With wmode (one word per wave, may be null), a wave that fell back takes the
+// union traversal directly in its next `sticky` walks (its bodies change little
+// from one substep to the next), then tries the list again: each walk is then a
+// deterministic function of the state and of the walks since the bodies were
+// set.  Both traversals apply at least the reference criterion to every body.
+//   COUNT: per-body interaction counts into icount (when not null) and the
+//          wave's total into *itotal; the own leaf is not counted;
+//   SELF:  exclude the own leaf by index (softening below the FP32 range).
+//          Without it the own leaf's offset is exactly 0 and its term is 0.
+// WPB independent wavefronts per workgroup, each on its own LDS slice; wave slot
+// s = WPB * blockIdx.x + (wavefront in block) takes wave order[s] with an order,
+// else wave s.  With WaveOrder::cost set every wave stores a cost in
+// node-step units (the dispatch order's key).  Waves that fell back add 1 to
+// *fallbacks when it is not null.
+#define WL_LCAP 128
+#define WL_SCAP 384
+
+__device__ __forceinline__ int lane_rank(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Orders the LDS accesses of one wavefront's lanes (the wave's own slice of the
+// workgroup's LDS; no other wavefront touches it).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool COUNT, bool SELF, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_walk32l(
+    const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
+    const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
+    double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
+    int *__restrict__ fallbacks, int *__restrict__ wmode, int sticky, double G, double eps2,
+    int lmax, WaveOrder wo) {
+    __shared__ int2 stk_all[WPB][WL_SCAP];
+    __shared__ float4 lst_all[WPB][WL_LCAP];
+    __shared__ int lbd_all[WPB][COUNT || SELF ? WL_LCAP : 1];
+    const int n_live = bp->n_live;
+    const int nnodes = bp->n_nodes;
+    if (bp->overflow || bp->order_bad) return;   // a failed build: no walk
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    int2 *const stk = stk_all[wib];
+    float4 *const lst = lst_all[wib];
+    int *const lbd = lbd_all[wib];
+    const int slot = (int)blockIdx.x * WPB + wib;
+    int wv = slot;
+    if (wo.order) {
+        if (slot >= wo.slots) return;
+        wv = wo.order[slot];
+        if (wv < 0 || wv >= wo.n) return;
+    }
+    const int b0 = wv * 64;
+    if (b0 >= n_live) {
+        if (wo.cost && lane == 0 && wv < wo.n) wo.cost[wv] = 0;
+        return;
+    }
+    const int b = b0 + lane;
+    const bool act = b < n_live;
+    const int me = act ? sidx[b] : -1;
+    const double x = act ? px[me] : 0.0, y = act ? py[me] : 0.0, z = act ? pz[me] : 0.0;
+    const float eps2f = (float)eps2;
+    // the group's bounding box
+    double lx = act ? x : DBL_MAX, ly = act ? y : DBL_MAX, lz = act ? z : DBL_MAX;
+    double hx = act ? x : -DBL_MAX, hy = act ? y : -DBL_MAX, hz = act ? z : -DBL_MAX;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lx = fmin(lx, __shfl_xor(lx, o, 64)); hx = fmax(hx, __shfl_xor(hx, o, 64));
+        ly = fmin(ly, __shfl_xor(ly, o, 64)); hy = fmax(hy, __shfl_xor(hy, o, 64));
+        lz = fmin(lz, __shfl_xor(lz, o, 64)); hz = fmax(hz, __shfl_xor(hz, o, 64));
+    }
+    const double gx = 0.5 * (lx + hx), gy = 0.5 * (ly + hy), gz = 0.5 * (lz + hz);
+    // half extents, widened by 2^-20 so the FP32 box holds every body
+    const float wx = (float)(0.5 * (hx - lx)) * (1.0f + 0x1p-20f);
+    const float wy = (float)(0.5 * (hy - ly)) * (1.0f + 0x1p-20f);
+    const float wz = (float)(0.5 * (hz - lz)) * (1.0f + 0x1p-20f);
+    const float qx = (float)(x - gx), qy = (float)(y - gy), qz = (float)(z - gz);
+
+    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+    float bx = 0.f, by = 0.f, bz = 0.f;   // block sums
+    int blk = 0;                           // 8-entry blocks since the last fold
+    int selfs = 0;                         // own-leaf entries seen (COUNT)
+    long long nent = 0;                    // list entries summed
+    int rounds = 0;
+    // sum the buffered list lst[0, nl) into every lane's block sums
+    auto flush = [&](int nl) {
+        const int n8 = (nl + 7) & ~7;
+        if (lane < n8 - nl) {   // padding: massless entries far away add exactly 0
+            lst[nl + lane] = make_float4(1e18f, 0.f, 0.f, 0.f);
+            if constexpr (COUNT || SELF) lbd[nl + lane] = -1;
+        }
+        wave_lds_sync();
+        for (int q = 0; q < n8; q += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float4 e = lst[q + u];
+                const float dx = e.x - qx, dy = e.y - qy, dz = e.z - qz;
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));
+                const float yv = __builtin_amdgcn_rsqf(r2);
+                float f = e.w * (yv * yv * yv);
+                if constexpr (COUNT || SELF) {
+                    const bool own = lbd[q + u] == me;
+                    if constexpr (SELF) f = own ? 0.f : f;
+                    if constexpr (COUNT) selfs += own ? 1 : 0;
+                }
+                bx = fmaf(dx, f, bx);
+                by = fmaf(dy, f, by);
+                bz = fmaf(dz, f, bz);
+            }
+            if (++blk == CSG_FOLD_EVERY / 8) {
+                blk = 0;
+                kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
+                bx = by = bz = 0.f;
+            }
+        }
+        nent += nl;
+        wave_lds_sync();   // every lane has read the list before it is refilled
+    };
+
+    const int mode = wmode ? wmode[wv] : 0;   // walks left to take the union traversal
+    if (lane == 0) stk[0] = make_int2(0, nnodes);
+    int top = mode > 0 ? 0 : 1, nl = 0;
+    bool fell = mode > 0;
+    while (top > 0) {
+        wave_lds_sync();
+        ++rounds;
+        const int take = top < 64 ? top : 64;
+        const bool has = lane < take;
+        const int2 ch = has ? stk[top - 1 - lane] : make_int2(0, 0);
+        top -= take;
+        const WNode32 nd = nodes[ch.x];
+        const float rx = (float)(nd.cx - gx), ry = (float)(nd.cy - gy), rz = (float)(nd.cz - gz);
+        const float tx = fmaxf(fabsf(rx) - wx, 0.f);
+        const float ty = fmaxf(fabsf(ry) - wy, 0.f);
+        const float tz = fmaxf(fabsf(rz) - wz, 0.f);
+        const float dm2 = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
+        // leaves carry s2 = -1: always taken
+        const bool acc = has && nd.s2 < dm2;
+        // chains start past the node that makes them, so the traversal always ends
+        const bool opn = has && !acc && ch.x + 1 < nd.skip;
+        const bool sib = has && nd.skip > ch.x && nd.skip < ch.y;
+        const unsigned long long A = __ballot(acc), O = __ballot(opn), S = __ballot(sib);
+        wave_lds_sync();   // every lane has read its chain before the stack is written
+        if (acc) {
+            const int k = nl + lane_rank(A);
+            lst[k] = make_float4(rx, ry, rz, nd.m);
+            if constexpr (COUNT || SELF) lbd[k] = nd.body;
+        }
+        nl += __popcll(A);
+        // the rest of each chain below the children, so children come off first
+        if (sib) stk[top + lane_rank(S)] = make_int2(nd.skip, ch.y);
+        top += __popcll(S);
+        if (opn) stk[top + lane_rank(O)] = make_int2(ch.x + 1, nd.skip);
+        top += __popcll(O);
+        if (nl > WL_LCAP - 64 || top == 0) {
+            flush(nl);
+            nl = 0;
+        }
+        // a round pops at most 64 chains and pushes at most 128
+        if (top > WL_SCAP - 64 || nent > lmax) {
+            fell = true;
+            break;
+        }
+    }
+    int cnt = 0, steps = 0;
+    unsigned long long wcnt = 0;
+    if (fell) {
+        sx = sy = sz = kx = ky = kz = 0.f;
+        int nrec = 0;
+        const TapeOut none{nullptr, nullptr, 0, nullptr};
+        // counting excludes the own leaf by index, as the union walk's counting forms do
+        walk32_union<COUNT, false, false, COUNT, SELF || COUNT>(nodes, nnodes, lane, b0, act, me,
+            x, y, z, eps2f, none, nrec, cnt, wcnt, sx, sy, sz, kx, ky, kz, steps);
+        if (lane == 0) {
+            if (mode == 0 && fallbacks) atomicAdd(fallbacks, 1);
+            if (wmode) wmode[wv] = mode > 0 ? mode - 1 : sticky;
+        }
+    } else {
+        kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
+        if constexpr (COUNT) {
+            cnt = act ? (int)nent - selfs : 0;
+            int t = cnt;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            wcnt = (unsigned long long)(unsigned)t;
+        }
+    }
+    // cost in node-step units: a list entry costs about half a union step
+    if (wo.cost && lane == 0) wo.cost[wv] = (int)(fell ? steps + nent / 2 : nent / 2) + 3 * rounds;
+    if (act) {
+        ax[me] = G * ((double)sx - (double)kx);
+        ay[me] = G * ((double)sy - (double)ky);
+        az[me] = G * ((double)sz - (double)kz);
+        if constexpr (COUNT) {
+            if (icount) icount[me] = cnt;
+        }
+    }
+    if constexpr (COUNT) {
         if (lane == 0) atomicAdd(itotal, wcnt);
     }
 }

@@ -380,7 +380,7 @@ int csg_substeps_sph(csg_ctx *c, int k, double dt, double G, double softening, d
                      int precision) {
     CSG_USE(c);
     c->theta_hint = theta;
-    if (precision < 0 || precision > 2 || k < 0) return COSMOSIM_E_ARG;
+    if (!prec_valid(precision) || k < 0) return COSMOSIM_E_ARG;
     if (k == 0) return COSMOSIM_OK;
     int s;
     timer_start(c);

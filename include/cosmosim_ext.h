@@ -35,7 +35,12 @@ enum {
 enum {
     COSMOSIM_PREC_FP64_STRICT = 0, /* reference arithmetic: IEEE sqrt and divide, no FMA contraction */
     COSMOSIM_PREC_FP64 = 1,        /* FP64 with refined hardware rsqrt and FMA */
-    COSMOSIM_PREC_FP32 = 2         /* FP32 on FP64-formed node offsets, compensated accumulation */
+    COSMOSIM_PREC_FP32 = 2,        /* FP32 on FP64-formed node offsets, compensated accumulation */
+    COSMOSIM_PREC_FP32_LIST = 3    /* FP32 on wave-shared interaction lists: a node is accepted for
+                                      64 bodies at once when the opening test holds at their
+                                      bounding box (so for each body), offsets from FP64-formed
+                                      group-relative positions, compensated accumulation; not
+                                      available to the adjoint, whose walks are taped per body */
 };
 
 typedef struct {
@@ -63,6 +68,9 @@ typedef struct {
     double ms_neighbors;   /* GPU time of neighbour-query traversals since timing was enabled */
     long long neighbor_queries; /* queries covered by ms_neighbors */
     double ms_sph;         /* GPU time of SPH density + force passes since timing was enabled */
+    int list_fallbacks;    /* interaction-list walk wavefronts that fell back to per-body
+                              traversal since timing was enabled */
+    int reserved;
 } CosmosimExtStats;
 
 /* Seeded initial conditions (own generator; the reference's generators are

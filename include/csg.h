@@ -29,7 +29,8 @@ typedef struct {
     double root[4];                 /* centre x, y, z and half size of the root cube */
     double ms_build, ms_walk, ms_kdk; /* GPU time per phase since timing was (re)enabled */
     int timed_walks;                /* walk launches covered by ms_walk */
-    int pad;
+    int list_fallbacks;             /* interaction-list walk wavefronts that fell back to
+                                       the union traversal since timing was (re)enabled */
     double ms_neighbors;            /* GPU time of neighbour-query traversals (timing on) */
     long long neighbor_queries;     /* queries covered by ms_neighbors */
     double ms_sph;                  /* GPU time of SPH density + force passes (timing on) */

@@ -12,6 +12,8 @@
  *   CosmosimSubsystem.cpp:12-43, src/renderer.c:348-364 (float snapshots)
  *   src/quasar.c:11-529    (quasar step and compaction, explicit random state)
  *   tests/test_physics.c:34-71 (direct-sum and energy judges)
+ * and the core's own FP32 walk on wave-shared interaction lists (cref_list_walk,
+ * which has no counterpart in the reference).
  * Floating-point expressions are kept in the reference's evaluation order so
  * that, compiled with -O2 -ffp-contract=off on x86-64, results are bit-identical
  * to the reference (checked by tests/test_oracle.py against tests/golden/,
@@ -238,6 +240,141 @@ void cref_forces(const OctreeNode *pool, int root, Body *bodies, int n, double G
             c = cref_walk_one(pool, root, bodies, i, G, eps2, theta);
         if (counts)
             counts[i] = (int)c;
+    }
+}
+
+/* ---------------------------------------------------------------- list walk
+ * Restatement of the core's FP32 walk on wave-shared interaction lists
+ * (k_walk32l; no counterpart in the reference, whose walk is per body,
+ * src/octree.c:171-218).  The bodies are taken in groups of 64 consecutive
+ * entries of `order` (the device's wavefront order).  For each group:
+ *   - its bounding box in FP64; centre g = 0.5 (lo + hi); half extents
+ *     w = fp32(0.5 (hi - lo)) * (1 + 2^-20) in FP32;
+ *   - a depth-first walk over the tree (children in slot order) in which a
+ *     leaf is always listed and an internal node is listed when
+ *     s2 < dmin2, with s2 = fp32((2h)^2 / theta^2) (FP64 product, +inf at
+ *     theta 0) and dmin2 = fmaf(tx, tx, fmaf(ty, ty, tz * tz)),
+ *     t = fmaxf(|fp32(C - g)| - w, 0) per axis, all FP32 (the device's
+ *     arithmetic, so the listed set is the device's); otherwise it is opened;
+ *   - every body of the group sums the whole list in FP64 (exact offsets),
+ *     skipping a leaf whose body_index is its own.
+ * acc (n x 3) receives G * sum for live bodies (0 for others), counts (may be
+ * NULL) the listed entries each body summed.  checks (may be NULL) receives
+ * [0] the largest |listed mass - live mass| / live mass over the groups (every
+ * body must be covered exactly once) and [1] the number of listed internal
+ * nodes that fail the reference criterion (2h)^2 / d^2 < theta^2 (FP64) for
+ * some body of their group by more than 1e-5 relative. */
+void cref_list_walk(const OctreeNode *pool, int root, const Body *bodies, int n,
+                    const int *order, int n_order, double G, double softening, double theta,
+                    double *acc, int *counts, double *checks)
+{
+    const double eps2 = softening * softening;
+    const double inv_th2 = theta > 0.0 ? 1.0 / (theta * theta) : HUGE_VAL;
+    double live_mass = 0.0;
+    for (int i = 0; i < n; ++i)
+        if (cref_live(&bodies[i]))
+            live_mass += bodies[i].mass;
+    memset(acc, 0, sizeof(double) * 3 * (size_t)n);
+    if (counts)
+        memset(counts, 0, sizeof(int) * (size_t)n);
+    double worst_mass = 0.0;
+    long bad = 0;
+    const int ngroups = (n_order + 63) / 64;
+#pragma omp parallel for schedule(dynamic, 4) reduction(max : worst_mass) reduction(+ : bad)
+    for (int g = 0; g < ngroups; ++g) {
+        const int b0 = 64 * g, b1 = b0 + 64 < n_order ? b0 + 64 : n_order;
+        double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        for (int j = b0; j < b1; ++j) {
+            const Body *b = &bodies[order[j]];
+            const double p[3] = {b->x, b->y, b->z};
+            for (int a = 0; a < 3; ++a) {
+                if (p[a] < lo[a]) lo[a] = p[a];
+                if (p[a] > hi[a]) hi[a] = p[a];
+            }
+        }
+        double gc[3];
+        float w[3];
+        for (int a = 0; a < 3; ++a) {
+            gc[a] = 0.5 * (lo[a] + hi[a]);
+            w[a] = (float)(0.5 * (hi[a] - lo[a])) * (1.0f + 0x1p-20f);
+        }
+        /* the list: node ids, by an explicit-stack depth-first walk */
+        int cap = 1024, len = 0;
+        int *list = malloc(sizeof(int) * (size_t)cap);
+        int stack[8 * 70 + 8];
+        int top = 0;
+        stack[top++] = root;
+        double listed = 0.0;
+        while (top > 0) {
+            const int id = stack[--top];
+            const OctreeNode *nd = &pool[id];
+            int take = nd->body_index >= 0;
+            if (!take) {
+                const float r[3] = {(float)(nd->cx - gc[0]), (float)(nd->cy - gc[1]),
+                                    (float)(nd->cz - gc[2])};
+                float t[3];
+                for (int a = 0; a < 3; ++a) {
+                    const float u = fabsf(r[a]) - w[a];
+                    t[a] = u > 0.0f ? u : 0.0f;
+                }
+                const float dm2 = fmaf(t[0], t[0], fmaf(t[1], t[1], t[2] * t[2]));
+                const double s = 2.0 * nd->half_size;
+                const float s2 = (float)(s * s * inv_th2);
+                take = s2 < dm2;
+                if (take) {
+                    for (int j = b0; j < b1; ++j) {
+                        const Body *b = &bodies[order[j]];
+                        const double dx = nd->cx - b->x, dy = nd->cy - b->y, dz = nd->cz - b->z;
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        if (!(s * s < theta * theta * d2 * (1.0 + 1e-5)))
+                            ++bad;
+                    }
+                }
+            }
+            if (take) {
+                if (len == cap) {
+                    cap *= 2;
+                    list = realloc(list, sizeof(int) * (size_t)cap);
+                }
+                list[len++] = id;
+                listed += nd->total_mass;
+                continue;
+            }
+            for (int c = 7; c >= 0; --c)
+                if (nd->children[c] >= 0)
+                    stack[top++] = nd->children[c];
+        }
+        const double dm = fabs(listed - live_mass) / live_mass;
+        if (dm > worst_mass)
+            worst_mass = dm;
+        for (int j = b0; j < b1; ++j) {
+            const int i = order[j];
+            const Body *b = &bodies[i];
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            int cnt = 0;
+            for (int k = 0; k < len; ++k) {
+                const OctreeNode *nd = &pool[list[k]];
+                if (nd->body_index == i)
+                    continue;
+                const double dx = nd->cx - b->x, dy = nd->cy - b->y, dz = nd->cz - b->z;
+                const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                const double f = nd->total_mass / (r2 * sqrt(r2));
+                sx += f * dx;
+                sy += f * dy;
+                sz += f * dz;
+                ++cnt;
+            }
+            acc[3 * i] = G * sx;
+            acc[3 * i + 1] = G * sy;
+            acc[3 * i + 2] = G * sz;
+            if (counts)
+                counts[i] = cnt;
+        }
+        free(list);
+    }
+    if (checks) {
+        checks[0] = worst_mass;
+        checks[1] = (double)bad;
     }
 }
 

@@ -107,6 +107,7 @@ class CpuRef(_Base):
         self.lib = L
         L.cref_build.argtypes = [_VP, _VP, _VP, _I]
         L.cref_forces.argtypes = [_VP, _I, _VP, _I, _D, _D, _D, _VP]
+        L.cref_list_walk.argtypes = [_VP, _I, _VP, _I, _VP, _I, _D, _D, _D, _VP, _VP, _VP]
         L.cref_init_acc.argtypes = [_VP, _I, _D, _D, _D, _VP]
         L.cref_step.argtypes = [_VP, _I, _D, _D, _D, _D, _VP]
         L.cref_kick_drift.argtypes = [_VP, _I, _D]
@@ -139,6 +140,19 @@ class CpuRef(_Base):
         self.lib.cref_forces(_p(pool), root, _p(bodies), len(bodies), _D(G), _D(softening),
                              _D(theta), _p(counts))
         return counts
+
+    def list_walk(self, pool, bodies, order, G, softening, theta, root=0):
+        """The core's interaction-list walk restated (cref_list_walk): groups of 64
+        consecutive bodies of `order`.  Returns (acc (n, 3), per-body list counts,
+        worst relative mass-coverage error of a group's list, number of listed
+        internal nodes that fail the per-body criterion)."""
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        acc = np.zeros((len(bodies), 3))
+        counts = np.zeros(len(bodies), dtype=np.int32)
+        checks = np.zeros(2)
+        self.lib.cref_list_walk(_p(pool), root, _p(bodies), len(bodies), _p(order), len(order),
+                                _D(G), _D(softening), _D(theta), _p(acc), _p(counts), _p(checks))
+        return acc, counts, float(checks[0]), int(checks[1])
 
     def _step_impl(self, bodies, dt, G, softening, theta, pool):
         self.lib.cref_step(_p(bodies), len(bodies), _D(dt), _D(G), _D(softening), _D(theta),
