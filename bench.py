@@ -5,9 +5,13 @@
 A step is one full KDK substep on the device (kick+drift, tree build, walk,
 kick) of the seeded 1,000,000-body two-disk merger (seed 3, eps 0.005,
 dt 5e-4), with every input resident in HBM before the clock starts.  The
-headline value is the FP32 walk at theta 0.5; the FP64 walks (strict reference
-arithmetic and refined-rsqrt FP64) are timed alongside at each theta.  Prints
-one JSON line.
+headline value is the FP32 walk on wave-shared interaction lists at theta 0.5
+(each wavefront's 64 bodies share one list of the cells that pass the opening
+test at their bounding box, which is at least the reference test for each of
+them); the FP32 walk with the reference's per-body test and the FP64 walks
+(strict reference arithmetic and refined-rsqrt FP64) are timed alongside at
+each theta, every walk with its error against the FP64 direct sum.  Prints one
+JSON line.
 """
 import argparse
 import json
@@ -69,29 +73,35 @@ def timed_run(ca, bodies, theta, precision, steps, warmup, energy=False):
 
 
 def accuracy(ca, sim, idx):
-    """Three errors on one state: FP32 vs FP64 on the same tree (precision),
-    FP64 vs FP64 direct sum (method), FP32 vs direct sum."""
+    """Errors on one state: the per-body FP32 walk vs FP64 on the same tree
+    (precision), FP64 vs FP64 direct sum (method), each FP32 walk vs the direct
+    sum, and the list walk vs the per-body FP64 walk (the two opening rules)."""
     sim.set_params(precision="fp64_strict")
     sim.compute_forces()
     a64 = sim.accelerations()
     sim.set_params(precision="fp32")
     sim.compute_forces()
     a32 = sim.accelerations()
+    sim.set_params(precision="fp32_list")
+    sim.compute_forces()
+    al = sim.accelerations()
     direct = sim.direct_forces(idx)
     r = {}
     r["fp32_vs_fp64_rms"], r["fp32_vs_fp64_max"] = rel_err(a32[idx], a64[idx])
     r["fp32_vs_fp64_rms_all"], r["fp32_vs_fp64_max_all"] = rel_err(a32, a64)
     r["fp64_vs_direct_rms"], r["fp64_vs_direct_max"] = rel_err(a64[idx], direct)
     r["fp32_vs_direct_rms"], r["fp32_vs_direct_max"] = rel_err(a32[idx], direct)
+    r["fp32_list_vs_direct_rms"], r["fp32_list_vs_direct_max"] = rel_err(al[idx], direct)
+    r["fp32_list_vs_fp64_rms_all"], r["fp32_list_vs_fp64_max_all"] = rel_err(al, a64)
     return r
 
 
-def per_step_accuracy(ca, bodies, idx, theta, steps):
+def per_step_accuracy(ca, bodies, idx, theta, steps, precision="fp32_list"):
     """FP32 substeps of the 1M merger (untimed): after every substep, the FP32
     walk's accelerations of the 4096-body subset against the FP64 direct sum on
     the same state (error (c)); errors (a) and (b) are taken at the first and
     last state by accuracy()."""
-    sim = ca.Simulation(bodies, softening=EPS, theta=theta, dt=DT, precision="fp32")
+    sim = ca.Simulation(bodies, softening=EPS, theta=theta, dt=DT, precision=precision)
     rms, mx = [], []
     for _ in range(steps):
         sim.advance(1)
@@ -119,7 +129,7 @@ def config_line(ca, recipe, n, seed, theta, eps, dt, steps, precision):
     ke1, pe1, p1 = sim.energy()
     sim.close()
     walk_ms = st.ms_walk / max(st.timed_walks, 1)
-    peak = FP32_PEAK_TFLOPS if precision == "fp32" else FP64_PEAK_TFLOPS
+    peak = FP32_PEAK_TFLOPS if precision.startswith("fp32") else FP64_PEAK_TFLOPS
     tflops = st.interactions * FLOP_PER_INTERACTION / (walk_ms * 1e-3) / 1e12
     return {"bodies": n, "substeps": steps, "precision": precision,
             "substeps_per_s": steps / (t1 - t0), "walk_ms": walk_ms,
@@ -385,7 +395,7 @@ def copy_calibration(nbytes=1 << 30, launches=5):
 
 
 def load_traffic():
-    p = os.path.join(ROOT, "profiles", "walk_fp32_1M_merger_traffic.json")
+    p = os.path.join(ROOT, "profiles", "walk_fp32_list_1M_merger_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -441,16 +451,17 @@ def main():
     thetas = [float(t) for t in args.thetas.split(",")]
     if 0.5 not in thetas:
         thetas.append(0.5)
-    precisions = ["fp32"] if args.headline_only else ["fp32", "fp64", "fp64_strict"]
+    precisions = (["fp32_list"] if args.headline_only
+                  else ["fp32_list", "fp32", "fp64", "fp64_strict"])
 
     results = {}
     head = None
     for th in thetas:
         for prec in precisions:
-            headline = (th == 0.5 and prec == "fp32")
+            headline = (th == 0.5 and prec == "fp32_list")
             sim, r = timed_run(ca, bodies, th, prec, args.steps, args.warmup,
                                energy=(not args.no_energy and th == 0.5 and prec != "fp64"))
-            if prec == "fp32":
+            if prec == "fp32_list":
                 results[f"theta{th}_accuracy_end"] = accuracy(ca, sim, idx)
             sim.close()
             results[f"theta{th}_{prec}"] = r
@@ -461,9 +472,9 @@ def main():
         s0.close()
 
     if not args.no_per_step:
-        results["per_step"] = {f"theta{th}": per_step_accuracy(ca, bodies, idx, th,
-                                                               args.per_step_steps)
-                               for th in thetas}
+        results["per_step"] = {f"theta{th}_{p}": per_step_accuracy(ca, bodies, idx, th,
+                                                                    args.per_step_steps, p)
+                               for th in thetas for p in ("fp32_list", "fp32")}
     if not args.headline_only and not args.no_configs:
         results["config0_20001_disk"] = config_line(ca, ca.IC_DISK, 20001, 1, 0.5, 0.01, 1e-3,
                                                     100, "fp64_strict")
@@ -487,12 +498,14 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "1M-body two-disk merger (seed 3), theta 0.5, eps 0.005, dt 5e-4, "
-                        "FP32 walk on FP64-formed offsets, FP64 state and KDK",
+                        "FP32 walk on wave-shared interaction lists (opening test at each "
+                        "wavefront's 64-body bounding box), offsets from FP64-formed "
+                        "group-relative positions, FP64 state and KDK",
             "bodies": N_BODIES, "theta": 0.5, "softening": EPS, "dt": DT,
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "k_walk32w",
+            "kernel": "k_walk32l",
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
@@ -520,8 +533,12 @@ def main():
     line["hbm_total_bytes"] = int(total_b)
     fp64 = results.get("theta0.5_fp64")
     strict = results.get("theta0.5_fp64_strict")
+    per_body = results.get("theta0.5_fp32")
     if fp64:
         line["fp32_speedup_vs_fp64"] = round(head["substeps_per_s"] / fp64["substeps_per_s"], 3)
+        if per_body:
+            line["fp32_per_body_speedup_vs_fp64"] = round(
+                per_body["substeps_per_s"] / fp64["substeps_per_s"], 3)
     if strict:
         line["fp32_speedup_vs_fp64_strict"] = round(
             head["substeps_per_s"] / strict["substeps_per_s"], 3)

@@ -2450,7 +2450,8 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
     const int nnodes = bp->n_nodes;
     if (bp->overflow || bp->order_bad) return;   // a failed build: no walk
     const int lane = threadIdx.x & 63;
-    const int wib = threadIdx.x >> 6;
+    // the wavefront's index in its block, as a wave-uniform (scalar) value
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     int2 *const stk = stk_all[wib];
     float4 *const lst = lst_all[wib];
     int *const lbd = lbd_all[wib];
@@ -2528,7 +2529,8 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
         wave_lds_sync();   // every lane has read the list before it is refilled
     };
 
-    const int mode = wmode ? wmode[wv] : 0;   // walks left to take the union traversal
+    // walks left to take the union traversal
+    const int mode = wmode ? __builtin_amdgcn_readfirstlane(wmode[wv]) : 0;
     if (lane == 0) stk[0] = make_int2(0, nnodes);
     int top = mode > 0 ? 0 : 1, nl = 0;
     bool fell = mode > 0;
