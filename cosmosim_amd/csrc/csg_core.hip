@@ -113,7 +113,7 @@ struct csg_ctx {
     // the kick-drift before the next walk already wrote its dispatch order (for these
     // waves per block, slots per XCD and waves), zeroed the dead bodies'
     // accelerations and the interaction total
-    int order_ready = 0, order_gw = 0, order_capg = 0, order_nw = 0;
+    int order_ready = 0, order_gw = 0, order_capg = 0, order_nw = 0, order_split = 0;
     int counts_valid = 0;   // icount holds the per-body counts of the last walk
     int nodes64_valid = 0;  // the FP64 node records belong to the current tree
     double4 *xyzm = nullptr;  // build scratch: (x, y, z, m) interleaved, body order
@@ -147,9 +147,13 @@ struct csg_ctx {
                                // 1M merger at theta 0.5: 1.35 / 1.29 / 1.31 ms walk)
     int *wl_fall = nullptr;    // list walks: waves that fell back (device counter)
     int *wmode = nullptr;      // list walks: per wave, walks left on the union traversal
-    int list_sticky = 8;       // walks a wave that fell back stays on the union traversal
+    int list_wave_split = 1;   // sub-ranges per XCD range of the list walk's dispatch order
+                               // (1M merger, theta 0.5: 1.27 / 1.33 / 1.40 ms for 1 / 2 / 3)
+    int list_sticky = 64;      // walks a wave that fell back stays on the union traversal
                                // (0: every walk tries the list; results then depend only
-                               // on the state, not on the walks before)
+                               // on the state, not on the walks before).  1M merger over
+                               // 100 substeps: 1.25 / 1.19 / 1.17 / 1.16 ms walk for
+                               // 8 / 16 / 32 / 64
 };
 
 #define CSG_CHECK(expr)                                                                      \
@@ -395,6 +399,10 @@ int csg_create(int device, int capacity, csg_ctx **out) {
     }
     if (const char *g = getenv("COSMOSIM_LIST_MAX")) c->list_max = atoi(g);
     if (const char *g = getenv("COSMOSIM_LIST_STICKY")) c->list_sticky = atoi(g) > 0 ? atoi(g) : 0;
+    if (const char *g = getenv("COSMOSIM_LIST_WAVE_SPLIT")) {
+        const int v = atoi(g);
+        if (v >= 1 && v <= WO_MAX_SPLIT) c->list_wave_split = v;
+    }
     if (const char *g = getenv("COSMOSIM_LIST_BLOCK_WAVES")) {
         const int v = atoi(g);
         if (v == 1 || v == 2 || v == 4) c->list_block_waves = v;
@@ -663,7 +671,7 @@ static int bbox_blocks(int n) {
 // in longest-first order from the previous walk's step counts, and then its waves
 // per block and slots per XCD range.
 static bool walk_order_params(const csg_ctx *c, int precision, bool taped, int nw, int *gw,
-                              int *capg) {
+                              int *capg, int *split) {
     const bool wave_mask32 = precision == COSMOSIM_PREC_FP32 && (c->walk_group32 == 0 || taped);
     const bool grouped32 = precision == COSMOSIM_PREC_FP32 && !wave_mask32;
     if (grouped32 || !c->wave_order || c->wcost_n != nw) return false;
@@ -673,6 +681,7 @@ static bool walk_order_params(const csg_ctx *c, int precision, bool taped, int n
     // independent wavefronts.
     if (precision == COSMOSIM_PREC_FP32_LIST) *gw = c->list_block_waves;
     else *gw = wave_mask32 ? c->walk_block_waves : min(c->walk_block_waves, CSG_TPB / 64);
+    *split = precision == COSMOSIM_PREC_FP32_LIST ? c->list_wave_split : c->wave_split;
     *capg = WO_CAP(blocks_for(nw, *gw));
     return true;
 }
@@ -687,15 +696,16 @@ static int launch_kick_drift(csg_ctx *c, double dt, int kicks, int *bbox = nullp
     if (bbox) *bbox = 0;
     if (c->n == 0) return COSMOSIM_OK;
     const int nw = blocks_for(c->n, 64);
-    int gw = 1, capg = 0;
-    if (bbox && next_prec >= 0 && walk_order_params(c, next_prec, false, nw, &gw, &capg)) {
+    int gw = 1, capg = 0, split = 1;
+    if (bbox && next_prec >= 0 && walk_order_params(c, next_prec, false, nw, &gw, &capg, &split)) {
         const int nkd = min(blocks_for(c->n, 1024), CSG_BBOX_BLOCKS);
         *bbox = nkd;
         k_kick_drift_order<<<WO_XCDS + nkd, 1024, 0, c->stream>>>(c->x, c->y, c->z, c->vx,
             c->vy, c->vz, c->ax, c->ay, c->az, c->m, c->n, dt, kicks, c->bbox_part, c->icount,
-            c->itotal, c->wcost, nw, gw, c->wave_split, capg, c->worder);
+            c->itotal, c->wcost, nw, gw, split, capg, c->worder);
         c->order_ready = 1;
         c->order_gw = gw;
+        c->order_split = split;
         c->order_capg = capg;
         c->order_nw = nw;
     } else if (bbox) {
@@ -903,9 +913,11 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     WaveOrder wo{nullptr, nw, grouped32 ? nullptr : c->wcost, 0};
     int grid = nb8, tpb = CSG_TPB;
     int gw = 1, capg = 0;
-    if (walk_order_params(c, precision, tape.rec != nullptr, nw, &gw, &capg)) {
-        if (!(pre && c->order_gw == gw && c->order_capg == capg && c->order_nw == nw))
-            k_wave_order<<<WO_XCDS, 1024, 0, st>>>(c->wcost, nw, gw, c->wave_split, capg,
+    int split = 1;
+    if (walk_order_params(c, precision, tape.rec != nullptr, nw, &gw, &capg, &split)) {
+        if (!(pre && c->order_gw == gw && c->order_capg == capg && c->order_nw == nw &&
+              c->order_split == split))
+            k_wave_order<<<WO_XCDS, 1024, 0, st>>>(c->wcost, nw, gw, split, capg,
                                                    c->worder);
         wo.order = c->worder;
         wo.slots = WO_XCDS * capg * gw;
