@@ -81,6 +81,70 @@ def test_list_walk_matches_oracle(theta):
     assert np.sqrt(np.mean(e ** 2)) < TOL_RMS and e.max() < TOL_MAX
 
 
+def _cluster(seed, n=3000):
+    """A collision run (300 bodies within 1e-9), exact duplicates (a depth-64 leaf) and
+    dead bodies."""
+    rng = np.random.default_rng(seed)
+    pos = rng.normal(size=(n, 3)) * np.array([2.0, 2.0, 0.2])
+    pos[:300] = pos[0] + rng.normal(size=(300, 3)) * 1e-9
+    pos[300:310] = pos[311]
+    b = ca.make_bodies(pos, vel=np.zeros((n, 3)), mass=rng.uniform(0.1, 3.0, n))
+    b["mass"][::41] = 0.0
+    return b
+
+
+def _uniform(n, seed):
+    rng = np.random.default_rng(seed)
+    return ca.make_bodies(rng.uniform(-1, 1, size=(n, 3)), vel=np.zeros((n, 3)),
+                          mass=rng.uniform(0.5, 1.5, n))
+
+
+EDGE = {
+    "cluster": lambda: _cluster(1),
+    "n1": lambda: _uniform(1, 3),
+    "n2": lambda: _uniform(2, 4),
+    "n65": lambda: _uniform(65, 5),
+    "uniform_4096": lambda: _uniform(4096, 2),
+}
+
+
+# Each offset carries an absolute error of up to 2^-24 (|C - g| + |x - g|), set by
+# the group's extent, not by the pair's distance.  In the cluster case 300 heavy
+# bodies (masses to 3) lie within 1e-9 of each other, far below the softening
+# length 0.01, and a group that holds some of them and some bodies 0.1 away puts
+# ~6e-9 of error on offsets whose true size is 1e-9: each such pair's term
+# G m dx / eps^3 is then off by G m 6e-9 / eps^3 ~ 1e-2 (the per-body walks form
+# each pair's offset in FP64 and do not have this).
+EDGE_TOL = {"cluster": (1e-3, 1e-2)}   # measured 3.3e-4 / 3.9e-3
+
+
+@pytest.mark.parametrize("case", list(EDGE))
+def test_list_walk_edge_inputs(case):
+    b = EDGE[case]()
+    a, counts, st, ra, rc, mass_err, bad = _against_oracle(b, 0.01, 0.5)
+    live = ~(b["mass"] <= 0)
+    assert st.list_fallbacks == 0 and mass_err < 1e-12 and bad == 0
+    assert np.array_equal(counts, rc)
+    assert np.all(a[~live] == 0.0)
+    if live.sum() > 1:
+        e = rel_err(a[live], ra[live])
+        rms = np.sqrt(np.mean(e ** 2))
+        print(f"\n{case}: FP32 vs oracle rms {rms:.2e} max {e.max():.2e}")
+        tol_rms, tol_max = EDGE_TOL.get(case, (TOL_RMS, TOL_MAX))
+        assert rms < tol_rms and e.max() < tol_max
+
+
+def test_list_walk_all_dead_and_empty():
+    b = _uniform(100, 6)
+    b["mass"][:] = 0.0
+    sim = ca.Simulation(b, softening=0.01, theta=0.5, precision="fp32_list")
+    sim.compute_forces()
+    assert np.all(sim.accelerations() == 0.0)
+    assert sim.stats().interactions == 0
+    sim.advance(2)
+    sim.close()
+
+
 def test_list_walk_theta0_is_direct_sum():
     n = 3000
     b = ca.generate(ca.IC_DISK, n, 5)
