@@ -769,7 +769,7 @@ static int launch_build(csg_ctx *c, bool with64, int bbox, int order) {
         }
         CSG_LAUNCHED();
         // an even number of passes: the sorted pairs are back in key / idx
-        k_runs<<<256, CSG_TPB, 0, st>>>(c->key, c->idx, c->x, c->y, c->z, c->bp, c->ext,
+        k_runs<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->key, c->idx, c->x, c->y, c->z, c->bp, c->ext,
                                         c->idx2, c->ext_tmp);
         CSG_LAUNCHED();
     }

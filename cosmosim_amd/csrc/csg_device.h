@@ -816,7 +816,9 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_sweep(
 // by (those levels, body index); ties keep index order.  Block j scans slice j
 // of the sorted keys in chunks of CSG_TPB for the first members of runs and
 // fixes each run it finds with the whole block (a run may extend past the
-// slice; it belongs to the slice of its first member).
+// slice; it belongs to the slice of its first member).  Launched with one block
+// per CSG_TPB bodies, so a slice is one chunk and blocks without a run start end
+// after one look at their keys.
 __global__ __launch_bounds__(CSG_TPB) void k_runs(
     const uint64_t *__restrict__ key, int *__restrict__ idx, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, BuildParams *__restrict__ bp,
@@ -825,7 +827,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_runs(
     __shared__ int s_start[CSG_TPB];
     const int nl = bp->n_live;
     double root[4] = {bp->root[0], bp->root[1], bp->root[2], bp->root[3]};
-    const int per = (nl + gridDim.x - 1) / gridDim.x;
+    const int per = max(CSG_TPB, (nl + (int)gridDim.x - 1) / (int)gridDim.x);
     const int lo = blockIdx.x * per, hi = min(nl, lo + per);
     int found = 0;
     for (int c0 = lo; c0 < hi; c0 += CSG_TPB) {
