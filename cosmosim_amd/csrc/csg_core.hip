@@ -842,21 +842,20 @@ static void launch_walk32w(csg_ctx *c, const int *sidx, double G, double eps2, d
         tape, wo);
 }
 
-// v: bit 1 = counts (per-body when lane_counts, and the total), bit 0 = own leaf
-// excluded by index
+// cnt: 0 no counts, 1 the total only (from the list lengths), 2 per-body counts
+// and the total; selfx: the own leaf excluded by index
 template <int WPB>
-static void launch_walk32l(csg_ctx *c, int v, const int *sidx, double G, double eps2, int grid,
-                           bool lane_counts, WaveOrder wo) {
-    int *ic = lane_counts ? c->icount : nullptr;
+static void launch_walk32l(csg_ctx *c, int cnt, bool selfx, const int *sidx, const int *spos,
+                           double G, double eps2, int grid, WaveOrder wo) {
+    int *ic = cnt == 2 ? c->icount : nullptr;
 #define CSG_WL(C, S)                                                                         \
     k_walk32l<C, S, WPB><<<grid, 64 * WPB, 0, c->stream>>>(c->nodes32, c->bp, sidx, c->x, c->y, \
         c->z, c->ax, c->ay, c->az, ic, c->itotal, c->wl_fall, c->list_sticky ? c->wmode : nullptr, \
-        c->list_sticky, G, eps2, c->list_max, wo)
-    switch (v) {
-    case 3: CSG_WL(true, true); break;
-    case 2: CSG_WL(true, false); break;
-    case 1: CSG_WL(false, true); break;
-    default: CSG_WL(false, false); break;
+        c->list_sticky, spos, c->lcp, G, eps2, c->list_max, wo)
+    if (selfx) {
+        if (cnt == 2) CSG_WL(2, true); else if (cnt == 1) CSG_WL(1, true); else CSG_WL(0, true);
+    } else {
+        if (cnt == 2) CSG_WL(2, false); else if (cnt == 1) CSG_WL(1, false); else CSG_WL(0, false);
     }
 #undef CSG_WL
 }
@@ -885,8 +884,13 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     hipStream_t st = c->stream;
     const double eps2 = softening * softening, th2 = theta * theta;
     c->theta_hint = theta;
-    if (prec_rec32(precision) && c->tree_valid && c->th32 != theta) {
-        // the FP32 records carry the opening angle: emit them again for this one
+    if (prec_rec32(precision) && c->tree_valid &&
+        (c->th32 != theta ||
+         (precision == COSMOSIM_PREC_FP32_LIST && hilbert && hilbert_order_for(c, precision) &&
+          !c->widx_valid))) {
+        // the FP32 records carry the opening angle: emit them again for this one;
+        // the list walk's groups also need the Peano-Hilbert order, which a build
+        // with no walk in view computes only from hilbert_min bodies
         const int s = launch_build(c, c->nodes64_valid != 0, 0, hilbert_order_for(c, precision));
         if (s) return s;
     }
@@ -928,15 +932,19 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     if (precision == COSMOSIM_PREC_FP32_LIST) {
         // wave-shared interaction lists over groups of 64 bodies consecutive in
         // Peano-Hilbert order, gw independent wavefronts per workgroup
-        const int *sidx_l = (hilbert && c->widx_valid) ? c->widx : c->idx;
+        const bool hil = hilbert && c->widx_valid;
+        const int *sidx_l = hil ? c->widx : c->idx;
         const int wpb = wo.order ? gw : c->list_block_waves;
         if (!wo.order) grid = blocks_for(nw, wpb);
         const bool selfx = !((float)eps2 >= 1e-12f);
-        const int v = ((lane_counts || count_total) ? 2 : 0) | (selfx ? 1 : 0);
+        // the total alone comes from the list lengths where every own leaf is listed
+        const bool own_listed = th2 < (1.0 / 3.0) * (1.0 - 1e-5);
+        const int cnt = lane_counts ? 2 : (count_total ? (own_listed ? 1 : 2) : 0);
+        const int *spos = hil ? c->hpos : nullptr;
         switch (wpb) {
-        case 2: launch_walk32l<2>(c, v, sidx_l, G, eps2, grid, lane_counts, wo); break;
-        case 4: launch_walk32l<4>(c, v, sidx_l, G, eps2, grid, lane_counts, wo); break;
-        default: launch_walk32l<1>(c, v, sidx_l, G, eps2, grid, lane_counts, wo); break;
+        case 2: launch_walk32l<2>(c, cnt, selfx, sidx_l, spos, G, eps2, grid, wo); break;
+        case 4: launch_walk32l<4>(c, cnt, selfx, sidx_l, spos, G, eps2, grid, wo); break;
+        default: launch_walk32l<1>(c, cnt, selfx, sidx_l, spos, G, eps2, grid, wo); break;
         }
         CSG_LAUNCHED();
         c->counts_valid = lane_counts ? 1 : 0;

@@ -2412,8 +2412,13 @@ __global__ __launch_bounds__(1024) void k_walk32w(
 // from one substep to the next), then tries the list again: each walk is then a
 // deterministic function of the state and of the walks since the bodies were
 // set.  Both traversals apply at least the reference criterion to every body.
-//   COUNT: per-body interaction counts into icount (when not null) and the
-//          wave's total into *itotal; the own leaf is not counted;
+//   CNT:   2: per-body interaction counts into icount (when not null) and the
+//          wave's total into *itotal, the own leaf not counted (by comparing
+//          every entry's leaf body with the lane's own); 1: only the total, from
+//          the list length (only for theta^2 < 1/3, where a body's own leaf is
+//          always listed; spos maps the wave order to sorted positions (null:
+//          the same) and L is the common-prefix array, for the bodies merged
+//          into another's depth-64 leaf); 0: no counts;
 //   SELF:  exclude the own leaf by index (softening below the FP32 range).
 //          Without it the own leaf's offset is exactly 0 and its term is 0.
 // WPB independent wavefronts per workgroup, each on its own LDS slice; wave slot
@@ -2437,17 +2442,18 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <bool COUNT, bool SELF, int WPB>
+template <int CNT, bool SELF, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_walk32l(
     const WNode32 *__restrict__ nodes, const BuildParams *__restrict__ bp,
     const int *__restrict__ sidx, const double *__restrict__ px, const double *__restrict__ py,
     const double *__restrict__ pz, double *__restrict__ ax, double *__restrict__ ay,
     double *__restrict__ az, int *__restrict__ icount, unsigned long long *__restrict__ itotal,
-    int *__restrict__ fallbacks, int *__restrict__ wmode, int sticky, double G, double eps2,
-    int lmax, WaveOrder wo) {
+    int *__restrict__ fallbacks, int *__restrict__ wmode, int sticky, const int *__restrict__ spos,
+    const signed char *__restrict__ L, double G, double eps2, int lmax, WaveOrder wo) {
+    constexpr bool FULL = CNT == 2, TOT = CNT >= 1;
     __shared__ int2 stk_all[WPB][WL_SCAP];
     __shared__ float4 lst_all[WPB][WL_LCAP];
-    __shared__ int lbd_all[WPB][COUNT || SELF ? WL_LCAP : 1];
+    __shared__ int lbd_all[WPB][FULL || SELF ? WL_LCAP : 1];
     const int n_live = bp->n_live;
     const int nnodes = bp->n_nodes;
     if (bp->overflow || bp->order_bad) return;   // a failed build: no walk
@@ -2493,7 +2499,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     float bx = 0.f, by = 0.f, bz = 0.f;   // block sums
     int blk = 0;                           // 8-entry blocks since the last fold
-    int selfs = 0;                         // own-leaf entries seen (COUNT)
+    int selfs = 0;                         // own-leaf entries seen (FULL)
     long long nent = 0;                    // list entries summed
     int rounds = 0;
     // sum the buffered list lst[0, nl) into every lane's block sums
@@ -2501,7 +2507,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
         const int n8 = (nl + 7) & ~7;
         if (lane < n8 - nl) {   // padding: massless entries far away add exactly 0
             lst[nl + lane] = make_float4(1e18f, 0.f, 0.f, 0.f);
-            if constexpr (COUNT || SELF) lbd[nl + lane] = -1;
+            if constexpr (FULL || SELF) lbd[nl + lane] = -1;
         }
         wave_lds_sync();
         for (int q = 0; q < n8; q += 8) {
@@ -2512,10 +2518,10 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
                 const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));
                 const float yv = __builtin_amdgcn_rsqf(r2);
                 float f = e.w * (yv * yv * yv);
-                if constexpr (COUNT || SELF) {
+                if constexpr (FULL || SELF) {
                     const bool own = lbd[q + u] == me;
                     if constexpr (SELF) f = own ? 0.f : f;
-                    if constexpr (COUNT) selfs += own ? 1 : 0;
+                    if constexpr (FULL) selfs += own ? 1 : 0;
                 }
                 bx = fmaf(dx, f, bx);
                 by = fmaf(dy, f, by);
@@ -2559,7 +2565,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
         if (acc) {
             const int k = nl + lane_rank(A);
             lst[k] = make_float4(rx, ry, rz, nd.m);
-            if constexpr (COUNT || SELF) lbd[k] = nd.body;
+            if constexpr (FULL || SELF) lbd[k] = nd.body;
         }
         nl += __popcll(A);
         // the rest of each chain below the children, so children come off first
@@ -2584,7 +2590,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
         int nrec = 0;
         const TapeOut none{nullptr, nullptr, 0, nullptr};
         // counting excludes the own leaf by index, as the union walk's counting forms do
-        walk32_union<COUNT, false, false, COUNT, SELF || COUNT>(nodes, nnodes, lane, b0, act, me,
+        walk32_union<FULL, false, false, TOT, SELF || TOT>(nodes, nnodes, lane, b0, act, me,
             x, y, z, eps2f, none, nrec, cnt, wcnt, sx, sy, sz, kx, ky, kz, steps);
         if (lane == 0) {
             if (mode == 0 && fallbacks) atomicAdd(fallbacks, 1);
@@ -2592,12 +2598,21 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
         }
     } else {
         kahan_add(sx, kx, bx); kahan_add(sy, ky, by); kahan_add(sz, kz, bz);
-        if constexpr (COUNT) {
+        if constexpr (FULL) {
             cnt = act ? (int)nent - selfs : 0;
             int t = cnt;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
             wcnt = (unsigned long long)(unsigned)t;
+        } else if constexpr (TOT) {
+            // every lane sums every entry but its own leaf, and its own leaf is
+            // listed (theta^2 < 1/3: no cell holding a body of the group passes the
+            // group test), unless the body merged into another's depth-64 leaf
+            const int pos = act ? (spos ? spos[b] : b) : 0;
+            const bool merged = act && pos > 0 && L[pos - 1] >= 64;
+            const int nact = __popcll(__ballot(act)), nmerged = __popcll(__ballot(merged));
+            wcnt = (unsigned long long)nact * (unsigned long long)nent -
+                   (unsigned long long)(nact - nmerged);
         }
     }
     // cost in node-step units: a list entry costs about half a union step
@@ -2606,11 +2621,11 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
         ax[me] = G * ((double)sx - (double)kx);
         ay[me] = G * ((double)sy - (double)ky);
         az[me] = G * ((double)sz - (double)kz);
-        if constexpr (COUNT) {
+        if constexpr (FULL) {
             if (icount) icount[me] = cnt;
         }
     }
-    if constexpr (COUNT) {
+    if constexpr (TOT) {
         if (lane == 0) atomicAdd(itotal, wcnt);
     }
 }

@@ -134,6 +134,25 @@ def test_list_walk_edge_inputs(case):
         assert rms < tol_rms and e.max() < tol_max
 
 
+@pytest.mark.parametrize("case,theta", [("merger", 0.3), ("merger", 0.5), ("cluster", 0.5),
+                                        ("merger", 0.7)])
+def test_total_from_list_lengths(case, theta):
+    # the last walk of a substep run keeps only the interaction total; below
+    # theta^2 = 1/3 it is taken from the list lengths (every own leaf is listed,
+    # bodies merged into another's depth-64 leaf count it), above from per-entry
+    # comparisons.  Either way it equals the per-body counts of a counting walk
+    # on the same state, whose accelerations it matches bit for bit
+    b = ca.generate(ca.IC_MERGER, 20000, 8) if case == "merger" else _cluster(2)
+    sim = _sim(b, NO_FALLBACK, softening=0.01, theta=theta, dt=1e-4, precision="fp32_list")
+    sim.advance(1)
+    total, a1 = sim.stats().interactions, sim.accelerations()
+    sim.compute_forces()
+    counts = sim.interaction_counts()
+    assert sim.stats().interactions == int(counts.sum()) == total
+    assert np.array_equal(sim.accelerations(), a1)
+    sim.close()
+
+
 def test_list_walk_all_dead_and_empty():
     b = _uniform(100, 6)
     b["mass"][:] = 0.0
