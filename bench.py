@@ -480,6 +480,11 @@ def main():
                                                     100, "fp64_strict")
         results["config1_100000_merger"] = config_line(ca, ca.IC_MERGER, 100000, 2, 0.5, 0.01,
                                                        1e-3, 1000, "fp64_strict")
+        # config 4's forward substep (the 4M-body merger, theta 0.7, eps 0.003) with
+        # each FP32 walk: the per-body one is the adjoint's, the list walk the fastest
+        for prec in ("fp32_list", "fp32"):
+            results[f"config4_4000000_forward_{prec}"] = config_line(
+                ca, ca.IC_MERGER, 4_000_000, 5, 0.7, 0.003, 5e-4, 50, prec)
     walk_flop = head["interactions"] * FLOP_PER_INTERACTION
     achieved = walk_flop / (head["walk_ms"] * 1e-3) / 1e12
     traffic = load_traffic()
