@@ -115,18 +115,19 @@ def per_step_accuracy(ca, bodies, idx, theta, steps, precision="fp32_list"):
             "fp32_vs_direct_max_max": float(np.max(mx))}
 
 
-def config_line(ca, recipe, n, seed, theta, eps, dt, steps, precision):
+def config_line(ca, recipe, n, seed, theta, eps, dt, steps, precision, energy=True):
     """Substeps/s, interactions/s and the walk's roofline fraction of one of the
-    smaller configurations, with its energy and momentum drift over the run."""
+    smaller configurations, with its energy and momentum drift over the run (the
+    energy's potential is a direct sum: energy=False skips it at large n)."""
     sim = ca.Simulation(ca.generate(recipe, n, seed), softening=eps, theta=theta, dt=dt,
                         precision=precision)
-    ke0, pe0, p0 = sim.energy()
+    ke0, pe0, p0 = sim.energy() if energy else (0.0, 0.0, np.zeros(3))
     sim.set_timing(True)
     t0 = time.perf_counter()
     sim.advance(steps)
     t1 = time.perf_counter()
     st = sim.stats()
-    ke1, pe1, p1 = sim.energy()
+    ke1, pe1, p1 = sim.energy() if energy else (0.0, 0.0, np.zeros(3))
     sim.close()
     walk_ms = st.ms_walk / max(st.timed_walks, 1)
     peak = FP32_PEAK_TFLOPS if precision.startswith("fp32") else FP64_PEAK_TFLOPS
@@ -137,8 +138,8 @@ def config_line(ca, recipe, n, seed, theta, eps, dt, steps, precision):
             "interactions": int(st.interactions),
             "interactions_per_s": st.interactions / (walk_ms * 1e-3),
             "walk_tflops": tflops, "walk_valu_frac": tflops / peak,
-            "energy_drift": abs((ke1 + pe1 - ke0 - pe0) / (ke0 + pe0)),
-            "momentum_drift": float(np.linalg.norm(p1 - p0))}
+            **({"energy_drift": abs((ke1 + pe1 - ke0 - pe0) / (ke0 + pe0)),
+                "momentum_drift": float(np.linalg.norm(p1 - p0))} if energy else {})}
 
 
 def cpu_baseline(ca, bodies, theta):
@@ -402,6 +403,11 @@ def load_traffic():
         return json.load(f)
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -458,6 +464,7 @@ def main():
     head = None
     for th in thetas:
         for prec in precisions:
+            progress(f"theta {th} {prec}")
             headline = (th == 0.5 and prec == "fp32_list")
             sim, r = timed_run(ca, bodies, th, prec, args.steps, args.warmup,
                                energy=(not args.no_energy and th == 0.5 and prec != "fp64"))
@@ -472,10 +479,12 @@ def main():
         s0.close()
 
     if not args.no_per_step:
+        progress("per-step accuracy")
         results["per_step"] = {f"theta{th}_{p}": per_step_accuracy(ca, bodies, idx, th,
                                                                     args.per_step_steps, p)
                                for th in thetas for p in ("fp32_list", "fp32")}
     if not args.headline_only and not args.no_configs:
+        progress("configurations 0, 1 and 4 (forward)")
         results["config0_20001_disk"] = config_line(ca, ca.IC_DISK, 20001, 1, 0.5, 0.01, 1e-3,
                                                     100, "fp64_strict")
         results["config1_100000_merger"] = config_line(ca, ca.IC_MERGER, 100000, 2, 0.5, 0.01,
@@ -484,7 +493,7 @@ def main():
         # each FP32 walk: the per-body one is the adjoint's, the list walk the fastest
         for prec in ("fp32_list", "fp32"):
             results[f"config4_4000000_forward_{prec}"] = config_line(
-                ca, ca.IC_MERGER, 4_000_000, 5, 0.7, 0.003, 5e-4, 50, prec)
+                ca, ca.IC_MERGER, 4_000_000, 5, 0.7, 0.003, 5e-4, 50, prec, energy=False)
     walk_flop = head["interactions"] * FLOP_PER_INTERACTION
     achieved = walk_flop / (head["walk_ms"] * 1e-3) / 1e12
     traffic = load_traffic()
@@ -550,20 +559,25 @@ def main():
     acc = results["theta0.5_accuracy_step0"]
     line["accuracy_theta0.5"] = {k: float(f"{v:.3e}") for k, v in acc.items()}
     if not args.no_adjoint:
+        progress("adjoint")
         adj = adjoint_runs(ca, args.adjoint_bodies, args.adjoint_steps, everies)
         line["adjoint"] = _rounded(adj)
         big = adj[f"config4_{args.adjoint_bodies}_fp32_theta0.7"]
         line["adjoint_substeps_per_s"] = round(big[f"K{everies[0]}"]["adjoint_substeps_per_s"], 3)
     if not args.no_neighbors:
+        progress("neighbours")
         line["neighbors"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
                              for k, v in neighbor_runs(ca, bodies).items()}
     if not args.no_sph:
+        progress("SPH")
         line["sph"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
                        for k, v in sph_runs(ca, bodies, cpu=not args.no_cpu).items()}
     if not args.no_quasar:
+        progress("quasar")
         line["quasar"] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v)
                           for k, v in quasar_runs(ca, cpu=not args.no_cpu).items()}
     if not args.no_cpu:
+        progress("CPU baseline")
         cb = cpu_baseline(ca, bodies, 0.5)
         line["cpu_baseline"] = {k: (round(v, 5) if isinstance(v, float) else v)
                                 for k, v in cb.items()}
