@@ -142,7 +142,9 @@ struct csg_ctx {
     int walk_block_waves = 1;  // waves per block of the ordered walks
     int wave_split = 2;        // sub-ranges per XCD range of the walk's dispatch order
                                // (1M merger: 1.74 / 1.68 / 1.72 / 1.71 ms walk for 1-4)
-    int list_max = 4096;       // list entries after which a list walk's wave falls back
+    int list_max = 6144;       // list entries after which a list walk's wave falls back
+                               // (1M merger over 100 substeps: 1.22 / 1.17 / 1.16 / 1.21 ms
+                               // walk for 3072 / 4096 / 6144 / 8192)
     int list_block_waves = 2;  // list walk: independent wavefronts per workgroup (1, 2, 4;
                                // 1M merger at theta 0.5: 1.35 / 1.29 / 1.31 ms walk)
     int *wl_fall = nullptr;    // list walks: waves that fell back (device counter)
