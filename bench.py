@@ -435,6 +435,10 @@ def main():
     ap.add_argument("--no-quasar", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to scale the traffic counters")
+    ap.add_argument("--walks", default="",
+                    help="only time these walk precisions (comma list, e.g. fp64,fp64_strict) "
+                         "on the 1M merger at --thetas, and print their figures as one JSON "
+                         "object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -455,6 +459,16 @@ def main():
     rng = np.random.default_rng(12345)
     idx = np.sort(rng.choice(N_BODIES, SUBSET, replace=False)).astype(np.int32)
     thetas = [float(t) for t in args.thetas.split(",")]
+    if args.walks:
+        walks = {}
+        for th in thetas:
+            for prec in args.walks.split(","):
+                progress(f"theta {th} {prec}")
+                sim, walks[f"theta{th}_{prec}"] = timed_run(ca, bodies, th, prec, args.steps,
+                                                            args.warmup)
+                sim.close()
+        print(json.dumps({"walks": _rounded(walks)}))
+        return
     if 0.5 not in thetas:
         thetas.append(0.5)
     precisions = (["fp32_list"] if args.headline_only
