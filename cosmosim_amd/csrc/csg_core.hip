@@ -142,9 +142,9 @@ struct csg_ctx {
     int walk_block_waves = 1;  // waves per block of the ordered walks
     int wave_split = 2;        // sub-ranges per XCD range of the walk's dispatch order
                                // (1M merger: 1.74 / 1.68 / 1.72 / 1.71 ms walk for 1-4)
-    int list_max = 6144;       // list entries after which a list walk's wave falls back
-                               // (1M merger over 100 substeps: 1.22 / 1.17 / 1.16 / 1.21 ms
-                               // walk for 3072 / 4096 / 6144 / 8192)
+    int list_max = 6144;       // list entries after which a list walk's wave falls back, up
+                               // to theta 0.5 (1M merger over 100 substeps: 1.22 / 1.17 /
+                               // 1.16 / 1.21 ms walk for 3072 / 4096 / 6144 / 8192)
     int list_block_waves = 2;  // list walk: independent wavefronts per workgroup (1, 2, 4;
                                // 1M merger at theta 0.5: 1.35 / 1.29 / 1.31 ms walk)
     int *wl_fall = nullptr;    // list walks: waves that fell back (device counter)
@@ -845,15 +845,20 @@ static void launch_walk32w(csg_ctx *c, const int *sidx, double G, double eps2, d
 }
 
 // cnt: 0 no counts, 1 the total only (from the list lengths), 2 per-body counts
-// and the total; selfx: the own leaf excluded by index
+// and the total; selfx: the own leaf excluded by index.  The fallback limit is
+// list_max up to theta 0.5 and shrinks as theta^-2 above it: a wide opening angle
+// shortens the union traversal that a long list competes with (1M merger, walk
+// at theta 0.7: 0.64 ms with a limit of 3135, 0.70 ms with 6144).
 template <int WPB>
 static void launch_walk32l(csg_ctx *c, int cnt, bool selfx, const int *sidx, const int *spos,
-                           double G, double eps2, int grid, WaveOrder wo) {
+                           double G, double eps2, double th2, int grid, WaveOrder wo) {
     int *ic = cnt == 2 ? c->icount : nullptr;
+    const double lm = (double)c->list_max * fmin(1.0, 0.25 / fmax(th2, 1e-12));
+    const int lmax = lm >= 2147483647.0 ? 2147483647 : (int)lm;
 #define CSG_WL(C, S)                                                                         \
     k_walk32l<C, S, WPB><<<grid, 64 * WPB, 0, c->stream>>>(c->nodes32, c->bp, sidx, c->x, c->y, \
         c->z, c->ax, c->ay, c->az, ic, c->itotal, c->wl_fall, c->list_sticky ? c->wmode : nullptr, \
-        c->list_sticky, spos, c->lcp, G, eps2, c->list_max, wo)
+        c->list_sticky, spos, c->lcp, G, eps2, lmax, wo)
     if (selfx) {
         if (cnt == 2) CSG_WL(2, true); else if (cnt == 1) CSG_WL(1, true); else CSG_WL(0, true);
     } else {
@@ -944,9 +949,9 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
         const int cnt = lane_counts ? 2 : (count_total ? (own_listed ? 1 : 2) : 0);
         const int *spos = hil ? c->hpos : nullptr;
         switch (wpb) {
-        case 2: launch_walk32l<2>(c, cnt, selfx, sidx_l, spos, G, eps2, grid, wo); break;
-        case 4: launch_walk32l<4>(c, cnt, selfx, sidx_l, spos, G, eps2, grid, wo); break;
-        default: launch_walk32l<1>(c, cnt, selfx, sidx_l, spos, G, eps2, grid, wo); break;
+        case 2: launch_walk32l<2>(c, cnt, selfx, sidx_l, spos, G, eps2, th2, grid, wo); break;
+        case 4: launch_walk32l<4>(c, cnt, selfx, sidx_l, spos, G, eps2, th2, grid, wo); break;
+        default: launch_walk32l<1>(c, cnt, selfx, sidx_l, spos, G, eps2, th2, grid, wo); break;
         }
         CSG_LAUNCHED();
         c->counts_valid = lane_counts ? 1 : 0;
