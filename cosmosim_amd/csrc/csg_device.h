@@ -2502,10 +2502,12 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
     int selfs = 0;                         // own-leaf entries seen (FULL)
     long long nent = 0;                    // list entries summed
     int rounds = 0;
-    // sum the buffered list lst[0, nl) into every lane's block sums
-    auto flush = [&](int nl) {
-        const int n8 = (nl + 7) & ~7;
-        if (lane < n8 - nl) {   // padding: massless entries far away add exactly 0
+    // sum the buffered list lst[0, nl) into every lane's block sums, in whole blocks of
+    // 8 entries; the last call pads the rest with massless entries, the others move it
+    // to the front of the list and return its length
+    auto flush = [&](int nl, bool last) -> int {
+        const int n8 = last ? (nl + 7) & ~7 : nl & ~7;
+        if (last && lane < n8 - nl) {   // padding: massless entries far away add exactly 0
             lst[nl + lane] = make_float4(1e18f, 0.f, 0.f, 0.f);
             if constexpr (FULL || SELF) lbd[nl + lane] = -1;
         }
@@ -2533,8 +2535,20 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
                 bx = by = bz = 0.f;
             }
         }
-        nent += nl;
+        const int rest = last ? 0 : nl - n8;   // below 8 (n8 >= 64 here): no overlap
+        nent += nl - rest;
+        float4 e{};
+        int bd = -1;
+        if (lane < rest) {
+            e = lst[n8 + lane];
+            if constexpr (FULL || SELF) bd = lbd[n8 + lane];
+        }
         wave_lds_sync();   // every lane has read the list before it is refilled
+        if (lane < rest) {
+            lst[lane] = e;
+            if constexpr (FULL || SELF) lbd[lane] = bd;
+        }
+        return rest;
     };
 
     // walks left to take the union traversal
@@ -2573,10 +2587,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk32l(
         top += __popcll(S);
         if (opn) stk[top + lane_rank(O)] = make_int2(ch.x + 1, nd.skip);
         top += __popcll(O);
-        if (nl > WL_LCAP - 64 || top == 0) {
-            flush(nl);
-            nl = 0;
-        }
+        if (nl > WL_LCAP - 64 || top == 0) nl = flush(nl, top == 0);
         // a round pops at most 64 chains and pushes at most 128
         if (top > WL_SCAP - 64 || nent > lmax) {
             fell = true;
