@@ -142,20 +142,20 @@ struct csg_ctx {
     int walk_block_waves = 1;  // waves per block of the ordered walks
     int wave_split = 2;        // sub-ranges per XCD range of the walk's dispatch order
                                // (1M merger: 1.74 / 1.68 / 1.72 / 1.71 ms walk for 1-4)
-    int list_max = 6144;       // list entries after which a list walk's wave falls back, up
-                               // to theta 0.5 (1M merger over 100 substeps: 1.22 / 1.17 /
-                               // 1.16 / 1.21 ms walk for 3072 / 4096 / 6144 / 8192)
+    int list_max = 4096;       // list entries after which a list walk's wave falls back, at
+                               // theta 0.5 (scaled by 0.25 / theta^2 in launch_walk32l)
     int list_block_waves = 2;  // list walk: independent wavefronts per workgroup (1, 2, 4;
                                // 1M merger at theta 0.5: 1.35 / 1.29 / 1.31 ms walk)
     int *wl_fall = nullptr;    // list walks: waves that fell back (device counter)
     int *wmode = nullptr;      // list walks: per wave, walks left on the union traversal
+    int *wcost_s = nullptr;    // the costs a dispatch order reads (k_cost_near of wcost)
     int list_wave_split = 1;   // sub-ranges per XCD range of the list walk's dispatch order
                                // (1M merger, theta 0.5: 1.27 / 1.33 / 1.40 ms for 1 / 2 / 3)
-    int list_sticky = 64;      // walks a wave that fell back stays on the union traversal
+    int list_sticky = 256;     // walks a wave that fell back stays on the union traversal
                                // (0: every walk tries the list; results then depend only
                                // on the state, not on the walks before).  1M merger over
-                               // 100 substeps: 1.25 / 1.19 / 1.17 / 1.16 ms walk for
-                               // 8 / 16 / 32 / 64
+                               // 100 substeps, limit 4096: 0.947 / 0.942 / 0.939 ms walk
+                               // for 64 / 256 / 100000
 };
 
 #define CSG_CHECK(expr)                                                                      \
@@ -206,7 +206,7 @@ static void free_buffers(csg_ctx *c) {
     dfree(c->idx); dfree(c->idx2); dfree(c->widx); dfree(c->hpos); dfree(c->nlast); dfree(c->rs_hist); dfree(c->rs_tot); dfree(c->rs_ctl); dfree(c->rs_status); dfree(c->xs); dfree(c->ys); dfree(c->zs); dfree(c->ms);
     dfree(c->lcp); dfree(c->lmin1); dfree(c->lmin2); dfree(c->cnt); dfree(c->nbase); dfree(c->scan_part);
     dfree(c->pref); dfree(c->dd_part); dfree(c->tpref); dfree(c->nodes); dfree(c->nodes32); dfree(c->icount);
-    dfree(c->wcost); dfree(c->worder); dfree(c->xyzm); dfree(c->wmode);
+    dfree(c->wcost); dfree(c->worder); dfree(c->xyzm); dfree(c->wmode); dfree(c->wcost_s);
     c->wcost_n = 0;
     c->cap = 0;
     c->node_cap = 0;
@@ -252,7 +252,7 @@ static int alloc_buffers(csg_ctx *c, int cap) {
               dalloc(c->lmin2, lmin2_bytes(C)) && dalloc(c->cnt, C1) && dalloc(c->nbase, C1) &&
               dalloc(c->scan_part, (size_t)nb) && dalloc(c->pref, 4 * C1) &&
               dalloc(c->dd_part, (size_t)4 * nb) && dalloc(c->tpref, (size_t)4 * CSG_TPB * nb) && dalloc(c->nodes, 8 * C + 4096) && dalloc(c->nodes32, 8 * C + 4096) && dalloc(c->nlast, 8 * C + 4096) &&
-              dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->wmode, C / 64 + 1) && dalloc(c->worder, 2 * (size_t)(C / 64 + 1) + 4096) &&
+              dalloc(c->icount, C) && dalloc(c->wcost, C / 64 + 1) && dalloc(c->wcost_s, C / 64 + 1) && dalloc(c->wmode, C / 64 + 1) && dalloc(c->worder, 2 * (size_t)(C / 64 + 1) + 4096) &&
               dalloc(c->xyzm, C) && dalloc(c->rs_ctl, 1) &&
               dalloc(c->rs_status, (size_t)256 * blocks_for((long long)C, RS_TILE));
     if (!ok) {
@@ -688,6 +688,14 @@ static bool walk_order_params(const csg_ctx *c, int precision, bool taped, int n
     return true;
 }
 
+// The costs the next dispatch order reads: each wave's last cost with its
+// neighbours' (k_cost_near), or the plain costs where there is no buffer for them.
+static const int *order_costs(csg_ctx *c, int nw) {
+    if (!c->wcost_s) return c->wcost;
+    k_cost_near<<<blocks_for(nw, CSG_TPB), CSG_TPB, 0, c->stream>>>(c->wcost, nw, c->wcost_s);
+    return c->wcost_s;
+}
+
 // bbox (may be null): also leave the drifted positions' bounding-box partials
 // in c->bbox_part and their block count in *bbox, for the build that follows.
 // next_prec >= 0 (with bbox): the precision of the untaped walk that follows the
@@ -702,9 +710,10 @@ static int launch_kick_drift(csg_ctx *c, double dt, int kicks, int *bbox = nullp
     if (bbox && next_prec >= 0 && walk_order_params(c, next_prec, false, nw, &gw, &capg, &split)) {
         const int nkd = min(blocks_for(c->n, 1024), CSG_BBOX_BLOCKS);
         *bbox = nkd;
+        const int *oc = order_costs(c, nw);
         k_kick_drift_order<<<WO_XCDS + nkd, 1024, 0, c->stream>>>(c->x, c->y, c->z, c->vx,
             c->vy, c->vz, c->ax, c->ay, c->az, c->m, c->n, dt, kicks, c->bbox_part, c->icount,
-            c->itotal, c->wcost, nw, gw, split, capg, c->worder);
+            c->itotal, oc, nw, gw, split, capg, c->worder);
         c->order_ready = 1;
         c->order_gw = gw;
         c->order_split = split;
@@ -846,14 +855,16 @@ static void launch_walk32w(csg_ctx *c, const int *sidx, double G, double eps2, d
 
 // cnt: 0 no counts, 1 the total only (from the list lengths), 2 per-body counts
 // and the total; selfx: the own leaf excluded by index.  The fallback limit is
-// list_max up to theta 0.5 and shrinks as theta^-2 above it: a wide opening angle
-// shortens the union traversal that a long list competes with (1M merger, walk
-// at theta 0.7: 0.64 ms with a limit of 3135, 0.70 ms with 6144).
+// list_max at theta 0.5 and scales as theta^-2: the lists grow as the opening
+// angle shrinks, and so does the union traversal a long list competes with (1M
+// merger over 100 substeps, walk ms for limits 3072 / 4096 / 6144: theta 0.4
+// 2.17 / 1.62 / 1.38, theta 0.5 1.15 / 0.94 / 0.99; theta 0.7 with 2090 / 3135:
+// 0.516 / 0.518).
 template <int WPB>
 static void launch_walk32l(csg_ctx *c, int cnt, bool selfx, const int *sidx, const int *spos,
                            double G, double eps2, double th2, int grid, WaveOrder wo) {
     int *ic = cnt == 2 ? c->icount : nullptr;
-    const double lm = (double)c->list_max * fmin(1.0, 0.25 / fmax(th2, 1e-12));
+    const double lm = (double)c->list_max * (0.25 / fmax(th2, 1e-12));
     const int lmax = lm >= 2147483647.0 ? 2147483647 : (int)lm;
 #define CSG_WL(C, S)                                                                         \
     k_walk32l<C, S, WPB><<<grid, 64 * WPB, 0, c->stream>>>(c->nodes32, c->bp, sidx, c->x, c->y, \
@@ -927,9 +938,10 @@ static int launch_walk(csg_ctx *c, int precision, double G, double softening, do
     int split = 1;
     if (walk_order_params(c, precision, tape.rec != nullptr, nw, &gw, &capg, &split)) {
         if (!(pre && c->order_gw == gw && c->order_capg == capg && c->order_nw == nw &&
-              c->order_split == split))
-            k_wave_order<<<WO_XCDS, 1024, 0, st>>>(c->wcost, nw, gw, split, capg,
-                                                   c->worder);
+              c->order_split == split)) {
+            const int *oc = order_costs(c, nw);
+            k_wave_order<<<WO_XCDS, 1024, 0, st>>>(oc, nw, gw, split, capg, c->worder);
+        }
         wo.order = c->worder;
         wo.slots = WO_XCDS * capg * gw;
         grid = WO_XCDS * capg;

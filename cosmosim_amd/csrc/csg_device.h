@@ -1998,7 +1998,7 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
 }
 
 // Dispatch order of the walks, from each wave's node-step count in the previous
-// walk (wo_cost: with its neighbours').  The walk runs blocks of gw waves (gw consecutive waves: 64 gw sorted,
+// walk (with its neighbours', k_cost_near).  The walk runs blocks of gw waves (gw consecutive waves: 64 gw sorted,
 // spatially adjacent bodies that share a CU and its scalar cache); block b of
 // the grid runs on XCD b % 8 (workgroups are dealt out to the XCDs
 // round-robin), and wave slot s of the grid walks wave order[s].  The blocks'
@@ -2020,20 +2020,27 @@ __global__ __launch_bounds__(CSG_TPB) void k_walk32(
 #define WO_REG 16
 #define WO_MAX_SPLIT 4
 #define WO_CAP(ng) ((((5 * (ng)) / 32 + 8) + 3) & ~3)
-// A wave's expected cost: the largest of its own and its neighbours' (two on each
-// side) costs in the previous walk.  Between two walks the bodies move and the
-// waves' membership shifts along the wave order, so an expensive (wide) group can
-// land a wave or two away from where it was; taking the neighbours' costs keeps
-// such a wave near the front of the dispatch (1M merger, list walk at theta 0.5,
-// 100 substeps: 0.99 ms with two neighbours a side, 1.01 with one, 1.12 ms with
-// each wave's own cost).
+__device__ __forceinline__ int wo_cost(const int *__restrict__ cost, int w) {
+    return min(max(cost[w], 0), 1 << 24);
+}
+
+// A wave's expected cost for the next dispatch order: the largest of its own and
+// its neighbours' (WO_NEAR on each side) costs in the previous walk.  Between two
+// walks the bodies move and the waves' membership shifts along the wave order, so
+// an expensive (wide) group can land a wave or two away from where it was; taking
+// the neighbours' costs keeps such a wave near the front of the dispatch (1M
+// merger, list walk at theta 0.5, 100 substeps: 0.99 ms with two neighbours a
+// side, 1.01 with one, 1.12 ms with each wave's own cost).
 #define WO_NEAR 2
-__device__ __forceinline__ int wo_cost(const int *__restrict__ cost, int w, int nw) {
+__global__ __launch_bounds__(CSG_TPB) void k_cost_near(const int *__restrict__ cost, int nw,
+                                                       int *__restrict__ out) {
+    const int w = blockIdx.x * CSG_TPB + threadIdx.x;
+    if (w >= nw) return;
     int c = 0;
 #pragma unroll
     for (int k = -WO_NEAR; k <= WO_NEAR; ++k)
-        if (w + k >= 0 && w + k < nw) c = max(c, min(max(cost[w + k], 0), 1 << 24));
-    return c;
+        if (w + k >= 0 && w + k < nw) c = max(c, wo_cost(cost, w + k));
+    out[w] = c;
 }
 __device__ __forceinline__ void wave_order_block(int r, const int *__restrict__ cost, int nw,
                                                  int gw, int split, int capg,
@@ -2051,7 +2058,7 @@ __device__ __forceinline__ void wave_order_block(int r, const int *__restrict__ 
         int c = 0;
         for (int k = 0; k < gw; ++k) {
             const int w = g * gw + k;
-            if (w < nw) c += wo_cost(cost, w, nw);
+            if (w < nw) c += wo_cost(cost, w);
         }
         return c;
     };
@@ -2065,7 +2072,7 @@ __device__ __forceinline__ void wave_order_block(int r, const int *__restrict__ 
     long long s = 0;
     if (regs) {
 #pragma unroll
-        for (int k = 0; k < WO_REG; ++k) cv[k] = c0 + k < c1 ? wo_cost(cost, c0 + k, nw) : 0;
+        for (int k = 0; k < WO_REG; ++k) cv[k] = c0 + k < c1 ? wo_cost(cost, c0 + k) : 0;
 #pragma unroll
         for (int k = 0; k < WO_REG; ++k) s += cv[k];
     } else {
