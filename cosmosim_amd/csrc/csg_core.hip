@@ -758,15 +758,16 @@ static int launch_build(csg_ctx *c, bool with64, int bbox, int order) {
     CSG_LAUNCHED();
     if (n > 0) {
         const int nt = blocks_for(n, RS_TILE);
-        k_keys<<<nt, RS_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bp, c->key, c->xyzm,
+        // the keys start in the buffer that makes the last pass end in key / idx
+        constexpr bool odd = CSG_SORT_PASSES % 2 != 0;
+        uint64_t *kin = odd ? c->key2 : c->key, *kout = odd ? c->key : c->key2;
+        int *vin = odd ? c->idx2 : c->idx, *vout = odd ? c->idx : c->idx2;
+        k_keys<<<nt, RS_TPB, 0, st>>>(c->x, c->y, c->z, c->m, n, c->bp, kin, c->xyzm,
                                       c->rs_ctl, c->scan_part, blocks_for(n + 1, CSG_TILE));
         if (c->rs_epoch > 0x7fffff00u) {   // epochs are 31 bits: start the words over
             clear_words(c->rs_status, (size_t)256 * blocks_for(c->cap, RS_TILE), st);
             c->rs_epoch = 0;
         }
-        uint64_t *kin = c->key, *kout = c->key2;
-        int *vin = c->idx, *vout = c->idx2;
-        static_assert(CSG_SORT_PASSES % 2 == 0, "the sorted pairs end in key / idx");
         for (int pass = 0; pass < CSG_SORT_PASSES; ++pass) {
             const unsigned ep = ++c->rs_epoch;
             if (pass == 0)
@@ -779,7 +780,7 @@ static int launch_build(csg_ctx *c, bool with64, int bbox, int order) {
             int *tv = vin; vin = vout; vout = tv;
         }
         CSG_LAUNCHED();
-        // an even number of passes: the sorted pairs are back in key / idx
+        // the sorted pairs end in key / idx
         k_runs<<<blocks_for(n, CSG_TPB), CSG_TPB, 0, st>>>(c->key, c->idx, c->x, c->y, c->z, c->bp, c->ext,
                                         c->idx2, c->ext_tmp);
         CSG_LAUNCHED();

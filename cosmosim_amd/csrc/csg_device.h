@@ -26,11 +26,13 @@
 #define CSG_TPB 256
 #define CSG_ITEMS 8
 #define CSG_TILE (CSG_TPB * CSG_ITEMS)
-// Levels in the sorted key.  Bodies sharing all of them (rare: leaves sit at
-// depths 10-12 at a million bodies) form collision runs resolved with their
-// deeper levels.  3 * CSG_KEY_LEVELS bits, sorted in CSG_SORT_PASSES 8-bit passes.
-// A live key is never all ones in those bits (that cell lies in the 1% margin
-// of the root cube), so the all-ones key of dead bodies sorts last.
+// Levels in the sorted key.  Bodies sharing all of them form collision runs
+// resolved with their deeper levels (k_runs; 8 runs on the 1M-body merger, whose
+// leaves sit at depths 10-18).  3 * CSG_KEY_LEVELS bits, sorted in CSG_SORT_PASSES
+// 8-bit passes.  13 levels take one pass fewer, but leave ~4,000 runs to resolve:
+// the 1M merger's build took 0.366 against 0.348 ms.  A live key is never all ones
+// in those bits (that cell lies in the 1% margin of the root cube), so the
+// all-ones key of dead bodies sorts last.
 #define CSG_KEY_LEVELS 16
 // Node records past the last node: padding the wave-mask walks may step onto
 // (they test for the end once per round of eight FP32 or four FP64 steps);
@@ -814,11 +816,51 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_sweep(
 // Groups of sorted bodies whose keys are equal (all CSG_KEY_LEVELS levels):
 // each member gets its levels CSG_KEY_LEVELS..63, then the members are re-ranked
 // by (those levels, body index); ties keep index order.  Block j scans slice j
-// of the sorted keys in chunks of CSG_TPB for the first members of runs and
-// fixes each run it finds with the whole block (a run may extend past the
-// slice; it belongs to the slice of its first member).  Launched with one block
-// per CSG_TPB bodies, so a slice is one chunk and blocks without a run start end
-// after one look at their keys.
+// of the sorted keys in chunks of CSG_TPB for the first members of runs (a run
+// may extend past the slice; it belongs to the slice of its first member).  The
+// thread that finds a run of at most CSG_RUN_SMALL members fixes it alone
+// (fix_run_small); longer runs are fixed by the whole block.  Launched with one
+// block per CSG_TPB bodies, so a slice is one chunk and blocks without a long run
+// end after one look at their keys.
+#define CSG_RUN_SMALL 16
+
+// One thread's fix of the run [b, e) of sorted bodies (the thread owns the range).
+__device__ void fix_run_small(int b, int e, int *__restrict__ idx, const double *__restrict__ x,
+                              const double *__restrict__ y, const double *__restrict__ z,
+                              const double *root, uint64_t *__restrict__ ext,
+                              int *__restrict__ tmp_idx, uint64_t *__restrict__ tmp_ext) {
+    for (int p = b; p < e; ++p) {
+        const int i = idx[p];
+        path_ext(x[i], y[i], z[i], root, ext + 3 * (size_t)p);
+    }
+    bool differ = false;
+    for (int p = b + 1; p < e; ++p) {
+        const uint64_t *u = ext + 3 * (size_t)p, *f = ext + 3 * (size_t)b;
+        differ |= (u[0] != f[0]) | (u[1] != f[1]) | (u[2] != f[2]);
+    }
+    if (!differ) return;   // sorted by body index already (the sort is stable)
+    for (int p = b; p < e; ++p) {
+        const uint64_t *u = ext + 3 * (size_t)p;
+        const int ip = idx[p];
+        int rank = 0;
+        for (int t = b; t < e; ++t) {
+            const uint64_t *f = ext + 3 * (size_t)t;
+            bool less;
+            if (f[0] != u[0]) less = f[0] < u[0];
+            else if (f[1] != u[1]) less = f[1] < u[1];
+            else if (f[2] != u[2]) less = f[2] < u[2];
+            else less = idx[t] < ip;
+            rank += less ? 1 : 0;
+        }
+        tmp_idx[b + rank] = ip;
+        for (int k = 0; k < 3; ++k) tmp_ext[3 * (size_t)(b + rank) + k] = u[k];
+    }
+    for (int p = b; p < e; ++p) {
+        idx[p] = tmp_idx[p];
+        for (int k = 0; k < 3; ++k) ext[3 * (size_t)p + k] = tmp_ext[3 * (size_t)p + k];
+    }
+}
+
 __global__ __launch_bounds__(CSG_TPB) void k_runs(
     const uint64_t *__restrict__ key, int *__restrict__ idx, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, BuildParams *__restrict__ bp,
@@ -836,7 +878,16 @@ __global__ __launch_bounds__(CSG_TPB) void k_runs(
         const int b = c0 + threadIdx.x;
         if (b < hi && b + 1 < nl) {
             const uint64_t k = key[b];
-            if (k == key[b + 1] && (b == 0 || key[b - 1] != k)) s_start[atomicAdd(&s_nstart, 1)] = b;
+            if (k == key[b + 1] && (b == 0 || key[b - 1] != k)) {
+                int e = b + 2;   // the run's end, followed while it stays short
+                while (e < nl && e - b <= CSG_RUN_SMALL && key[e] == k) ++e;
+                if (e - b <= CSG_RUN_SMALL) {
+                    fix_run_small(b, e, idx, x, y, z, root, ext, tmp_idx, tmp_ext);
+                    atomicAdd(&bp->n_runs, 1);
+                } else {
+                    s_start[atomicAdd(&s_nstart, 1)] = b;
+                }
+            }
         }
         __syncthreads();
         const int ns = s_nstart;
