@@ -17,6 +17,7 @@ and sums the list for each body.  Checks:
     accelerations and counts;
   * on the 1M-body merger: at least as accurate as the per-body FP64 walk
     against the FP64 direct sum on a 4096-body subset, at theta 0.4, 0.5, 0.7;
+  * the dispatch order changes no bit of a substep run (with fallbacks);
   * a device-resident substep run with the list walk tracks the FP64-strict
     trajectory, and the adjoint refuses the list precision.
 """
@@ -235,6 +236,23 @@ def test_fallen_back_waves_stay_on_the_union_walk():
     s.compute_forces()
     assert s.stats().list_fallbacks == nw
     s.close()
+
+
+def test_dispatch_order_changes_no_result():
+    # the dispatch order (longest first from each wave's and its neighbours' last
+    # costs) only schedules the wavefronts: the same substeps without it give the
+    # same bits, and so does a run whose fallbacks start from a small limit
+    b = ca.generate(ca.IC_MERGER, 40000, 6)
+    out = []
+    for env in ({}, {"COSMOSIM_WAVE_ORDER": "0"}):
+        s = _sim(b, {"COSMOSIM_LIST_MAX": "1500", **env}, softening=0.01, theta=0.5,
+                 dt=1e-4, precision="fp32_list")
+        s.set_timing(True)
+        s.advance(4)
+        out.append((s.accelerations(), s.state()[0], s.stats().list_fallbacks))
+        s.close()
+    assert out[0][2] > 0
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
 @pytest.fixture(scope="module")
